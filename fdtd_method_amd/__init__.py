@@ -1,0 +1,12 @@
+"""MI355X (gfx950) FDTD Yee time step behind the reference's FDTD class API.
+
+The compute path is libfdtd_mi355x.so (hand-written HIP kernels, C-ABI in
+include/fdtd_mi355x.h); this package holds its Python bindings and the
+host-side mirror of the reference interface.
+"""
+from .structures import (C, PI, Axis, Component, CurrentParameters, Parameters, Precision,
+                         SelectedFields)
+from .fdtd import FDTD, Field, calibration_copy
+
+__all__ = ["C", "PI", "Axis", "Component", "CurrentParameters", "Parameters", "Precision",
+           "SelectedFields", "FDTD", "Field", "calibration_copy"]

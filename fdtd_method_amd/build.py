@@ -1,0 +1,67 @@
+"""Build recipes for the native parts of fdtd_method_amd.
+
+``build_library()`` compiles the HIP sources under ``csrc/`` for gfx950 into
+``fdtd_method_amd/libfdtd_mi355x.so`` (in-tree, so it travels with the
+source tree).  Every call compiles; nothing is cached.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+LIB_NAME = "libfdtd_mi355x.so"
+LIB_PATH = os.path.join(PKG, LIB_NAME)
+
+HIP_SOURCES = ["fdtd_runtime.hip", "fdtd_batch.hip"]
+
+# -ffp-contract=off keeps every a*b+c unfused, as in the reference's CPU
+# build (no FMA): the FP64 path is then bit-identical to the reference.
+# -packed-fp32-ops keeps the compiler from emitting packed FP32 math.
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc() -> str:
+    for cand in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build_library(verbose: bool = False, extra_flags=None) -> str:
+    """Compile the HIP library for gfx950; returns the .so path."""
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    tmp = LIB_PATH + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, *(extra_flags or []), f"-I{INCLUDE}", f"-I{CSRC}", *srcs,
+           "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr[-8000:]}")
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+def build_oracle(verbose: bool = False) -> None:
+    """Compile the CPU checker (oracle/liboracle.so) and, when the reference
+    checkout exists, oracle/_ref/libfdtd_ref.so."""
+    odir = os.path.join(ROOT, "oracle")
+    targets = ["oracle"]
+    ref = os.environ.get("FDTD_REFERENCE", "/root/reference")
+    if os.path.isdir(os.path.join(ref, "src", "FDTD")):
+        targets.append("ref")
+    cmd = ["make", "-B", "-C", odir, f"REFERENCE={ref}", *targets]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if verbose:
+        print(res.stdout)
+    if res.returncode != 0:
+        raise RuntimeError(f"oracle build failed:\n{res.stdout[-4000:]}\n{res.stderr[-4000:]}")

@@ -1,0 +1,570 @@
+// gfx950 kernels for the Yee time step (B/2, E, B/2 per step) on a periodic
+// grid stored as flat structure-of-arrays, index = i + j*Ni + k*Ni*Nj.
+//
+// Reference semantics (file:line):
+//   B half step      include/FDTD_kokkos/kokkos_functors.h:128-151 (== src/FDTD/FDTD.cpp:99-130)
+//   E full step      include/FDTD_kokkos/kokkos_functors.h:64-90   (Jx->Ex, Jy->Ey, Jz->Ez)
+//   periodic wrap    include/FDTD/FDTD.h:24-31
+//   step order       src/FDTD_kokkos/FDTD_kokkos.cpp:91-104
+//
+// Two kernel families:
+//  * plain_update_B / plain_update_E: one thread per cell, in place, the
+//    reference's three-sweep structure.  This is the FP64 parity path.
+//  * fused_sweep: one sweep per time step.  Consecutive steps run
+//    "... B/2 | B/2, E, B/2 | B/2 ..."; the two adjacent B half steps read
+//    the same E, so their increment d is computed once and applied twice as
+//    (B + d) + d -- bit-identical to the reference's two separate sweeps.
+//    The steady-state sweep therefore maps (E_n, B_{n+1/2}) to
+//    (E_{n+1}, B_{n+3/2}); the final sweep of a run applies d once, giving
+//    (E_{n+1}, B_{n+1}).  A run of n steps is one plain B half step plus n
+//    fused sweeps.
+//
+// All arithmetic keeps the reference's C++ evaluation order and the file is
+// compiled with -ffp-contract=off, so FP64 results are bit-identical to the
+// reference's CPU fields.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fdtd {
+
+// ---------------------------------------------------------------------------
+// Types
+
+template <typename S> struct ComputeOf { using type = S; };
+template <> struct ComputeOf<_Float16> { using type = float; };
+
+// Coefficients of one precision path.  With per-component storage scales
+// s (stored = s * physical) each coefficient absorbs s_target / s_source;
+// with unit scales they are exactly the reference's coefficients
+// (src/FDTD/FDTD.cpp:43-53).
+template <typename C> struct Coefs {
+    // E update: Ex += jx*Jx + ex_bz*(Bz - Bz[j-1]) - ex_by*(By - By[k-1]), ...
+    C ex_bz, ex_by, ey_bx, ey_bz, ez_by, ez_bx;
+    C jx, jy, jz;
+    // B update: Bx += bx_ey*(Ey[k+1] - Ey) - bx_ez*(Ez[j+1] - Ez), ...
+    C bx_ey, bx_ez, by_ez, by_ex, bz_ex, bz_ey;
+};
+
+// A point current source.  Its J component is evaluated in FP64 at
+// t = (n+1)*dt before step n (perf-tests/sample/sample.cpp:74):
+//   kind 0: amp
+//   kind 1: amp * sin(2*pi*(t - t0)/period + phase) * exp(-(t - t0)^2/(2 sigma^2))
+//           (envelope omitted when sigma <= 0)
+struct PointSource {
+    int i, j, k, comp;
+    int kind, pad0;
+    double amp, period, t0, sigma, phase;
+};
+
+constexpr double kPi = 3.14159265358;  // include/Constants.h:14 (truncated)
+
+__device__ __forceinline__ double source_value(const PointSource& s, double t) {
+    if (s.kind == 0) return s.amp;
+    const double tau = t - s.t0;
+    double v = s.amp * sin(2.0 * kPi * tau / s.period + s.phase);
+    if (s.sigma > 0.0) v = v * exp(-(tau * tau) / (2.0 * s.sigma * s.sigma));
+    return v;
+}
+
+// Wrap for an index known to lie in [-1, N]: -1 -> N-1, N -> 0
+// (same mapping as include/FDTD/FDTD.h:24-31).
+__device__ __forceinline__ int wrap(int i, int n) {
+    return i < 0 ? n - 1 : (i == n ? 0 : i);
+}
+
+template <typename S, int V> struct alignas(sizeof(S) * V) Pack { S v[V]; };
+
+template <typename S, typename C, int V>
+__device__ __forceinline__ void load_pack(const S* __restrict__ p, C (&out)[V]) {
+    const Pack<S, V> pk = *reinterpret_cast<const Pack<S, V>*>(p);
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[v] = static_cast<C>(pk.v[v]);
+}
+
+template <typename S, typename C, int V>
+__device__ __forceinline__ void store_pack(S* __restrict__ p, const C (&in)[V]) {
+    Pack<S, V> pk;
+#pragma unroll
+    for (int v = 0; v < V; ++v) pk.v[v] = static_cast<S>(in[v]);
+    *reinterpret_cast<Pack<S, V>*>(p) = pk;
+}
+
+// Field pointers of one grid (current buffers, or in/out buffer pairs).
+template <typename S> struct Fields {
+    S* e[3];
+    S* b[3];
+};
+
+// ---------------------------------------------------------------------------
+// Plain three-sweep kernels (one thread per cell, in place).
+
+template <typename S, typename C>
+__global__ void __launch_bounds__(256)
+plain_update_B(Fields<S> f, Coefs<C> c, int Ni, int Nj, int Nk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= Ni || j >= Nj) return;
+    const int plane = Ni * Nj;
+    const int idx = i + j * Ni + k * plane;
+    const int i_next = wrap(i + 1, Ni) + j * Ni + k * plane;
+    const int j_next = i + wrap(j + 1, Nj) * Ni + k * plane;
+    const int k_next = i + j * Ni + wrap(k + 1, Nk) * plane;
+    const C ex = f.e[0][idx], ey = f.e[1][idx], ez = f.e[2][idx];
+    const C dbx = c.bx_ey * (C(f.e[1][k_next]) - ey) - c.bx_ez * (C(f.e[2][j_next]) - ez);
+    const C dby = c.by_ez * (C(f.e[2][i_next]) - ez) - c.by_ex * (C(f.e[0][k_next]) - ex);
+    const C dbz = c.bz_ex * (C(f.e[0][j_next]) - ex) - c.bz_ey * (C(f.e[1][i_next]) - ey);
+    f.b[0][idx] = static_cast<S>(C(f.b[0][idx]) + dbx);
+    f.b[1][idx] = static_cast<S>(C(f.b[1][idx]) + dby);
+    f.b[2][idx] = static_cast<S>(C(f.b[2][idx]) + dbz);
+}
+
+// Source values of step n: vals[q] = J of source q at t = (n+1)*dt, in FP64.
+// One thread per source; the sweeps then only add precomputed values.
+__global__ void __launch_bounds__(64)
+eval_sources(const PointSource* __restrict__ src, int nsrc, const long long* __restrict__ step_dev,
+             long long step, double dt, double* __restrict__ vals) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nsrc) return;
+    const long long n = step_dev ? *step_dev : step;
+    vals[q] = source_value(src[q], static_cast<double>(n + 1) * dt);
+}
+
+// Adds the J term of every source located in cells (ib..ib+V-1, j, k) of the
+// E components' partial sums t[d][v] = first curl term, as the reference's
+// (cur*J + curl term) -- kokkos_functors.h:81-89.
+template <typename C, int V>
+__device__ __forceinline__ void add_point_sources(C (&t)[3][V], const Coefs<C>& c,
+                                                  const PointSource* __restrict__ src,
+                                                  const double* __restrict__ vals, int nsrc,
+                                                  int ib, int j, int k) {
+    for (int q = 0; q < nsrc; ++q) {
+        const int sk = src[q].k, sj = src[q].j, si = src[q].i;
+        if (sk != k || sj != j || si < ib || si >= ib + V) continue;
+        const int comp = src[q].comp;
+        const double val = vals[q];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if (si != ib + v) continue;
+            if (comp == 0) t[0][v] = c.jx * C(val) + t[0][v];
+            if (comp == 1) t[1][v] = c.jy * C(val) + t[1][v];
+            if (comp == 2) t[2][v] = c.jz * C(val) + t[2][v];
+        }
+    }
+}
+
+// Dense J (type JT, may be null) plus point sources with values `vals`.
+template <typename S, typename C, typename JT>
+__global__ void __launch_bounds__(256)
+plain_update_E(Fields<S> f, const JT* __restrict__ jx, const JT* __restrict__ jy,
+               const JT* __restrict__ jz, const PointSource* __restrict__ src,
+               const double* __restrict__ vals, int nsrc, Coefs<C> c, int Ni, int Nj, int Nk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= Ni || j >= Nj) return;
+    const int plane = Ni * Nj;
+    const int idx = i + j * Ni + k * plane;
+    const int i_pred = wrap(i - 1, Ni) + j * Ni + k * plane;
+    const int j_pred = i + wrap(j - 1, Nj) * Ni + k * plane;
+    const int k_pred = i + j * Ni + wrap(k - 1, Nk) * plane;
+    const C bx = f.b[0][idx], by = f.b[1][idx], bz = f.b[2][idx];
+    C t[3][1];
+    t[0][0] = c.ex_bz * (bz - C(f.b[2][j_pred]));
+    t[1][0] = c.ey_bx * (bx - C(f.b[0][k_pred]));
+    t[2][0] = c.ez_by * (by - C(f.b[1][i_pred]));
+    if (jx != nullptr) {
+        t[0][0] = c.jx * C(jx[idx]) + t[0][0];
+        t[1][0] = c.jy * C(jy[idx]) + t[1][0];
+        t[2][0] = c.jz * C(jz[idx]) + t[2][0];
+    }
+    if (nsrc > 0) add_point_sources<C, 1>(t, c, src, vals, nsrc, i, j, k);
+    const C ex = C(f.e[0][idx]) + (t[0][0] - c.ex_by * (by - C(f.b[1][k_pred])));
+    const C ey = C(f.e[1][idx]) + (t[1][0] - c.ey_bz * (bz - C(f.b[2][i_pred])));
+    const C ez = C(f.e[2][idx]) + (t[2][0] - c.ez_bx * (bx - C(f.b[0][j_pred])));
+    f.e[0][idx] = static_cast<S>(ex);
+    f.e[1][idx] = static_cast<S>(ey);
+    f.e[2][idx] = static_cast<S>(ez);
+}
+
+// ---------------------------------------------------------------------------
+// Fused sweep: LDS-staged 2.5D z-march over full-row tiles.
+//
+// A workgroup owns `ty` consecutive full x-rows [j0, j0+ty) of planes
+// [k0, k1) and marches k upward.  Thread (r, x) holds V consecutive cells
+// i = x*V .. x*V+V-1 of tile row r; row r == ty is the halo row j0+ty, whose
+// E_{n+1} the B update of row ty-1 needs and which the next tile also owns.
+// Per plane kk the workgroup
+//   1. loads E_n(kk), B(kk) (coalesced, V cells per lane) and stages the B
+//      components needed as neighbours in LDS,
+//   2. computes E_{n+1}(kk) (x/y neighbours from LDS or registers, the k-1
+//      neighbour from registers) and stores it if kk is an owned plane,
+//   3. computes B(kk-1) from E_{n+1}(kk-1) (LDS, registers) and
+//      E_{n+1}(kk) (registers) and stores it,
+//   4. stages E_{n+1}(kk) in LDS for the next plane.
+// Inputs and outputs are different buffers (ping-pong), so halo rows and
+// the wrapped planes are recomputed from unmodified data.
+//
+// LDS rows (each Ni values of type C), only what neighbours need:
+//   Bz, Bx: rows j0-1 .. j0+ty-1   (y neighbour j-1; Bz also x neighbour)
+//   By    : rows j0   .. j0+ty     (x neighbour i-1)
+//   Ex    : rows j0+1 .. j0+ty     (y neighbour j+1)
+//   Ey    : rows j0   .. j0+ty-1   (x neighbour i+1)
+//   Ez    : rows j0   .. j0+ty     (x and y neighbours)
+// => (6*TY + 4) * Ni * sizeof(C) bytes for a tile height TY.
+
+struct SweepGeom {
+    int Ni, Nj, Nk;
+    int TY;       // rows per tile
+    int kchunk;   // planes per z chunk
+    int ntile_y;  // ceil(Nj / TY)
+    int ntiles;   // ntile_y * nchunk
+    int per_xcd;  // ceil(ntiles / 8): tiles handled by each XCD
+};
+
+__host__ __device__ inline size_t fused_lds_bytes(int TY, int Ni, int csize) {
+    return static_cast<size_t>(6 * TY + 4) * Ni * csize;
+}
+
+// Largest workgroup of the fused sweep for V cells of size csize per lane:
+// wide lanes hold more registers, so their workgroups are capped lower.
+__host__ __device__ constexpr int fused_max_threads(int V, int csize) {
+    return V * csize >= 32 ? 512 : 1024;
+}
+
+template <typename S, typename C, typename JT>
+struct SweepArgs {
+    Fields<S> in;
+    Fields<S> out;
+    const JT* j[3];                 // dense J or null
+    const PointSource* src;         // point sources (device)
+    const double* vals;             // their J values for this step
+    int nsrc;
+    int pad;
+    Coefs<C> c;
+    SweepGeom g;
+};
+
+template <typename S, typename C, typename JT, int V, bool STEADY>
+__device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int tile_y, int chunk,
+                                           C* __restrict__ lds) {
+    const int Ni = a.g.Ni, Nj = a.g.Nj, Nk = a.g.Nk, TY = a.g.TY;
+    const int lanes = Ni / V;
+    const int j0 = tile_y * TY;
+    const int ty = min(TY, Nj - j0);
+    const int k0 = chunk * a.g.kchunk;
+    const int k1 = min(k0 + a.g.kchunk, Nk);
+    const int tid = threadIdx.x;
+    const int r = tid / lanes;
+    const int ib = (tid - r * lanes) * V;
+    const bool active = r <= ty;
+    const bool own = r < ty;
+    const bool halo = r == ty;
+    const int j = (j0 + r >= Nj) ? j0 + r - Nj : j0 + r;  // halo row wraps
+    const int jm = (j0 == 0) ? Nj - 1 : j0 - 1;
+    const int plane = Ni * Nj;
+    const int im0 = (ib == 0) ? Ni - 1 : ib - 1;        // x neighbour of element 0
+    const int ipl = (ib + V == Ni) ? 0 : ib + V;        // x neighbour of element V-1
+    const int row_off = j * Ni + ib;
+    const int rowm_off = jm * Ni + ib;
+
+    C* const sBz = lds;
+    C* const sBx = sBz + (TY + 1) * Ni;
+    C* const sBy = sBx + (TY + 1) * Ni;
+    C* const sEx = sBy + (TY + 1) * Ni;
+    C* const sEy = sEx + TY * Ni;
+    C* const sEz = sEy + TY * Ni;
+
+    const Coefs<C> c = a.c;
+
+    C eP[3][V];  // E_{n+1} at plane kk-1 (own cells)
+    C bP[3][V];  // B at plane kk-1 (own cells)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int v = 0; v < V; ++v) { eP[d][v] = C(0); bP[d][v] = C(0); }
+
+    if (active) {
+        const int kp = (k0 == 0) ? Nk - 1 : k0 - 1;
+        const int off = kp * plane + row_off;
+        load_pack<S, C, V>(a.in.b[0] + off, bP[0]);
+        load_pack<S, C, V>(a.in.b[1] + off, bP[1]);
+    }
+
+    for (int kk = k0; kk <= k1; ++kk) {
+        const int kw = (kk == Nk) ? 0 : kk;
+        const int off = kw * plane + row_off;
+        C en[3][V], bn[3][V];
+        if (active) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                load_pack<S, C, V>(a.in.b[d] + off, bn[d]);
+                load_pack<S, C, V>(a.in.e[d] + off, en[d]);
+            }
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                if (own) {
+                    sBz[(r + 1) * Ni + ib + v] = bn[2][v];
+                    sBx[(r + 1) * Ni + ib + v] = bn[0][v];
+                }
+                sBy[r * Ni + ib + v] = bn[1][v];
+            }
+            if (halo) {
+                const int offm = kw * plane + rowm_off;
+                C mz[V], mx[V];
+                load_pack<S, C, V>(a.in.b[2] + offm, mz);
+                load_pack<S, C, V>(a.in.b[0] + offm, mx);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    sBz[ib + v] = mz[v];
+                    sBx[ib + v] = mx[v];
+                }
+            }
+        }
+        __syncthreads();
+
+        C e1[3][V];
+        if (active) {
+            // First curl term of each E component, then the J term in front
+            // of it as in the reference: (cur*J + first) - second.
+            C t[3][V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const C by_im = (v > 0) ? bn[1][v - 1] : sBy[r * Ni + im0];
+                t[0][v] = c.ex_bz * (bn[2][v] - sBz[r * Ni + ib + v]);
+                t[1][v] = c.ey_bx * (bn[0][v] - bP[0][v]);
+                t[2][v] = c.ez_by * (bn[1][v] - by_im);
+            }
+            if (a.j[0] != nullptr) {
+                C jv[V];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    load_pack<JT, C, V>(a.j[d] + off, jv);
+                    const C cj = d == 0 ? c.jx : (d == 1 ? c.jy : c.jz);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) t[d][v] = cj * jv[v] + t[d][v];
+                }
+            }
+            if (a.nsrc > 0) add_point_sources<C, V>(t, c, a.src, a.vals, a.nsrc, ib, j, kw);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                e1[0][v] = en[0][v] + (t[0][v] - c.ex_by * (bn[1][v] - bP[1][v]));
+                e1[2][v] = en[2][v] + (t[2][v] - c.ez_bx * (bn[0][v] - sBx[r * Ni + ib + v]));
+                if (own) {
+                    const C bz_im = (v > 0) ? bn[2][v - 1] : sBz[(r + 1) * Ni + im0];
+                    e1[1][v] = en[1][v] + (t[1][v] - c.ey_bz * (bn[2][v] - bz_im));
+                } else {
+                    e1[1][v] = C(0);
+                }
+            }
+            if (own && kk < k1) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) store_pack<S, C, V>(a.out.e[d] + off, e1[d]);
+            }
+            if (own && kk > k0) {
+                const int offb = (kk - 1) * plane + row_off;
+                C nb[3][V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const C ey_ip = (v < V - 1) ? eP[1][v + 1] : sEy[r * Ni + ipl];
+                    const C ez_ip = (v < V - 1) ? eP[2][v + 1] : sEz[r * Ni + ipl];
+                    const C ex_jp = sEx[r * Ni + ib + v];
+                    const C ez_jp = sEz[(r + 1) * Ni + ib + v];
+                    const C dbx = c.bx_ey * (e1[1][v] - eP[1][v]) - c.bx_ez * (ez_jp - eP[2][v]);
+                    const C dby = c.by_ez * (ez_ip - eP[2][v]) - c.by_ex * (e1[0][v] - eP[0][v]);
+                    const C dbz = c.bz_ex * (ex_jp - eP[0][v]) - c.bz_ey * (ey_ip - eP[1][v]);
+                    nb[0][v] = bP[0][v] + dbx;
+                    nb[1][v] = bP[1][v] + dby;
+                    nb[2][v] = bP[2][v] + dbz;
+                    if (STEADY) {
+                        nb[0][v] = nb[0][v] + dbx;
+                        nb[1][v] = nb[1][v] + dby;
+                        nb[2][v] = nb[2][v] + dbz;
+                    }
+                }
+#pragma unroll
+                for (int d = 0; d < 3; ++d) store_pack<S, C, V>(a.out.b[d] + offb, nb[d]);
+            }
+        }
+        __syncthreads();
+
+        if (active && kk < k1) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                if (r >= 1) sEx[(r - 1) * Ni + ib + v] = e1[0][v];
+                if (own) sEy[r * Ni + ib + v] = e1[1][v];
+                sEz[r * Ni + ib + v] = e1[2][v];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                eP[d][v] = e1[d][v];
+                bP[d][v] = bn[d][v];
+            }
+    }
+}
+
+// Tile t of a sweep is handled by the workgroup whose XCD (blockIdx % 8,
+// the hardware's round-robin dispatch) owns the contiguous tile range
+// [xcd*per_xcd, (xcd+1)*per_xcd): tiles sharing halo rows and z-chunk
+// boundary planes then share one L2.
+template <typename S, typename C, typename JT, int V, bool STEADY>
+__global__ void __launch_bounds__(fused_max_threads(V, sizeof(C)))
+fused_sweep(SweepArgs<S, C, JT> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int b = blockIdx.x;
+    const int t = (b & 7) * a.g.per_xcd + (b >> 3);
+    if (t >= a.g.ntiles) return;
+    const int tile_y = t % a.g.ntile_y;
+    const int chunk = t / a.g.ntile_y;
+    fused_tile<S, C, JT, V, STEADY>(a, tile_y, chunk, reinterpret_cast<C*>(lds_raw));
+}
+
+// ---------------------------------------------------------------------------
+// Diagnostics (FP64 accumulation, deterministic two-pass reductions).
+
+constexpr int kReduceThreads = 256;
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
+        if (tid < s) sh[tid] += sh[tid + s];
+        __syncthreads();
+    }
+    const T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = kReduceThreads / 2; s > 0; s >>= 1) {
+        if (tid < s) sh[tid] = fmax(sh[tid], sh[tid + s]);
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Per-block partials of sum(x^2) (scaled by inv_scale) and max|x| for one array.
+template <typename S>
+__global__ void __launch_bounds__(kReduceThreads)
+sumsq_maxabs_partial(const S* __restrict__ x, size_t n, double inv_scale,
+                     double* __restrict__ part) {
+    __shared__ double sh[kReduceThreads];
+    double s = 0.0, m = 0.0;
+    for (size_t i = blockIdx.x * static_cast<size_t>(kReduceThreads) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * kReduceThreads) {
+        const double v = static_cast<double>(x[i]) * inv_scale;
+        s += v * v;
+        m = fmax(m, fabs(v));
+    }
+    s = block_sum(s, sh);
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s;
+        part[2 * blockIdx.x + 1] = m;
+    }
+}
+
+// Per-block partials of sum((a - b)^2), sum(b^2), max|a - b| where a is any
+// precision (divided by its storage scale) and b is FP64.
+template <typename S>
+__global__ void __launch_bounds__(kReduceThreads)
+diff_partial(const S* __restrict__ a, double inv_scale, const double* __restrict__ b, size_t n,
+             double* __restrict__ part) {
+    __shared__ double sh[kReduceThreads];
+    double sd = 0.0, sb = 0.0, m = 0.0;
+    for (size_t i = blockIdx.x * static_cast<size_t>(kReduceThreads) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * kReduceThreads) {
+        const double bv = b[i];
+        const double d = static_cast<double>(a[i]) * inv_scale - bv;
+        sd += d * d;
+        sb += bv * bv;
+        m = fmax(m, fabs(d));
+    }
+    sd = block_sum(sd, sh);
+    sb = block_sum(sb, sh);
+    m = block_max(m, sh);
+    if (threadIdx.x == 0) {
+        part[3 * blockIdx.x] = sd;
+        part[3 * blockIdx.x + 1] = sb;
+        part[3 * blockIdx.x + 2] = m;
+    }
+}
+
+// Final pass over `nblk` partial records of width w; column c is a sum
+// unless (max_mask >> c) & 1.  Deterministic: one block, fixed order.
+__global__ void __launch_bounds__(kReduceThreads)
+reduce_partials(const double* __restrict__ part, int nblk, int w, unsigned max_mask,
+                double* __restrict__ out) {
+    __shared__ double sh[kReduceThreads];
+    for (int col = 0; col < w; ++col) {
+        const bool is_max = (max_mask >> col) & 1u;
+        double acc = 0.0;
+        for (int b = threadIdx.x; b < nblk; b += kReduceThreads) {
+            const double v = part[b * w + col];
+            acc = is_max ? fmax(acc, v) : acc + v;
+        }
+        const double r = is_max ? block_max(acc, sh) : block_sum(acc, sh);
+        if (threadIdx.x == 0) out[col] = r;
+    }
+}
+
+// Element-wise conversions between the FP64 host layout and storage types.
+template <typename S>
+__global__ void __launch_bounds__(256)
+convert_from_f64(const double* __restrict__ src, S* __restrict__ dst, size_t n, double scale) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * 256)
+        dst[i] = static_cast<S>(src[i] * scale);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256)
+convert_to_f64(const S* __restrict__ src, double* __restrict__ dst, size_t n, double inv_scale) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * 256)
+        dst[i] = static_cast<double>(src[i]) * inv_scale;
+}
+
+// Multiply a storage array by an exact power of two (FP16 rescaling).
+template <typename S>
+__global__ void __launch_bounds__(256)
+scale_array(S* __restrict__ x, size_t n, float factor) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * 256)
+        x[i] = static_cast<S>(static_cast<float>(x[i]) * factor);
+}
+
+// Fill with zeros (used instead of hipMemsetAsync so that step sequences can
+// be captured in HIP graphs).
+__global__ void __launch_bounds__(256)
+zero_bytes(uint4* __restrict__ p, size_t n16) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; i < n16;
+         i += static_cast<size_t>(gridDim.x) * 256)
+        p[i] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+__global__ void advance_step(long long* step, long long by) { *step += by; }
+
+// Streaming copy of a known byte count, 16 B per lane: used to calibrate the
+// HBM byte counters against an exactly known traffic.
+__global__ void __launch_bounds__(256)
+copy_calibration(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; i < n16;
+         i += static_cast<size_t>(gridDim.x) * 256)
+        dst[i] = src[i];
+}
+
+}  // namespace fdtd

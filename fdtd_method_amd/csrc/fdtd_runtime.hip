@@ -1,0 +1,1002 @@
+// Host runtime of libfdtd_mi355x.so: grid objects in HBM, HIP streams,
+// launches, diagnostics and the C-ABI declared in include/fdtd_mi355x.h.
+//
+// Reference counterparts (file:line):
+//   fdtd_create          FDTD::FDTD            src/FDTD/FDTD.cpp:3-61
+//   fdtd_upload/download FDTD::get_field       src/FDTD/FDTD.cpp:138-151
+//   fdtd_zero_currents   FDTD::zeroed_currents src/FDTD/FDTD.cpp:132-136
+//   fdtd_update_fields*  FDTD::update_fields   src/FDTD/FDTD.cpp:153-157
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "fdtd_kernels.hpp"
+#include "fdtd_mi355x.h"
+
+namespace {
+
+constexpr double kC = 3e10;              // include/Constants.h:9
+constexpr double kPiRef = 3.14159265358; // include/Constants.h:14
+constexpr int kMaxSparseCurrents = 64;   // beyond this J is kept dense
+constexpr size_t kLdsLimit = 160 * 1024; // gfx950 LDS per workgroup
+constexpr int kReduceBlocks = 1024;
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(FDTD_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct Tuning {
+    int V = 1, TY = 1, kchunk = 1;
+};
+
+size_t storage_size(int precision) {
+    return precision == FDTD_FP64 ? 8 : (precision == FDTD_FP32 ? 4 : 2);
+}
+size_t compute_size(int precision) { return precision == FDTD_FP64 ? 8 : 4; }
+size_t current_size(int precision) { return precision == FDTD_FP64 ? 8 : 4; }
+
+}  // namespace
+
+struct fdtd_grid {
+    fdtd_parameters params{};
+    int Ni = 0, Nj = 0, Nk = 0;
+    size_t N = 0;
+    double dt = 0;
+    int precision = FDTD_FP64;
+    size_t esize = 8;
+    void* E[2][3] = {};
+    void* B[2][3] = {};
+    int cur = 0;
+    void* J[3] = {};
+    bool j_dense_active = false;
+    double* staging = nullptr;
+    double scales[6] = {1, 1, 1, 1, 1, 1};
+    hipStream_t stream = nullptr;
+    long long step = 0;
+    std::map<long long, double> sparse_j;  // key = index*3 + comp
+    std::vector<fdtd::PointSource> waveforms;
+    fdtd::PointSource* d_src = nullptr;
+    double* d_vals = nullptr;  // source values of the step being launched
+    int d_src_cap = 0;
+    int nsrc = 0;
+    bool src_dirty = true;
+    Tuning tune;
+    bool fused_ok = false;
+    double* d_part = nullptr;
+    double* d_out = nullptr;
+    hipEvent_t ev[3] = {};
+};
+
+namespace {
+
+// -------------------------------------------------------------------------
+// Coefficients, src/FDTD/FDTD.cpp:43-53 (FP64, reference order), then the
+// storage-scale ratios (exact powers of two) and a final rounding to C.
+template <typename C>
+fdtd::Coefs<C> make_coefs(const fdtd_grid* g) {
+    const double cdt = kC * g->dt;
+    const double cEx = cdt / g->params.dx, cEy = cdt / g->params.dy, cEz = cdt / g->params.dz;
+    const double cBx = cdt / (2.0 * g->params.dx), cBy = cdt / (2.0 * g->params.dy),
+                 cBz = cdt / (2.0 * g->params.dz);
+    const double cur = -4.0 * kPiRef * g->dt;
+    const double* s = g->scales;  // Ex Ey Ez Bx By Bz
+    fdtd::Coefs<C> c;
+    c.ex_bz = C(cEy * (s[0] / s[5]));
+    c.ex_by = C(cEz * (s[0] / s[4]));
+    c.ey_bx = C(cEz * (s[1] / s[3]));
+    c.ey_bz = C(cEx * (s[1] / s[5]));
+    c.ez_by = C(cEx * (s[2] / s[4]));
+    c.ez_bx = C(cEy * (s[2] / s[3]));
+    c.jx = C(cur * s[0]);
+    c.jy = C(cur * s[1]);
+    c.jz = C(cur * s[2]);
+    c.bx_ey = C(cBz * (s[3] / s[1]));
+    c.bx_ez = C(cBy * (s[3] / s[2]));
+    c.by_ez = C(cBx * (s[4] / s[2]));
+    c.by_ex = C(cBz * (s[4] / s[0]));
+    c.bz_ex = C(cBy * (s[5] / s[0]));
+    c.bz_ey = C(cBx * (s[5] / s[1]));
+    return c;
+}
+
+int grid_blocks(size_t n) {
+    size_t b = (n + 255) / 256;
+    return static_cast<int>(std::min<size_t>(b, 8192));
+}
+
+// -------------------------------------------------------------------------
+// Fused-sweep launch shapes.
+
+int v_candidates(int precision, int* out) {
+    if (precision == FDTD_FP64) { out[0] = 2; out[1] = 1; out[2] = 4; return 3; }
+    if (precision == FDTD_FP32) { out[0] = 4; out[1] = 2; out[2] = 1; out[3] = 8; return 4; }
+    out[0] = 8; out[1] = 4; out[2] = 2; out[3] = 1;
+    return 4;
+}
+
+int max_threads(const fdtd_grid* g, int V) {
+    return fdtd::fused_max_threads(V, (int)compute_size(g->precision));
+}
+
+bool shape_valid(const fdtd_grid* g, int V, int TY, int kchunk) {
+    if (V <= 0 || TY <= 0 || kchunk <= 0) return false;
+    if (V != 1 && V != 2 && V != 4 && V != 8) return false;
+    if (g->precision == FDTD_FP64 && V > 4) return false;
+    if (g->Ni % V != 0) return false;
+    const int lanes = g->Ni / V;
+    if ((TY + 1) * lanes > max_threads(g, V)) return false;
+    if (fdtd::fused_lds_bytes(TY, g->Ni, (int)compute_size(g->precision)) > kLdsLimit) return false;
+    return true;
+}
+
+int default_kchunk(int Nj, int Nk, int TY) {
+    const int ntile_y = (Nj + TY - 1) / TY;
+    int nchunk = std::max(1, std::min(Nk, (1024 + ntile_y - 1) / ntile_y));
+    int kchunk = (Nk + nchunk - 1) / nchunk;
+    return std::max(1, kchunk);
+}
+
+// Largest TY that fits two workgroups per CU in LDS (else one), for the
+// widest V dividing Ni.
+bool default_tuning(const fdtd_grid* g, Tuning* t) {
+    int vc[4];
+    const int nv = v_candidates(g->precision, vc);
+    const int cs = (int)compute_size(g->precision);
+    for (int q = 0; q < nv; ++q) {
+        const int V = vc[q];
+        if (g->Ni % V) continue;
+        const int lanes = g->Ni / V;
+        const int mt = max_threads(g, V);
+        if (2 * lanes > mt) continue;
+        const int tymax = std::min(g->Nj, mt / lanes - 1);
+        int best = 0;
+        for (int ty = tymax; ty >= 1; --ty)
+            if (fdtd::fused_lds_bytes(ty, g->Ni, cs) <= kLdsLimit / 2) { best = ty; break; }
+        if (!best)
+            for (int ty = tymax; ty >= 1; --ty)
+                if (fdtd::fused_lds_bytes(ty, g->Ni, cs) <= kLdsLimit) { best = ty; break; }
+        if (!best) continue;
+        t->V = V;
+        t->TY = best;
+        t->kchunk = default_kchunk(g->Nj, g->Nk, best);
+        return true;
+    }
+    return false;
+}
+
+fdtd::SweepGeom make_geom(const fdtd_grid* g, const Tuning& t) {
+    fdtd::SweepGeom s;
+    s.Ni = g->Ni;
+    s.Nj = g->Nj;
+    s.Nk = g->Nk;
+    s.TY = t.TY;
+    s.kchunk = t.kchunk;
+    s.ntile_y = (g->Nj + t.TY - 1) / t.TY;
+    const int nchunk = (g->Nk + t.kchunk - 1) / t.kchunk;
+    s.ntiles = s.ntile_y * nchunk;
+    s.per_xcd = (s.ntiles + 7) / 8;
+    return s;
+}
+
+template <typename S>
+fdtd::Fields<S> fields_of(fdtd_grid* g, int buf) {
+    fdtd::Fields<S> f;
+    for (int d = 0; d < 3; ++d) {
+        f.e[d] = static_cast<S*>(g->E[buf][d]);
+        f.b[d] = static_cast<S*>(g->B[buf][d]);
+    }
+    return f;
+}
+
+// Push the combined source list (sparse J values as kind-0 sources, then
+// waveform sources) to the device when it changed.
+int flush_sources(fdtd_grid* g) {
+    if (!g->src_dirty) return FDTD_OK;
+    std::vector<fdtd::PointSource> all;
+    all.reserve(g->sparse_j.size() + g->waveforms.size());
+    for (const auto& kv : g->sparse_j) {
+        const long long idx = kv.first / 3;
+        fdtd::PointSource s{};
+        s.comp = (int)(kv.first % 3);
+        s.i = (int)(idx % g->Ni);
+        s.j = (int)((idx / g->Ni) % g->Nj);
+        s.k = (int)(idx / ((long long)g->Ni * g->Nj));
+        s.kind = 0;
+        s.amp = kv.second;
+        all.push_back(s);
+    }
+    all.insert(all.end(), g->waveforms.begin(), g->waveforms.end());
+    if ((int)all.size() > g->d_src_cap) {
+        if (g->d_src) HIP_TRY(hipFree(g->d_src));
+        if (g->d_vals) HIP_TRY(hipFree(g->d_vals));
+        g->d_src = nullptr;
+        g->d_vals = nullptr;
+        const int cap = std::max<int>(16, (int)all.size());
+        HIP_TRY(hipMalloc(&g->d_src, sizeof(fdtd::PointSource) * cap));
+        HIP_TRY(hipMalloc(&g->d_vals, sizeof(double) * cap));
+        g->d_src_cap = cap;
+    }
+    if (!all.empty())
+        HIP_TRY(hipMemcpyAsync(g->d_src, all.data(), sizeof(fdtd::PointSource) * all.size(),
+                               hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    g->nsrc = (int)all.size();
+    g->src_dirty = false;
+    return FDTD_OK;
+}
+
+// J values of all point sources for step `step` (t = (step+1)*dt).
+int eval_sources(fdtd_grid* g, long long step) {
+    if (g->nsrc == 0) return FDTD_OK;
+    hipLaunchKernelGGL(fdtd::eval_sources, dim3((g->nsrc + 63) / 64), dim3(64), 0, g->stream,
+                       g->d_src, g->nsrc, (const long long*)nullptr, step, g->dt, g->d_vals);
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+// ---- plain kernels ----------------------------------------------------------
+
+template <typename S>
+int launch_plain_B(fdtd_grid* g) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    const dim3 block(64, 4, 1);
+    const dim3 grid((g->Ni + 63) / 64, (g->Nj + 3) / 4, g->Nk);
+    hipLaunchKernelGGL((fdtd::plain_update_B<S, C>), grid, block, 0, g->stream,
+                       fields_of<S>(g, g->cur), make_coefs<C>(g), g->Ni, g->Nj, g->Nk);
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+template <typename S>
+int launch_plain_E(fdtd_grid* g, long long step) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    using JT = C;
+    const dim3 block(64, 4, 1);
+    const dim3 grid((g->Ni + 63) / 64, (g->Nj + 3) / 4, g->Nk);
+    const JT* j[3] = {nullptr, nullptr, nullptr};
+    if (g->j_dense_active)
+        for (int d = 0; d < 3; ++d) j[d] = static_cast<const JT*>(g->J[d]);
+    if (int rc = eval_sources(g, step)) return rc;
+    hipLaunchKernelGGL((fdtd::plain_update_E<S, C, JT>), grid, block, 0, g->stream,
+                       fields_of<S>(g, g->cur), j[0], j[1], j[2], g->d_src, g->d_vals, g->nsrc,
+                       make_coefs<C>(g), g->Ni, g->Nj, g->Nk);
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+int plain_B(fdtd_grid* g) {
+    switch (g->precision) {
+        case FDTD_FP64: return launch_plain_B<double>(g);
+        case FDTD_FP32: return launch_plain_B<float>(g);
+        default: return launch_plain_B<_Float16>(g);
+    }
+}
+
+int plain_E(fdtd_grid* g, long long step) {
+    switch (g->precision) {
+        case FDTD_FP64: return launch_plain_E<double>(g, step);
+        case FDTD_FP32: return launch_plain_E<float>(g, step);
+        default: return launch_plain_E<_Float16>(g, step);
+    }
+}
+
+// ---- fused sweep --------------------------------------------------------------
+
+template <typename S, int V, bool STEADY>
+int launch_fused_v(fdtd_grid* g, const Tuning& t, long long step, int in, int out) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    using JT = C;
+    auto kern = fdtd::fused_sweep<S, C, JT, V, STEADY>;
+    fdtd::SweepArgs<S, C, JT> a;
+    a.in = fields_of<S>(g, in);
+    a.out = fields_of<S>(g, out);
+    for (int d = 0; d < 3; ++d)
+        a.j[d] = g->j_dense_active ? static_cast<const JT*>(g->J[d]) : nullptr;
+    a.src = g->d_src;
+    a.vals = g->d_vals;
+    a.nsrc = g->nsrc;
+    a.pad = 0;
+    a.c = make_coefs<C>(g);
+    a.g = make_geom(g, t);
+    const size_t lds = fdtd::fused_lds_bytes(t.TY, g->Ni, sizeof(C));
+    static std::once_flag once;
+    std::call_once(once, [&] {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
+    });
+    const int threads = (t.TY + 1) * (g->Ni / V);
+    if (int rc = eval_sources(g, step)) return rc;
+    hipLaunchKernelGGL(kern, dim3(8 * a.g.per_xcd), dim3(threads), lds, g->stream, a);
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+template <typename S, bool STEADY>
+int launch_fused_s(fdtd_grid* g, const Tuning& t, long long step, int in, int out) {
+    switch (t.V) {
+        case 1: return launch_fused_v<S, 1, STEADY>(g, t, step, in, out);
+        case 2: return launch_fused_v<S, 2, STEADY>(g, t, step, in, out);
+        case 4: return launch_fused_v<S, 4, STEADY>(g, t, step, in, out);
+        case 8:
+            if constexpr (sizeof(S) <= 4) return launch_fused_v<S, 8, STEADY>(g, t, step, in, out);
+            [[fallthrough]];
+        default: return fail(FDTD_ERR_UNSUPPORTED, "unsupported V=%d", t.V);
+    }
+}
+
+int launch_fused(fdtd_grid* g, const Tuning& t, bool steady, long long step, int in, int out) {
+    switch (g->precision) {
+        case FDTD_FP64:
+            return steady ? launch_fused_s<double, true>(g, t, step, in, out)
+                          : launch_fused_s<double, false>(g, t, step, in, out);
+        case FDTD_FP32:
+            return steady ? launch_fused_s<float, true>(g, t, step, in, out)
+                          : launch_fused_s<float, false>(g, t, step, in, out);
+        default:
+            return steady ? launch_fused_s<_Float16, true>(g, t, step, in, out)
+                          : launch_fused_s<_Float16, false>(g, t, step, in, out);
+    }
+}
+
+// n steps: B half step in place, then n fused sweeps between the buffers.
+int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
+    int rc = plain_B(g);
+    if (rc) return rc;
+    if (mid) HIP_TRY(hipEventRecord(mid, g->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        const int in = g->cur, out = 1 - g->cur;
+        rc = launch_fused(g, g->tune, s + 1 < nsteps, g->step + s, in, out);
+        if (rc) return rc;
+        g->cur = out;
+    }
+    g->step += nsteps;
+    return FDTD_OK;
+}
+
+int run_3sweep(fdtd_grid* g, int nsteps) {
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = plain_B(g);
+        if (!rc) rc = plain_E(g, g->step);
+        if (!rc) rc = plain_B(g);
+        if (rc) return rc;
+        g->step += 1;
+    }
+    return FDTD_OK;
+}
+
+// ---- conversions and reductions ------------------------------------------------
+
+int ensure_staging(fdtd_grid* g) {
+    if (!g->staging) HIP_TRY(hipMalloc(&g->staging, g->N * sizeof(double)));
+    return FDTD_OK;
+}
+
+template <typename S>
+void launch_from_f64(const double* src, void* dst, size_t n, double scale, hipStream_t s) {
+    hipLaunchKernelGGL(fdtd::convert_from_f64<S>, dim3(grid_blocks(n)), dim3(256), 0, s, src,
+                       static_cast<S*>(dst), n, scale);
+}
+
+template <typename S>
+void launch_to_f64(const void* src, double* dst, size_t n, double inv, hipStream_t s) {
+    hipLaunchKernelGGL(fdtd::convert_to_f64<S>, dim3(grid_blocks(n)), dim3(256), 0, s,
+                       static_cast<const S*>(src), dst, n, inv);
+}
+
+void from_f64(int prec, const double* src, void* dst, size_t n, double scale, hipStream_t s) {
+    if (prec == FDTD_FP32) launch_from_f64<float>(src, dst, n, scale, s);
+    else launch_from_f64<_Float16>(src, dst, n, scale, s);
+}
+
+void to_f64(int prec, const void* src, double* dst, size_t n, double inv, hipStream_t s) {
+    if (prec == FDTD_FP32) launch_to_f64<float>(src, dst, n, inv, s);
+    else launch_to_f64<_Float16>(src, dst, n, inv, s);
+}
+
+int zero_fill(void* p, size_t bytes, hipStream_t s) {
+    const size_t n16 = bytes / 16;
+    if (n16)
+        hipLaunchKernelGGL(fdtd::zero_bytes, dim3(grid_blocks(n16)), dim3(256), 0, s,
+                           static_cast<uint4*>(p), n16);
+    const size_t tail = bytes - n16 * 16;
+    if (tail) HIP_TRY(hipMemsetAsync(static_cast<char*>(p) + n16 * 16, 0, tail, s));
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+int ensure_dense_j(fdtd_grid* g) {
+    const size_t bytes = g->N * current_size(g->precision);
+    for (int d = 0; d < 3; ++d)
+        if (!g->J[d]) HIP_TRY(hipMalloc(&g->J[d], bytes + 16));
+    return FDTD_OK;
+}
+
+// sum x^2 and max|x| of one storage array (physical units).
+int reduce_field(fdtd_grid* g, const void* p, double inv, double* sumsq, double* maxabs) {
+    const int nb = std::min<int>(kReduceBlocks, grid_blocks(g->N));
+    switch (g->precision) {
+        case FDTD_FP64:
+            hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<double>, dim3(nb), dim3(256), 0,
+                               g->stream, static_cast<const double*>(p), g->N, inv, g->d_part);
+            break;
+        case FDTD_FP32:
+            hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<float>, dim3(nb), dim3(256), 0,
+                               g->stream, static_cast<const float*>(p), g->N, inv, g->d_part);
+            break;
+        default:
+            hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<_Float16>, dim3(nb), dim3(256), 0,
+                               g->stream, static_cast<const _Float16*>(p), g->N, inv, g->d_part);
+    }
+    hipLaunchKernelGGL(fdtd::reduce_partials, dim3(1), dim3(256), 0, g->stream, g->d_part, nb, 2,
+                       2u, g->d_out);
+    HIP_TRY(hipGetLastError());
+    double r[2];
+    HIP_TRY(hipMemcpyAsync(r, g->d_out, sizeof r, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    *sumsq = r[0];
+    *maxabs = r[1];
+    return FDTD_OK;
+}
+
+int check_component(int c) {
+    if (c < 0 || c > 8) return fail(FDTD_ERR_INVALID_COMPONENT, "ERROR: Invalid field component");
+    return FDTD_OK;
+}
+
+void* field_ptr(fdtd_grid* g, int c) {
+    return c < 3 ? g->E[g->cur][c] : g->B[g->cur][c - 3];
+}
+
+}  // namespace
+
+// =============================================================================
+// C-ABI
+// =============================================================================
+
+extern "C" {
+
+const char* fdtd_last_error(void) { return g_last_error.c_str(); }
+
+int fdtd_create(const fdtd_parameters* p, double dt, int precision, fdtd_grid** out) {
+    if (!p || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    // src/FDTD/FDTD.cpp:5-7
+    if (p->Ni <= 0 || p->Nj <= 0 || p->Nk <= 0 || !(dt > 0))
+        return fail(FDTD_ERR_INVALID_PARAMETERS, "ERROR: invalid parameters");
+    if (precision < FDTD_FP64 || precision > FDTD_FP16)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid precision %d", precision);
+    const long long cells = (long long)p->Ni * p->Nj * p->Nk;
+    if (cells >= (1LL << 31))
+        return fail(FDTD_ERR_INVALID_PARAMETERS, "grid of %lld cells exceeds 2^31-1", cells);
+    fdtd_grid* g = new fdtd_grid();
+    g->params = *p;
+    g->Ni = p->Ni;
+    g->Nj = p->Nj;
+    g->Nk = p->Nk;
+    g->N = (size_t)cells;
+    g->dt = dt;
+    g->precision = precision;
+    g->esize = storage_size(precision);
+    auto cleanup = [&](int rc) {
+        fdtd_destroy(g);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(FDTD_ERR_HIP, "hipStreamCreate failed"));
+    const size_t bytes = g->N * g->esize + 16;
+    for (int b = 0; b < 2; ++b)
+        for (int d = 0; d < 3; ++d) {
+            if (hipMalloc(&g->E[b][d], bytes) != hipSuccess ||
+                hipMalloc(&g->B[b][d], bytes) != hipSuccess)
+                return cleanup(fail(FDTD_ERR_HIP, "out of device memory for %lld cells", cells));
+        }
+    for (int d = 0; d < 3; ++d) {
+        int rc = zero_fill(g->E[0][d], bytes, g->stream);
+        if (!rc) rc = zero_fill(g->B[0][d], bytes, g->stream);
+        if (rc) return cleanup(rc);
+    }
+    if (hipMalloc(&g->d_part, sizeof(double) * 3 * kReduceBlocks) != hipSuccess ||
+        hipMalloc(&g->d_out, sizeof(double) * 8) != hipSuccess)
+        return cleanup(fail(FDTD_ERR_HIP, "out of device memory"));
+    for (auto& e : g->ev)
+        if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(FDTD_ERR_HIP, "hipEventCreate"));
+    g->fused_ok = default_tuning(g, &g->tune);
+    if (hipStreamSynchronize(g->stream) != hipSuccess)
+        return cleanup(fail(FDTD_ERR_HIP, "initialisation failed"));
+    *out = g;
+    return FDTD_OK;
+}
+
+int fdtd_destroy(fdtd_grid* g) {
+    if (!g) return FDTD_OK;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (int b = 0; b < 2; ++b)
+        for (int d = 0; d < 3; ++d) {
+            if (g->E[b][d]) (void)hipFree(g->E[b][d]);
+            if (g->B[b][d]) (void)hipFree(g->B[b][d]);
+        }
+    for (int d = 0; d < 3; ++d)
+        if (g->J[d]) (void)hipFree(g->J[d]);
+    if (g->staging) (void)hipFree(g->staging);
+    if (g->d_src) (void)hipFree(g->d_src);
+    if (g->d_vals) (void)hipFree(g->d_vals);
+    if (g->d_part) (void)hipFree(g->d_part);
+    if (g->d_out) (void)hipFree(g->d_out);
+    for (auto& e : g->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+    return FDTD_OK;
+}
+
+int fdtd_upload(fdtd_grid* g, int c, const double* host, size_t count) {
+    if (!g || !host) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (int rc = check_component(c)) return rc;
+    if (count != g->N)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "count %zu != grid cells %zu", count, g->N);
+    if (c >= 6) {
+        if (int rc = ensure_dense_j(g)) return rc;
+        const int d = c - 6;
+        if (g->precision == FDTD_FP64) {
+            HIP_TRY(hipMemcpyAsync(g->J[d], host, g->N * 8, hipMemcpyHostToDevice, g->stream));
+        } else {
+            if (int rc = ensure_staging(g)) return rc;
+            HIP_TRY(hipMemcpyAsync(g->staging, host, g->N * 8, hipMemcpyHostToDevice, g->stream));
+            launch_from_f64<float>(g->staging, g->J[d], g->N, 1.0, g->stream);
+        }
+        if (!g->j_dense_active) {
+            // the other two components start from zero
+            for (int o = 0; o < 3; ++o)
+                if (o != d)
+                    if (int rc = zero_fill(g->J[o], g->N * current_size(g->precision), g->stream))
+                        return rc;
+            g->j_dense_active = true;
+        }
+        for (auto it = g->sparse_j.begin(); it != g->sparse_j.end();) {
+            if (it->first % 3 == d) it = g->sparse_j.erase(it);
+            else ++it;
+        }
+        g->src_dirty = true;
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        return FDTD_OK;
+    }
+    void* dst = field_ptr(g, c);
+    if (g->precision == FDTD_FP64) {
+        HIP_TRY(hipMemcpyAsync(dst, host, g->N * 8, hipMemcpyHostToDevice, g->stream));
+    } else {
+        if (int rc = ensure_staging(g)) return rc;
+        HIP_TRY(hipMemcpyAsync(g->staging, host, g->N * 8, hipMemcpyHostToDevice, g->stream));
+        from_f64(g->precision, g->staging, dst, g->N, g->scales[c], g->stream);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return FDTD_OK;
+}
+
+int fdtd_download(fdtd_grid* g, int c, double* host, size_t count) {
+    if (!g || !host) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (int rc = check_component(c)) return rc;
+    if (count != g->N)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "count %zu != grid cells %zu", count, g->N);
+    if (c >= 6) {
+        const int d = c - 6;
+        if (g->j_dense_active) {
+            if (g->precision == FDTD_FP64) {
+                HIP_TRY(hipMemcpyAsync(host, g->J[d], g->N * 8, hipMemcpyDeviceToHost, g->stream));
+            } else {
+                if (int rc = ensure_staging(g)) return rc;
+                launch_to_f64<float>(g->J[d], g->staging, g->N, 1.0, g->stream);
+                HIP_TRY(hipMemcpyAsync(host, g->staging, g->N * 8, hipMemcpyDeviceToHost, g->stream));
+            }
+            HIP_TRY(hipStreamSynchronize(g->stream));
+        } else {
+            std::memset(host, 0, g->N * sizeof(double));
+        }
+        for (const auto& kv : g->sparse_j)
+            if (kv.first % 3 == d) host[kv.first / 3] = kv.second;
+        return FDTD_OK;
+    }
+    const void* src = field_ptr(g, c);
+    if (g->precision == FDTD_FP64) {
+        HIP_TRY(hipMemcpyAsync(host, src, g->N * 8, hipMemcpyDeviceToHost, g->stream));
+    } else {
+        if (int rc = ensure_staging(g)) return rc;
+        to_f64(g->precision, src, g->staging, g->N, 1.0 / g->scales[c], g->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host, g->staging, g->N * 8, hipMemcpyDeviceToHost, g->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return FDTD_OK;
+}
+
+int fdtd_zero_currents(fdtd_grid* g) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    g->j_dense_active = false;
+    if (!g->sparse_j.empty()) g->src_dirty = true;
+    g->sparse_j.clear();
+    return FDTD_OK;
+}
+
+int fdtd_set_current_values(fdtd_grid* g, int n, const long long* index, const int* comp,
+                            const double* value) {
+    if (!g || n < 0 || (n > 0 && (!index || !comp || !value)))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    for (int q = 0; q < n; ++q) {
+        if (comp[q] < FDTD_JX || comp[q] > FDTD_JZ)
+            return fail(FDTD_ERR_INVALID_COMPONENT, "ERROR: Invalid field component");
+        if (index[q] < 0 || (size_t)index[q] >= g->N)
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "index %lld out of range", index[q]);
+    }
+    if (g->j_dense_active) {
+        // scatter into the dense arrays (host round trip of single values)
+        for (int q = 0; q < n; ++q) {
+            const int d = comp[q] - FDTD_JX;
+            if (g->precision == FDTD_FP64) {
+                HIP_TRY(hipMemcpyAsync(static_cast<double*>(g->J[d]) + index[q], &value[q], 8,
+                                       hipMemcpyHostToDevice, g->stream));
+            } else {
+                const float v = (float)value[q];
+                HIP_TRY(hipMemcpyAsync(static_cast<float*>(g->J[d]) + index[q], &v, 4,
+                                       hipMemcpyHostToDevice, g->stream));
+                HIP_TRY(hipStreamSynchronize(g->stream));
+            }
+        }
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        return FDTD_OK;
+    }
+    for (int q = 0; q < n; ++q)
+        g->sparse_j[index[q] * 3 + (comp[q] - FDTD_JX)] = value[q];
+    g->src_dirty = true;
+    if ((int)g->sparse_j.size() > kMaxSparseCurrents) {
+        // Too many cells for the per-plane source scan: materialise dense J.
+        if (int rc = ensure_dense_j(g)) return rc;
+        for (int d = 0; d < 3; ++d)
+            if (int rc = zero_fill(g->J[d], g->N * current_size(g->precision), g->stream)) return rc;
+        g->j_dense_active = true;
+        std::vector<long long> idx;
+        std::vector<int> cc;
+        std::vector<double> vv;
+        for (const auto& kv : g->sparse_j) {
+            idx.push_back(kv.first / 3);
+            cc.push_back((int)(kv.first % 3) + FDTD_JX);
+            vv.push_back(kv.second);
+        }
+        g->sparse_j.clear();
+        return fdtd_set_current_values(g, (int)idx.size(), idx.data(), cc.data(), vv.data());
+    }
+    return FDTD_OK;
+}
+
+int fdtd_set_point_sources(fdtd_grid* g, int n, const fdtd_point_source* src) {
+    if (!g || n < 0 || (n > 0 && !src)) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    std::vector<fdtd::PointSource> w;
+    for (int q = 0; q < n; ++q) {
+        const fdtd_point_source& s = src[q];
+        if (s.i < 0 || s.i >= g->Ni || s.j < 0 || s.j >= g->Nj || s.k < 0 || s.k >= g->Nk)
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "source %d outside the grid", q);
+        if (s.comp < 0 || s.comp > 2)
+            return fail(FDTD_ERR_INVALID_COMPONENT, "ERROR: Invalid field component");
+        if (s.kind != 0 && s.kind != 1)
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "source %d: unknown kind %d", q, s.kind);
+        if (s.kind == 1 && !(s.period != 0.0))
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "source %d: zero period", q);
+        fdtd::PointSource p{};
+        p.i = s.i; p.j = s.j; p.k = s.k; p.comp = s.comp; p.kind = s.kind;
+        p.amp = s.amp; p.period = s.period; p.t0 = s.t0; p.sigma = s.sigma; p.phase = s.phase;
+        w.push_back(p);
+    }
+    g->waveforms = w;
+    g->src_dirty = true;
+    return FDTD_OK;
+}
+
+int fdtd_update_fields(fdtd_grid* g, int nsteps) {
+    if (!g || nsteps < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (nsteps == 0) return FDTD_OK;
+    if (int rc = flush_sources(g)) return rc;
+    if (!g->fused_ok) return run_3sweep(g, nsteps);
+    return run_fused(g, nsteps);
+}
+
+int fdtd_update_fields_3sweep(fdtd_grid* g, int nsteps) {
+    if (!g || nsteps < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (int rc = flush_sources(g)) return rc;
+    return run_3sweep(g, nsteps);
+}
+
+int fdtd_get_step(const fdtd_grid* g, long long* step) {
+    if (!g || !step) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    *step = g->step;
+    return FDTD_OK;
+}
+
+int fdtd_set_step(fdtd_grid* g, long long step) {
+    if (!g || step < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    g->step = step;
+    return FDTD_OK;
+}
+
+int fdtd_synchronize(fdtd_grid* g) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return FDTD_OK;
+}
+
+int fdtd_get_info(const fdtd_grid* g, int dims[3], int* precision, long long* cells) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (dims) { dims[0] = g->Ni; dims[1] = g->Nj; dims[2] = g->Nk; }
+    if (precision) *precision = g->precision;
+    if (cells) *cells = (long long)g->N;
+    return FDTD_OK;
+}
+
+int fdtd_sum_squares(fdtd_grid* g, double out[6]) {
+    if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    for (int c = 0; c < 6; ++c) {
+        double m;
+        if (int rc = reduce_field(g, field_ptr(g, c), 1.0 / g->scales[c], &out[c], &m)) return rc;
+    }
+    return FDTD_OK;
+}
+
+int fdtd_max_abs(fdtd_grid* g, double out[6]) {
+    if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    for (int c = 0; c < 6; ++c) {
+        double s;
+        if (int rc = reduce_field(g, field_ptr(g, c), 1.0 / g->scales[c], &s, &out[c])) return rc;
+    }
+    return FDTD_OK;
+}
+
+int fdtd_compare(fdtd_grid* t, fdtd_grid* r, double out[18]) {
+    if (!t || !r || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (r->precision != FDTD_FP64) return fail(FDTD_ERR_INVALID_ARGUMENT, "reference grid must be FP64");
+    if (t->Ni != r->Ni || t->Nj != r->Nj || t->Nk != r->Nk)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "grid shapes differ");
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    const int nb = std::min<int>(kReduceBlocks, grid_blocks(t->N));
+    for (int c = 0; c < 6; ++c) {
+        const double* b = static_cast<const double*>(field_ptr(r, c));
+        const double inv = 1.0 / t->scales[c];
+        const void* a = field_ptr(t, c);
+        switch (t->precision) {
+            case FDTD_FP64:
+                hipLaunchKernelGGL(fdtd::diff_partial<double>, dim3(nb), dim3(256), 0, t->stream,
+                                   static_cast<const double*>(a), inv, b, t->N, t->d_part);
+                break;
+            case FDTD_FP32:
+                hipLaunchKernelGGL(fdtd::diff_partial<float>, dim3(nb), dim3(256), 0, t->stream,
+                                   static_cast<const float*>(a), inv, b, t->N, t->d_part);
+                break;
+            default:
+                hipLaunchKernelGGL(fdtd::diff_partial<_Float16>, dim3(nb), dim3(256), 0, t->stream,
+                                   static_cast<const _Float16*>(a), inv, b, t->N, t->d_part);
+        }
+        hipLaunchKernelGGL(fdtd::reduce_partials, dim3(1), dim3(256), 0, t->stream, t->d_part, nb,
+                           3, 4u, t->d_out);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out + 3 * c, t->d_out, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                               t->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    return FDTD_OK;
+}
+
+int fdtd_set_scales(fdtd_grid* g, const double scales[6]) {
+    if (!g || !scales) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    for (int c = 0; c < 6; ++c) {
+        int e;
+        const double m = std::frexp(scales[c], &e);
+        if (!(scales[c] > 0) || m != 0.5)
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "scale %d (%g) is not a power of two", c, scales[c]);
+    }
+    if (g->precision != FDTD_FP16) {
+        for (int c = 0; c < 6; ++c)
+            if (scales[c] != 1.0)
+                return fail(FDTD_ERR_UNSUPPORTED, "storage scales apply to FP16 grids only");
+        return FDTD_OK;
+    }
+    for (int c = 0; c < 6; ++c) {
+        const double f = scales[c] / g->scales[c];
+        if (f != 1.0)
+            for (int b = 0; b < 2; ++b) {
+                void* p = c < 3 ? g->E[b][c] : g->B[b][c - 3];
+                hipLaunchKernelGGL(fdtd::scale_array<_Float16>, dim3(grid_blocks(g->N)), dim3(256), 0,
+                                   g->stream, static_cast<_Float16*>(p), g->N, (float)f);
+            }
+        g->scales[c] = scales[c];
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return FDTD_OK;
+}
+
+int fdtd_get_scales(const fdtd_grid* g, double scales[6]) {
+    if (!g || !scales) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    for (int c = 0; c < 6; ++c) scales[c] = g->scales[c];
+    return FDTD_OK;
+}
+
+int fdtd_autoscale(fdtd_grid* g, int target_log2, double floor_mag) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (g->precision != FDTD_FP16) return FDTD_OK;
+    double mx[6];
+    if (int rc = fdtd_max_abs(g, mx)) return rc;
+    double s[6];
+    for (int c = 0; c < 6; ++c) {
+        const double m = std::max(mx[c], floor_mag);
+        if (!(m > 0)) { s[c] = g->scales[c]; continue; }
+        int e;
+        std::frexp(m, &e);  // m < 2^e
+        int k = target_log2 - e;
+        k = std::max(-60, std::min(60, k));
+        s[c] = std::ldexp(1.0, k);
+    }
+    return fdtd_set_scales(g, s);
+}
+
+int fdtd_set_tuning(fdtd_grid* g, int V, int TY, int kchunk) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    Tuning t = g->tune;
+    if (V > 0) t.V = V;
+    if (TY > 0) t.TY = TY;
+    if (kchunk > 0) t.kchunk = kchunk;
+    if (!shape_valid(g, t.V, t.TY, t.kchunk))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "launch shape V=%d TY=%d kchunk=%d invalid for Ni=%d",
+                    t.V, t.TY, t.kchunk, g->Ni);
+    g->tune = t;
+    g->fused_ok = true;
+    return FDTD_OK;
+}
+
+int fdtd_get_tuning(const fdtd_grid* g, int out[3]) {
+    if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    out[0] = g->tune.V;
+    out[1] = g->tune.TY;
+    out[2] = g->tune.kchunk;
+    return g->fused_ok ? FDTD_OK : fail(FDTD_ERR_UNSUPPORTED, "no fused launch shape fits this grid");
+}
+
+int fdtd_time_update_fields(fdtd_grid* g, int nsteps, int path, float* ms_total, float* ms_main) {
+    if (!g || nsteps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (int rc = flush_sources(g)) return rc;
+    HIP_TRY(hipEventRecord(g->ev[0], g->stream));
+    int rc;
+    if (path == FDTD_PATH_FUSED && g->fused_ok) {
+        rc = run_fused(g, nsteps, g->ev[1]);
+    } else {
+        HIP_TRY(hipEventRecord(g->ev[1], g->stream));
+        rc = run_3sweep(g, nsteps);
+    }
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(g->ev[2], g->stream));
+    HIP_TRY(hipEventSynchronize(g->ev[2]));
+    float a = 0, b = 0;
+    HIP_TRY(hipEventElapsedTime(&a, g->ev[0], g->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&b, g->ev[1], g->ev[2]));
+    if (ms_total) *ms_total = a;
+    if (ms_main) *ms_main = b;
+    return FDTD_OK;
+}
+
+int fdtd_autotune(fdtd_grid* g, int reps, int out[3], float* best_ms) {
+    if (!g || reps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (int rc = flush_sources(g)) return rc;
+    // Time sweeps between the two buffers of the grid; the fields are saved
+    // to and restored from scratch copies so tuning leaves the state intact.
+    std::vector<void*> saved;
+    const size_t bytes = g->N * g->esize;
+    for (int d = 0; d < 6; ++d) {
+        void* p = nullptr;
+        HIP_TRY(hipMalloc(&p, bytes));
+        HIP_TRY(hipMemcpyAsync(p, field_ptr(g, d), bytes, hipMemcpyDeviceToDevice, g->stream));
+        saved.push_back(p);
+    }
+    int vc[4];
+    const int nv = v_candidates(g->precision, vc);
+    Tuning best = g->tune;
+    float best_t = 1e30f;
+    const int cs = (int)compute_size(g->precision);
+    for (int q = 0; q < nv; ++q) {
+        const int V = vc[q];
+        if (g->Ni % V) continue;
+        const int lanes = g->Ni / V;
+        for (int TY = 1; TY <= std::min(g->Nj, 16); ++TY) {
+            if ((TY + 1) * lanes > max_threads(g, V)) break;
+            if (fdtd::fused_lds_bytes(TY, g->Ni, cs) > kLdsLimit) break;
+            const int kdef = default_kchunk(g->Nj, g->Nk, TY);
+            const int kcs[3] = {kdef, std::max(1, kdef / 2), std::min(g->Nk, kdef * 2)};
+            for (int kc : kcs) {
+                Tuning t{V, TY, kc};
+                if (!shape_valid(g, V, TY, kc)) continue;
+                // warm-up then timed reps
+                if (int rc = launch_fused(g, t, true, g->step, g->cur, 1 - g->cur)) return rc;
+                HIP_TRY(hipEventRecord(g->ev[0], g->stream));
+                for (int r = 0; r < reps; ++r)
+                    if (int rc = launch_fused(g, t, true, g->step, (r & 1) ? 1 - g->cur : g->cur,
+                                              (r & 1) ? g->cur : 1 - g->cur))
+                        return rc;
+                HIP_TRY(hipEventRecord(g->ev[1], g->stream));
+                HIP_TRY(hipEventSynchronize(g->ev[1]));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
+                ms /= reps;
+                if (ms < best_t) {
+                    best_t = ms;
+                    best = t;
+                }
+            }
+        }
+    }
+    for (int d = 0; d < 6; ++d) {
+        HIP_TRY(hipMemcpyAsync(field_ptr(g, d), saved[d], bytes, hipMemcpyDeviceToDevice, g->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    for (void* p : saved) HIP_TRY(hipFree(p));
+    if (best_t >= 1e30f) return fail(FDTD_ERR_UNSUPPORTED, "no fused launch shape fits this grid");
+    g->tune = best;
+    g->fused_ok = true;
+    if (out) { out[0] = best.V; out[1] = best.TY; out[2] = best.kchunk; }
+    if (best_ms) *best_ms = best_t;
+    return FDTD_OK;
+}
+
+int fdtd_calibration_copy(size_t bytes, int iters, float* ms) {
+    if (bytes == 0 || bytes % 16 || iters <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    void *a = nullptr, *b = nullptr;
+    HIP_TRY(hipMalloc(&a, bytes));
+    HIP_TRY(hipMalloc(&b, bytes));
+    hipStream_t s;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    const size_t n16 = bytes / 16;
+    hipLaunchKernelGGL(fdtd::zero_bytes, dim3(grid_blocks(n16)), dim3(256), 0, s,
+                       static_cast<uint4*>(a), n16);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int nb = 256 * 8 * 4;
+    hipLaunchKernelGGL(fdtd::copy_calibration, dim3(nb), dim3(256), 0, s,
+                       static_cast<const uint4*>(a), static_cast<uint4*>(b), n16);
+    HIP_TRY(hipEventRecord(e0, s));
+    for (int r = 0; r < iters; ++r)
+        hipLaunchKernelGGL(fdtd::copy_calibration, dim3(nb), dim3(256), 0, s,
+                           static_cast<const uint4*>(a), static_cast<uint4*>(b), n16);
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t / iters;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    return FDTD_OK;
+}
+
+// Batched API: implemented in fdtd_batch.hip.
+
+}  // extern "C"

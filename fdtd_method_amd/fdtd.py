@@ -1,0 +1,281 @@
+"""Python mirror of the reference's FDTD class on the MI355X library.
+
+``FDTD`` has the public surface of ``FDTD_openmp::FDTD`` /
+``FDTD_kokkos::FDTD`` (include/FDTD/FDTD.h:8-41,
+include/FDTD_kokkos/FDTD_kokkos.h:7-36):
+
+=====================================  ==========================================
+reference                              here
+=====================================  ==========================================
+``FDTD(Parameters, FP dt)``            ``FDTD(parameters, dt, precision=FP64)``
+``Field& get_field(Component)``        ``get_field(component) -> Field``
+``virtual void update_fields()``       ``update_fields(nsteps=1)``
+``void zeroed_currents()``             ``zeroed_currents()``
+=====================================  ==========================================
+
+Errors match the reference's: ``ValueError("ERROR: invalid parameters")``
+(std::invalid_argument, src/FDTD/FDTD.cpp:6) and
+``LookupError("ERROR: Invalid field component")`` (std::logic_error, :149).
+
+``Field`` is a host mirror of one device array supporting ``f[idx]``
+(OpenMP style) and ``f(idx)`` (Kokkos style).  Reads download lazily;
+writes to E/B mark the mirror dirty and are uploaded before the next step;
+single-cell writes to J go to the device as sparse current values (the
+drivers' 8-cell box source, perf-tests/sample/sample.cpp:67-81).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .structures import Component, Parameters, Precision
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Field:
+    """Host mirror of one component (the reference's ``Field&``)."""
+
+    def __init__(self, owner: "FDTD", comp: Component):
+        self._owner = owner
+        self._comp = Component(comp)
+        self._host: np.ndarray | None = None
+        self._dirty = False
+
+    # -- mirror management ---------------------------------------------------
+    def _pull(self) -> np.ndarray:
+        if self._host is None:
+            self._host = self._owner.download(self._comp)
+        return self._host
+
+    def _invalidate(self) -> None:
+        if not self._dirty:
+            self._host = None
+
+    def _push(self) -> None:
+        if self._dirty:
+            self._owner.upload(self._comp, self._host)
+            self._dirty = False
+
+    # -- element access ------------------------------------------------------
+    def __len__(self) -> int:
+        return self._owner.cells
+
+    def size(self) -> int:
+        return self._owner.cells
+
+    def __getitem__(self, idx):
+        return self._pull()[idx]
+
+    def __call__(self, idx):
+        return self._pull()[idx]
+
+    def __setitem__(self, idx, value):
+        if self._comp >= Component.JX and np.isscalar(idx) and np.isscalar(value) \
+                and not self._dirty:
+            # sparse current write: goes straight to the device list
+            self._owner.set_current_values([int(idx)], [int(self._comp)], [float(value)])
+            if self._host is not None:
+                self._host[idx] = value
+            return
+        host = self._pull()
+        host[idx] = value
+        self._dirty = True
+
+    def numpy(self) -> np.ndarray:
+        """Copy of the current values (FP64)."""
+        self._owner._flush()
+        return self._pull().copy()
+
+    def assign(self, values) -> None:
+        """Replace the whole field."""
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if a.size != self._owner.cells:
+            raise ValueError("size mismatch")
+        self._host = a.copy()
+        self._dirty = True
+
+
+class FDTD:
+    """Yee solver on one MI355X grid (reference API, see module docstring)."""
+
+    def __init__(self, parameters: Parameters, dt: float, precision: Precision = Precision.FP64):
+        L = _lib.load()
+        p = _lib.Parameters(parameters.Ni, parameters.Nj, parameters.Nk, parameters.ax,
+                            parameters.bx, parameters.ay, parameters.by, parameters.az,
+                            parameters.bz, parameters.dx, parameters.dy, parameters.dz)
+        h = ctypes.c_void_p()
+        _lib.check(L.fdtd_create(ctypes.byref(p), float(dt), int(precision), ctypes.byref(h)))
+        self._h = h
+        self._L = L
+        self.parameters = parameters
+        self.dt = float(dt)
+        self.precision = Precision(precision)
+        self.shape = (parameters.Ni, parameters.Nj, parameters.Nk)
+        self.cells = parameters.Ni * parameters.Nj * parameters.Nk
+        self._fields = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._L.fdtd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- reference API ---------------------------------------------------------
+    def get_field(self, component) -> Field:
+        try:
+            comp = Component(component)
+        except ValueError:
+            raise LookupError("ERROR: Invalid field component") from None
+        f = self._fields.get(comp)
+        if f is None:
+            f = self._fields[comp] = Field(self, comp)
+        return f
+
+    def update_fields(self, nsteps: int = 1) -> None:
+        """nsteps reference update_fields() calls (fused sweeps)."""
+        self._flush()
+        _lib.check(self._L.fdtd_update_fields(self._h, int(nsteps)))
+        self._invalidate()
+
+    def update_fields_3sweep(self, nsteps: int = 1) -> None:
+        """Same steps in the reference's B/2, E, B/2 three-sweep structure."""
+        self._flush()
+        _lib.check(self._L.fdtd_update_fields_3sweep(self._h, int(nsteps)))
+        self._invalidate()
+
+    def zeroed_currents(self) -> None:
+        for c in (Component.JX, Component.JY, Component.JZ):
+            f = self._fields.get(c)
+            if f is not None:
+                f._host = None
+                f._dirty = False
+        _lib.check(self._L.fdtd_zero_currents(self._h))
+
+    # -- extensions ---------------------------------------------------------------
+    def _flush(self) -> None:
+        for f in self._fields.values():
+            f._push()
+
+    def _invalidate(self) -> None:
+        for c, f in self._fields.items():
+            if c < Component.JX:
+                f._invalidate()
+
+    def upload(self, comp, values) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        if a.size != self.cells:
+            raise ValueError(f"expected {self.cells} values, got {a.size}")
+        _lib.check(self._L.fdtd_upload(self._h, int(comp), _dptr(a), a.size))
+
+    def download(self, comp) -> np.ndarray:
+        out = np.empty(self.cells, dtype=np.float64)
+        _lib.check(self._L.fdtd_download(self._h, int(comp), _dptr(out), out.size))
+        return out
+
+    def set_current_values(self, index, comp, value) -> None:
+        idx = np.ascontiguousarray(index, dtype=np.int64)
+        cc = np.ascontiguousarray(comp, dtype=np.int32)
+        vv = np.ascontiguousarray(value, dtype=np.float64)
+        _lib.check(self._L.fdtd_set_current_values(
+            self._h, idx.size, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+            cc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dptr(vv)))
+
+    def set_point_sources(self, sources) -> None:
+        """sources: iterable of dicts {i,j,k,comp(0..2),kind,amp,period,t0,sigma,phase}."""
+        arr = (_lib.PointSource * max(1, len(sources)))()
+        for q, s in enumerate(sources):
+            arr[q] = _lib.PointSource(int(s["i"]), int(s["j"]), int(s["k"]), int(s["comp"]),
+                                      int(s.get("kind", 1)), 0, float(s.get("amp", 1.0)),
+                                      float(s.get("period", 1.0)), float(s.get("t0", 0.0)),
+                                      float(s.get("sigma", 0.0)), float(s.get("phase", 0.0)))
+        _lib.check(self._L.fdtd_set_point_sources(self._h, len(sources), arr))
+
+    @property
+    def step(self) -> int:
+        v = ctypes.c_longlong()
+        _lib.check(self._L.fdtd_get_step(self._h, ctypes.byref(v)))
+        return v.value
+
+    @step.setter
+    def step(self, n: int) -> None:
+        _lib.check(self._L.fdtd_set_step(self._h, int(n)))
+
+    def synchronize(self) -> None:
+        _lib.check(self._L.fdtd_synchronize(self._h))
+
+    def fields(self):
+        """[Ex, Ey, Ez, Bx, By, Bz] as FP64 numpy arrays."""
+        self._flush()
+        return [self.download(c) for c in range(6)]
+
+    def sum_squares(self):
+        out = (ctypes.c_double * 6)()
+        _lib.check(self._L.fdtd_sum_squares(self._h, out))
+        return list(out)
+
+    def max_abs(self):
+        out = (ctypes.c_double * 6)()
+        _lib.check(self._L.fdtd_max_abs(self._h, out))
+        return list(out)
+
+    def compare(self, ref: "FDTD"):
+        """Per component (sum (t-r)^2, sum r^2, max|t-r|) against an FP64 grid."""
+        out = (ctypes.c_double * 18)()
+        _lib.check(self._L.fdtd_compare(self._h, ref._h, out))
+        return [tuple(out[3 * c:3 * c + 3]) for c in range(6)]
+
+    def set_scales(self, scales) -> None:
+        arr = (ctypes.c_double * 6)(*[float(s) for s in scales])
+        _lib.check(self._L.fdtd_set_scales(self._h, arr))
+
+    def get_scales(self):
+        arr = (ctypes.c_double * 6)()
+        _lib.check(self._L.fdtd_get_scales(self._h, arr))
+        return list(arr)
+
+    def autoscale(self, target_log2: int = 12, magnitude_floor: float = 0.0) -> None:
+        _lib.check(self._L.fdtd_autoscale(self._h, int(target_log2), float(magnitude_floor)))
+
+    def set_tuning(self, V=0, TY=0, kchunk=0) -> None:
+        _lib.check(self._L.fdtd_set_tuning(self._h, int(V), int(TY), int(kchunk)))
+
+    def get_tuning(self):
+        out = (ctypes.c_int * 3)()
+        _lib.check(self._L.fdtd_get_tuning(self._h, out))
+        return tuple(out)
+
+    def autotune(self, reps: int = 5):
+        out = (ctypes.c_int * 3)()
+        ms = ctypes.c_float()
+        _lib.check(self._L.fdtd_autotune(self._h, int(reps), out, ctypes.byref(ms)))
+        return tuple(out), ms.value
+
+    def time_update_fields(self, nsteps: int, path: int = _lib.PATH_FUSED):
+        """(ms_total, ms_main) of nsteps steps, HIP events on the grid's stream."""
+        self._flush()
+        a, b = ctypes.c_float(), ctypes.c_float()
+        _lib.check(self._L.fdtd_time_update_fields(self._h, int(nsteps), int(path),
+                                                   ctypes.byref(a), ctypes.byref(b)))
+        self._invalidate()
+        return a.value, b.value
+
+
+def calibration_copy(nbytes: int, iters: int = 10) -> float:
+    """Average ms of one streaming device copy of nbytes."""
+    ms = ctypes.c_float()
+    _lib.check(_lib.load().fdtd_calibration_copy(int(nbytes), int(iters), ctypes.byref(ms)))
+    return ms.value

@@ -1,0 +1,179 @@
+/*
+ * C-ABI of the MI355X (gfx950) Yee-step library, libfdtd_mi355x.so.
+ *
+ * This is the drop-in boundary for the reference's FDTD class API
+ * (include/FDTD/FDTD.h:8-41, include/FDTD_kokkos/FDTD_kokkos.h:7-36): plain
+ * C types, opaque handles, int status codes.  No exception and no C++ or
+ * torch type crosses it.  The C++ class FDTD_mi355x::FDTD (FDTD_mi355x.hpp)
+ * is the header-only counterpart of the reference class built on these
+ * entry points; Python binds them with ctypes (fdtd_method_amd/_lib.py).
+ *
+ * Layout: every field is a flat array of N = Ni*Nj*Nk cells, index
+ * i + j*Ni + k*Ni*Nj (src/FDTD/FDTD.cpp:66-78).  Host buffers are always
+ * FP64 (the reference's FP, include/FP.h:3); the device stores fields in the
+ * precision chosen at creation.
+ *
+ * Threading: one HIP stream per grid; calls on one grid are serialised by
+ * the caller.  Stepping is asynchronous; downloads and reductions
+ * synchronise the grid's stream.
+ */
+#ifndef FDTD_MI355X_H
+#define FDTD_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The messages of the first two match the reference's
+ * exceptions (src/FDTD/FDTD.cpp:6 and :149). */
+#define FDTD_OK 0
+#define FDTD_ERR_INVALID_PARAMETERS (-1) /* "ERROR: invalid parameters"       */
+#define FDTD_ERR_INVALID_COMPONENT (-2)  /* "ERROR: Invalid field component"  */
+#define FDTD_ERR_INVALID_ARGUMENT (-3)
+#define FDTD_ERR_HIP (-4)
+#define FDTD_ERR_UNSUPPORTED (-5)
+
+/* Component ordinals of include/Enums.h:5. */
+#define FDTD_EX 0
+#define FDTD_EY 1
+#define FDTD_EZ 2
+#define FDTD_BX 3
+#define FDTD_BY 4
+#define FDTD_BZ 5
+#define FDTD_JX 6
+#define FDTD_JY 7
+#define FDTD_JZ 8
+
+/* Device storage precision.  FP64 is the parity path; FP32 stores and
+ * computes in float; FP16 stores _Float16 with per-component power-of-two
+ * scales and computes in float. */
+#define FDTD_FP64 0
+#define FDTD_FP32 1
+#define FDTD_FP16 2
+
+/* Stepping paths for fdtd_time_update_fields. */
+#define FDTD_PATH_FUSED 0
+#define FDTD_PATH_3SWEEP 1
+
+/* include/Structures.h:23-38 (FDTD_struct::Parameters), same field order. */
+typedef struct fdtd_parameters {
+    int Ni, Nj, Nk;
+    double ax, bx, ay, by, az, bz;
+    double dx, dy, dz;
+} fdtd_parameters;
+
+/* A point current source on component comp (0=Jx, 1=Jy, 2=Jz) of cell
+ * (i,j,k), evaluated in FP64 on the device at t = (n+1)*dt before step n
+ * (the sampling of perf-tests/sample/sample.cpp:74):
+ *   kind 0: amp
+ *   kind 1: amp*sin(2*pi*(t-t0)/period + phase)*exp(-(t-t0)^2/(2*sigma^2))
+ *           (no envelope when sigma <= 0); pi = 3.14159265358 as in
+ *           include/Constants.h:14. */
+typedef struct fdtd_point_source {
+    int i, j, k, comp;
+    int kind, reserved;
+    double amp, period, t0, sigma, phase;
+} fdtd_point_source;
+
+typedef struct fdtd_grid fdtd_grid;   /* one Yee grid and its HIP stream */
+typedef struct fdtd_batch fdtd_batch; /* many grids stepped by one launch */
+
+/* Message of the last failing call on this host thread ("" if none). */
+const char* fdtd_last_error(void);
+
+/* FDTD(Parameters, FP dt) -- src/FDTD/FDTD.cpp:3-61,
+ * src/FDTD_kokkos/FDTD_kokkos.cpp:5-67.  Fails with
+ * FDTD_ERR_INVALID_PARAMETERS if any N <= 0 or dt <= 0.  All nine fields
+ * start at zero. */
+int fdtd_create(const fdtd_parameters* params, double dt, int precision, fdtd_grid** out);
+int fdtd_destroy(fdtd_grid* grid);
+
+/* Write side of get_field(Component) (src/FDTD/FDTD.cpp:138-151): copy a
+ * full FP64 host array into component `component` (0..8). count must be N. */
+int fdtd_upload(fdtd_grid* grid, int component, const double* host, size_t count);
+/* Read side of get_field(Component): FP64 copy of the current values. */
+int fdtd_download(fdtd_grid* grid, int component, double* host, size_t count);
+
+/* zeroed_currents() -- src/FDTD/FDTD.cpp:132-136: J := 0 (dense J and the
+ * values set by fdtd_set_current_values; point sources are kept). */
+int fdtd_zero_currents(fdtd_grid* grid);
+
+/* Sparse form of get_field(JX|JY|JZ)[index] = value (the drivers'
+ * per-step box source, perf-tests/sample/sample.cpp:67-81): sets J at n
+ * cells; values persist until changed or zeroed like the reference's J. */
+int fdtd_set_current_values(fdtd_grid* grid, int n, const long long* index,
+                            const int* component, const double* value);
+
+/* Replace the list of waveform point sources (evaluated on the device). */
+int fdtd_set_point_sources(fdtd_grid* grid, int n, const fdtd_point_source* sources);
+
+/* nsteps calls of update_fields() -- src/FDTD/FDTD.cpp:153-157,
+ * src/FDTD_kokkos/FDTD_kokkos.cpp:91-104 -- as one B half step plus
+ * nsteps fused E-then-double-half-B sweeps.  Bit-identical to the
+ * reference's B/2,E,B/2 sequence in FP64. */
+int fdtd_update_fields(fdtd_grid* grid, int nsteps);
+/* Same steps in the reference's structure: three in-place sweeps per step
+ * (update_B, update_E, update_B -- src/FDTD/FDTD.cpp:63-130). */
+int fdtd_update_fields_3sweep(fdtd_grid* grid, int nsteps);
+
+/* Steps completed so far (sources are evaluated at (step+1)*dt). */
+int fdtd_get_step(const fdtd_grid* grid, long long* step);
+int fdtd_set_step(fdtd_grid* grid, long long step);
+int fdtd_synchronize(fdtd_grid* grid);
+
+/* Grid facts: dims[3], precision, cells. */
+int fdtd_get_info(const fdtd_grid* grid, int dims[3], int* precision, long long* cells);
+
+/* FP64 diagnostics over E and B (Ex,Ey,Ez,Bx,By,Bz), in physical units:
+ * sum of squares (field energy is their total / (8 pi) * dx*dy*dz) and max |x|. */
+int fdtd_sum_squares(fdtd_grid* grid, double out[6]);
+int fdtd_max_abs(fdtd_grid* grid, double out[6]);
+/* Error of grid `test` (any precision) against FP64 grid `ref` of the same
+ * shape, per component c: out[3c] = sum (t-r)^2, out[3c+1] = sum r^2,
+ * out[3c+2] = max |t-r|. */
+int fdtd_compare(fdtd_grid* test, fdtd_grid* ref, double out[18]);
+
+/* FP16 storage scales (stored = scale * physical), powers of two; the
+ * current contents are rescaled exactly.  fdtd_autoscale picks for each
+ * component the largest power of two that keeps max|stored| <= 2^target_log2
+ * given an expected physical magnitude floor (used when a field is zero). */
+int fdtd_set_scales(fdtd_grid* grid, const double scales[6]);
+int fdtd_get_scales(const fdtd_grid* grid, double scales[6]);
+int fdtd_autoscale(fdtd_grid* grid, int target_log2, double magnitude_floor);
+
+/* Fused-sweep launch shape: V cells per lane along x, TY rows per tile,
+ * kchunk planes per z chunk (0 = keep current).  fdtd_autotune times the
+ * candidate shapes on scratch buffers and keeps the fastest. */
+int fdtd_set_tuning(fdtd_grid* grid, int V, int TY, int kchunk);
+int fdtd_get_tuning(const fdtd_grid* grid, int out[3]);
+int fdtd_autotune(fdtd_grid* grid, int reps, int out[3], float* best_ms);
+
+/* Timed run on the grid's stream with HIP events: ms_total covers the whole
+ * call (including the B half step of the fused path), ms_main only the
+ * nsteps sweep launches (fused) or step triples (3-sweep). */
+int fdtd_time_update_fields(fdtd_grid* grid, int nsteps, int path, float* ms_total, float* ms_main);
+
+/* Copy `bytes` (multiple of 16) device-to-device `iters` times with a
+ * streaming kernel; ms = average per copy.  Calibrates byte counters. */
+int fdtd_calibration_copy(size_t bytes, int iters, float* ms);
+
+/* ---- Batched solves: many independent grids advanced by one persistent
+ * launch per step.  All grids must share one precision. ---- */
+int fdtd_batch_create(int n, fdtd_grid* const* grids, fdtd_batch** out);
+int fdtd_batch_destroy(fdtd_batch* batch);
+/* nsteps steps of every grid (same semantics as fdtd_update_fields). */
+int fdtd_batch_update_fields(fdtd_batch* batch, int nsteps);
+/* Capture nsteps steps into a HIP graph once and replay it `replays` times. */
+int fdtd_batch_update_fields_graph(fdtd_batch* batch, int nsteps, int replays);
+int fdtd_batch_synchronize(fdtd_batch* batch);
+/* ms per step of the batched launch and of one launch per grid, timed with
+ * HIP events over nsteps steps each (fields advance by 2*nsteps). */
+int fdtd_batch_time(fdtd_batch* batch, int nsteps, float* ms_batched, float* ms_sequential);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDTD_MI355X_H */

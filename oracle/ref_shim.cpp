@@ -1,0 +1,61 @@
+// C entry points over the reference's own FDTD_openmp::FDTD class -- TEST
+// INFRASTRUCTURE ONLY.  Compiled together with the reference sources
+// src/FDTD/FDTD.cpp (read in place from the reference checkout) by
+// oracle/Makefile into oracle/_ref/libfdtd_ref.so, so that the oracle and the
+// golden fixtures can be checked against the reference's actual machine code.
+//
+// Wrapped interface: include/FDTD/FDTD.h:8-41
+//   FDTD(Parameters, FP dt)            -> ref_create
+//   Field& get_field(Component)        -> ref_get / ref_set (component ordinals of include/Enums.h:5)
+//   virtual void update_fields()       -> ref_update_fields (called nsteps times)
+//   void zeroed_currents()             -> ref_zeroed_currents
+#include <cstring>
+#include <exception>
+
+#include "FDTD.h"
+
+using FDTD_openmp::FDTD;
+
+extern "C" {
+
+void* ref_create(int Ni, int Nj, int Nk, double ax, double bx, double ay, double by,
+                 double az, double bz, double dx, double dy, double dz, double dt) {
+    FDTD_struct::Parameters p{Ni, Nj, Nk, ax, bx, ay, by, az, bz, dx, dy, dz};
+    try {
+        return new FDTD(p, dt);
+    } catch (const std::exception&) {
+        return nullptr;
+    }
+}
+
+void ref_destroy(void* h) { delete static_cast<FDTD*>(h); }
+
+int ref_set(void* h, int comp, const double* src, long long n) {
+    try {
+        auto& f = static_cast<FDTD*>(h)->get_field(static_cast<FDTD_enums::Component>(comp));
+        if ((long long)f.size() != n) return -2;
+        std::memcpy(f.data(), src, sizeof(double) * n);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_get(void* h, int comp, double* dst, long long n) {
+    try {
+        auto& f = static_cast<FDTD*>(h)->get_field(static_cast<FDTD_enums::Component>(comp));
+        if ((long long)f.size() != n) return -2;
+        std::memcpy(dst, f.data(), sizeof(double) * n);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+void ref_update_fields(void* h, int nsteps) {
+    for (int s = 0; s < nsteps; ++s) static_cast<FDTD*>(h)->update_fields();
+}
+
+void ref_zeroed_currents(void* h) { static_cast<FDTD*>(h)->zeroed_currents(); }
+
+}  // extern "C"
