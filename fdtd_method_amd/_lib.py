@@ -26,6 +26,7 @@ EXPORTED = [
     "fdtd_zero_currents", "fdtd_set_current_values", "fdtd_set_point_sources",
     "fdtd_update_fields", "fdtd_update_fields_3sweep", "fdtd_get_step", "fdtd_set_step",
     "fdtd_synchronize", "fdtd_get_info", "fdtd_sum_squares", "fdtd_max_abs", "fdtd_compare",
+    "fdtd_checksum",
     "fdtd_set_scales", "fdtd_get_scales", "fdtd_autoscale", "fdtd_set_tuning",
     "fdtd_get_tuning", "fdtd_autotune", "fdtd_time_update_fields", "fdtd_calibration_copy",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_update_fields",
@@ -91,6 +92,7 @@ def load():
         "fdtd_sum_squares": ([_vp, _dp], _i),
         "fdtd_max_abs": ([_vp, _dp], _i),
         "fdtd_compare": ([_vp, _vp, _dp], _i),
+        "fdtd_checksum": ([_vp, ctypes.POINTER(ctypes.c_ulonglong)], _i),
         "fdtd_set_scales": ([_vp, _dp], _i),
         "fdtd_get_scales": ([_vp, _dp], _i),
         "fdtd_autoscale": ([_vp, _i, ctypes.c_double], _i),

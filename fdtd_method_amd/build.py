@@ -51,6 +51,31 @@ def build_library(verbose: bool = False, extra_flags=None) -> str:
     return LIB_PATH
 
 
+BIN_DIR = os.path.join(PKG, "bin")
+CPP_TEST_DIR = os.path.join(ROOT, "tests", "cpp")
+CXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wno-unused-function"]
+
+
+def _gxx(src: str, out: str, rpath: str, verbose: bool) -> None:
+    cmd = ["g++", *CXX_FLAGS, f"-I{INCLUDE}", src, "-o", out, f"-L{PKG}", "-lfdtd_mi355x",
+           f"-Wl,-rpath,{rpath}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{res.stderr[-6000:]}")
+
+
+def build_cpp_tests(verbose: bool = False) -> None:
+    """Compile the host-side C++ programs that use the drop-in class:
+    the sample driver (fdtd_method_amd/bin/sample) and the C++ unit test
+    (tests/cpp/test_fdtd_mi355x).  Both link libfdtd_mi355x.so."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    _gxx(os.path.join(CSRC, "sample.cpp"), os.path.join(BIN_DIR, "sample"), "$ORIGIN/..", verbose)
+    _gxx(os.path.join(CPP_TEST_DIR, "test_fdtd_mi355x.cpp"),
+         os.path.join(CPP_TEST_DIR, "test_fdtd_mi355x"), "$ORIGIN/../../fdtd_method_amd", verbose)
+
+
 def build_oracle(verbose: bool = False) -> None:
     """Compile the CPU checker (oracle/liboracle.so) and, when the reference
     checkout exists, oracle/_ref/libfdtd_ref.so."""

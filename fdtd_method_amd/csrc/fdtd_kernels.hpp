@@ -231,7 +231,7 @@ __host__ __device__ inline size_t fused_lds_bytes(int TY, int Ni, int csize) {
 // Largest workgroup of the fused sweep for V cells of size csize per lane:
 // wide lanes hold more registers, so their workgroups are capped lower.
 __host__ __device__ constexpr int fused_max_threads(int V, int csize) {
-    return V * csize >= 32 ? 512 : 1024;
+    return V * csize >= 32 ? 512 : (csize == 8 && V == 2 ? 768 : 1024);
 }
 
 template <typename S, typename C, typename JT>
@@ -286,44 +286,54 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
 #pragma unroll
         for (int v = 0; v < V; ++v) { eP[d][v] = C(0); bP[d][v] = C(0); }
 
+    // Plane loads, software-pipelined one plane ahead.  Halo-row threads do
+    // not need Ey; its slot carries Bz of row j0-1 instead, and bm carries
+    // Bx of row j0-1.
+    C en[3][V], bn[3][V], bm[V];
+    auto load_plane = [&](int kw, C (&e)[3][V], C (&b)[3][V], C (&m)[V]) {
+        const int off = kw * plane + row_off;
+        load_pack<S, C, V>(a.in.b[0] + off, b[0]);
+        load_pack<S, C, V>(a.in.b[1] + off, b[1]);
+        load_pack<S, C, V>(a.in.b[2] + off, b[2]);
+        load_pack<S, C, V>(a.in.e[0] + off, e[0]);
+        load_pack<S, C, V>(a.in.e[2] + off, e[2]);
+        if (own) {
+            load_pack<S, C, V>(a.in.e[1] + off, e[1]);
+        } else {
+            const int offm = kw * plane + rowm_off;
+            load_pack<S, C, V>(a.in.b[2] + offm, e[1]);
+            load_pack<S, C, V>(a.in.b[0] + offm, m);
+        }
+    };
+
     if (active) {
         const int kp = (k0 == 0) ? Nk - 1 : k0 - 1;
         const int off = kp * plane + row_off;
         load_pack<S, C, V>(a.in.b[0] + off, bP[0]);
         load_pack<S, C, V>(a.in.b[1] + off, bP[1]);
+        load_plane(k0, en, bn, bm);
     }
 
     for (int kk = k0; kk <= k1; ++kk) {
         const int kw = (kk == Nk) ? 0 : kk;
         const int off = kw * plane + row_off;
-        C en[3][V], bn[3][V];
         if (active) {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                load_pack<S, C, V>(a.in.b[d] + off, bn[d]);
-                load_pack<S, C, V>(a.in.e[d] + off, en[d]);
-            }
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 if (own) {
                     sBz[(r + 1) * Ni + ib + v] = bn[2][v];
                     sBx[(r + 1) * Ni + ib + v] = bn[0][v];
+                } else {
+                    sBz[ib + v] = en[1][v];
+                    sBx[ib + v] = bm[v];
                 }
                 sBy[r * Ni + ib + v] = bn[1][v];
             }
-            if (halo) {
-                const int offm = kw * plane + rowm_off;
-                C mz[V], mx[V];
-                load_pack<S, C, V>(a.in.b[2] + offm, mz);
-                load_pack<S, C, V>(a.in.b[0] + offm, mx);
-#pragma unroll
-                for (int v = 0; v < V; ++v) {
-                    sBz[ib + v] = mz[v];
-                    sBx[ib + v] = mx[v];
-                }
-            }
         }
         __syncthreads();
+
+        C en2[3][V], bn2[3][V], bm2[V];
+        if (active && kk < k1) load_plane((kk + 1 == Nk) ? 0 : kk + 1, en2, bn2, bm2);
 
         C e1[3][V];
         if (active) {
@@ -404,7 +414,11 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
             for (int v = 0; v < V; ++v) {
                 eP[d][v] = e1[d][v];
                 bP[d][v] = bn[d][v];
+                en[d][v] = en2[d][v];
+                bn[d][v] = bn2[d][v];
             }
+#pragma unroll
+        for (int v = 0; v < V; ++v) bm[v] = bm2[v];
     }
 }
 
@@ -557,6 +571,45 @@ zero_bytes(uint4* __restrict__ p, size_t n16) {
 }
 
 __global__ void advance_step(long long* step, long long by) { *step += by; }
+
+// Order-independent checksum of the bit patterns of an array: the sum mod
+// 2^64 of a 64-bit mix of (bits, index).  Integer addition makes the result
+// independent of the reduction order, so equal arrays give equal sums.
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename S>
+__device__ __forceinline__ unsigned long long bits_of(S v) {
+    if constexpr (sizeof(S) == 8) return __builtin_bit_cast(unsigned long long, v);
+    else if constexpr (sizeof(S) == 4) return __builtin_bit_cast(unsigned int, v);
+    else return __builtin_bit_cast(unsigned short, v);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(kReduceThreads)
+checksum_partial(const S* __restrict__ x, size_t n, unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long sh[kReduceThreads];
+    unsigned long long acc = 0;
+    for (size_t i = blockIdx.x * static_cast<size_t>(kReduceThreads) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * kReduceThreads)
+        acc += mix64(bits_of(x[i]) ^ (static_cast<unsigned long long>(i) * 0xD6E8FEB86659FD93ull));
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ void __launch_bounds__(kReduceThreads)
+checksum_final(const unsigned long long* __restrict__ part, int nblk,
+               unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long sh[kReduceThreads];
+    unsigned long long acc = 0;
+    for (int b = threadIdx.x; b < nblk; b += kReduceThreads) acc += part[b];
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) *out = acc;
+}
 
 // Streaming copy of a known byte count, 16 B per lane: used to calibrate the
 // HBM byte counters against an exactly known traffic.

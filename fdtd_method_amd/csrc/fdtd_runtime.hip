@@ -152,37 +152,53 @@ bool shape_valid(const fdtd_grid* g, int V, int TY, int kchunk) {
     return true;
 }
 
+// z-chunk length for a tile height: the per-XCD tile count should fill
+// the XCD's 32 CUs in whole rounds (no tail), the grid should have at least
+// 256 tiles, and longer chunks recompute fewer boundary planes.
 int default_kchunk(int Nj, int Nk, int TY) {
     const int ntile_y = (Nj + TY - 1) / TY;
-    int nchunk = std::max(1, std::min(Nk, (1024 + ntile_y - 1) / ntile_y));
-    int kchunk = (Nk + nchunk - 1) / nchunk;
-    return std::max(1, kchunk);
+    int best_k = Nk;
+    double best_score = -1.0;
+    for (int k = std::min(Nk, 8); k <= Nk; ++k) {
+        const int nchunk = (Nk + k - 1) / k;
+        if (k != Nk && (nchunk - 1) * k >= Nk) continue;
+        const int ntiles = ntile_y * nchunk;
+        const int per_xcd = (ntiles + 7) / 8;
+        const int rounds = (per_xcd + 31) / 32;
+        const double fill = double(ntiles) / (rounds * 256.0);
+        const double score = fill * (1.0 - 1.0 / (k + 1.0));
+        if (score > best_score + 1e-9) {
+            best_score = score;
+            best_k = k;
+        }
+    }
+    return best_k;
 }
 
-// Largest TY that fits two workgroups per CU in LDS (else one), for the
-// widest V dividing Ni.
+// Widest V giving 16-byte lanes that divides Ni, then the largest TY with at
+// most 640 (FP32/FP16) or 768 (FP64) threads and 80 KiB of LDS per
+// workgroup; the autotuner refines this.
 bool default_tuning(const fdtd_grid* g, Tuning* t) {
     int vc[4];
     const int nv = v_candidates(g->precision, vc);
     const int cs = (int)compute_size(g->precision);
-    for (int q = 0; q < nv; ++q) {
-        const int V = vc[q];
-        if (g->Ni % V) continue;
-        const int lanes = g->Ni / V;
-        const int mt = max_threads(g, V);
-        if (2 * lanes > mt) continue;
-        const int tymax = std::min(g->Nj, mt / lanes - 1);
-        int best = 0;
-        for (int ty = tymax; ty >= 1; --ty)
-            if (fdtd::fused_lds_bytes(ty, g->Ni, cs) <= kLdsLimit / 2) { best = ty; break; }
-        if (!best)
-            for (int ty = tymax; ty >= 1; --ty)
-                if (fdtd::fused_lds_bytes(ty, g->Ni, cs) <= kLdsLimit) { best = ty; break; }
-        if (!best) continue;
-        t->V = V;
-        t->TY = best;
-        t->kchunk = default_kchunk(g->Nj, g->Nk, best);
-        return true;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int q = 0; q < nv; ++q) {
+            const int V = vc[q];
+            if (g->Ni % V) continue;
+            const int lanes = g->Ni / V;
+            const int cap = cs == 8 ? 768 : 640;
+            const int mt = pass == 0 ? std::min(cap, max_threads(g, V)) : max_threads(g, V);
+            const size_t lds_cap = pass == 0 ? kLdsLimit / 2 : kLdsLimit;
+            int best = 0;
+            for (int ty = std::min(g->Nj, mt / lanes - 1); ty >= 1; --ty)
+                if (fdtd::fused_lds_bytes(ty, g->Ni, cs) <= lds_cap) { best = ty; break; }
+            if (!best) continue;
+            t->V = V;
+            t->TY = best;
+            t->kchunk = default_kchunk(g->Nj, g->Nk, best);
+            return true;
+        }
     }
     return false;
 }
@@ -802,6 +818,34 @@ int fdtd_compare(fdtd_grid* t, fdtd_grid* r, double out[18]) {
                                t->stream));
     }
     HIP_TRY(hipStreamSynchronize(t->stream));
+    return FDTD_OK;
+}
+
+int fdtd_checksum(fdtd_grid* g, unsigned long long out[6]) {
+    if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    const int nb = std::min<int>(kReduceBlocks, grid_blocks(g->N));
+    auto* part = reinterpret_cast<unsigned long long*>(g->d_part);
+    auto* res = reinterpret_cast<unsigned long long*>(g->d_out);
+    for (int c = 0; c < 6; ++c) {
+        const void* p = field_ptr(g, c);
+        switch (g->precision) {
+            case FDTD_FP64:
+                hipLaunchKernelGGL(fdtd::checksum_partial<double>, dim3(nb), dim3(256), 0, g->stream,
+                                   static_cast<const double*>(p), g->N, part);
+                break;
+            case FDTD_FP32:
+                hipLaunchKernelGGL(fdtd::checksum_partial<float>, dim3(nb), dim3(256), 0, g->stream,
+                                   static_cast<const float*>(p), g->N, part);
+                break;
+            default:
+                hipLaunchKernelGGL(fdtd::checksum_partial<_Float16>, dim3(nb), dim3(256), 0,
+                                   g->stream, static_cast<const _Float16*>(p), g->N, part);
+        }
+        hipLaunchKernelGGL(fdtd::checksum_final, dim3(1), dim3(256), 0, g->stream, part, nb, res + c);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipMemcpyAsync(out, res, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
     return FDTD_OK;
 }
 

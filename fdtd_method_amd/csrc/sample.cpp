@@ -1,0 +1,134 @@
+// Counterpart of the reference's perf-test drivers
+// (perf-tests/sample/sample.cpp:14-168, perf-tests/kokkos_sample/kokkos_sample.cpp:16-185)
+// on the MI355X drop-in class: a spherical wave from the drivers' 8-cell
+// cos^2 box current, switched on for the first period/dt = 40 steps.
+//
+//   sample                 N = 32, 100 iterations
+//   sample N iters         as the reference
+//   sample N iters [fp64|fp32|fp16] [--dump FILE]
+//
+// Prints "Execution time: X s" and the 10x10 Ex slice around the centre of
+// plane Nk/2 in the reference's format (setw(12), fixed, setprecision(5)).
+// --dump writes Ex (N^3 doubles, index i + j*N + k*N*N) to FILE.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <string>
+
+#include "FDTD_mi355x.hpp"
+
+using namespace FDTD_mi355x;
+
+static int spherical_wave(int n, int it, Precision prec, const char* dump) {
+    // perf-tests/sample/sample.cpp:15-31
+    CurrentParameters cur_param{8, 4, 0.2};
+    const double T = cur_param.period;
+    const double Tx = cur_param.period_x, Ty = cur_param.period_y, Tz = cur_param.period_z;
+    cur_param.iterations = static_cast<int>(static_cast<double>(cur_param.period) / cur_param.dt);
+    auto current = [T, Tx, Ty, Tz](double x, double y, double z, double t) {
+        return std::sin(2.0 * FDTD_const::PI * t / T) *
+               std::pow(std::cos(2.0 * FDTD_const::PI * x / Tx), 2.0) *
+               std::pow(std::cos(2.0 * FDTD_const::PI * y / Ty), 2.0) *
+               std::pow(std::cos(2.0 * FDTD_const::PI * z / Tz), 2.0);
+    };
+
+    const double d = FDTD_const::C;
+    const double boundary = static_cast<double>(n) / 2.0 * d;
+    Parameters params{n, n, n, -boundary, boundary, -boundary, boundary, -boundary, boundary,
+                      d, d, d};
+    FDTD method(params, cur_param.dt, prec);
+
+    const int cur_time = std::min(cur_param.iterations, it);
+    const int start_i = static_cast<int>(std::floor((-Tx / 4.0 - params.ax) / params.dx));
+    const int start_j = static_cast<int>(std::floor((-Ty / 4.0 - params.ay) / params.dy));
+    const int start_k = static_cast<int>(std::floor((-Tz / 4.0 - params.az) / params.dz));
+    const int max_i = static_cast<int>(std::floor((Tx / 4.0 - params.ax) / params.dx));
+    const int max_j = static_cast<int>(std::floor((Ty / 4.0 - params.ay) / params.dy));
+    const int max_k = static_cast<int>(std::floor((Tz / 4.0 - params.az) / params.dz));
+
+    auto start = std::chrono::high_resolution_clock::now();
+    for (int t = 0; t < cur_time; ++t) {
+        for (int k = start_k; k < max_k; ++k)
+            for (int j = start_j; j < max_j; ++j)
+                for (int i = start_i; i < max_i; ++i) {
+                    const int index = i + j * params.Ni + k * params.Ni * params.Nj;
+                    const double value =
+                        current(static_cast<double>(i) * params.dx, static_cast<double>(j) * params.dy,
+                                static_cast<double>(k) * params.dz,
+                                static_cast<double>(t + 1) * cur_param.dt);
+                    method.get_field(Component::JX)[index] = value;
+                    method.get_field(Component::JY)[index] = value;
+                    method.get_field(Component::JZ)[index] = value;
+                }
+        method.update_fields();
+    }
+    method.zeroed_currents();
+    if (it > cur_time) method.update_fields(it - cur_time);
+    method.synchronize();
+    auto end = std::chrono::high_resolution_clock::now();
+    std::chrono::duration<double> elapsed = end - start;
+    std::cout << "Execution time: " << elapsed.count() << " s" << std::endl;
+
+    Field& Ex = method.get_field(Component::EX);
+    const int k = params.Nk / 2;
+    for (int j = params.Nj / 2 - 5; j < params.Nj / 2 + 5; j++) {
+        for (int i = params.Ni / 2 - 5; i < params.Ni / 2 + 5; i++) {
+            const int index = i + j * params.Ni + k * params.Ni * params.Nj;
+            std::cout << std::setw(12) << std::fixed << std::setprecision(5) << (double)Ex[index];
+        }
+        std::cout << std::endl;
+    }
+    std::cout << std::endl;
+
+    if (dump) {
+        std::ofstream out(dump, std::ios::binary);
+        const double* p = Ex.data();
+        out.write(reinterpret_cast<const char*>(p), sizeof(double) * method.cells());
+        if (!out) {
+            std::cerr << "ERROR: cannot write " << dump << std::endl;
+            return 1;
+        }
+    }
+    return 0;
+}
+
+int main(int argc, char* argv[]) {
+    int N = 32, iters = 100;
+    Precision prec = Precision::FP64;
+    const char* dump = nullptr;
+    int pos = 0;
+    for (int a = 1; a < argc; ++a) {
+        if (std::strcmp(argv[a], "--dump") == 0 && a + 1 < argc) {
+            dump = argv[++a];
+        } else if (std::strcmp(argv[a], "fp64") == 0) {
+            prec = Precision::FP64;
+        } else if (std::strcmp(argv[a], "fp32") == 0) {
+            prec = Precision::FP32;
+        } else if (std::strcmp(argv[a], "fp16") == 0) {
+            prec = Precision::FP16;
+        } else if (pos == 0) {
+            N = std::atoi(argv[a]);
+            ++pos;
+        } else if (pos == 1) {
+            iters = std::atoi(argv[a]);
+            ++pos;
+        } else {
+            pos = 3;
+        }
+    }
+    if (pos == 1 || pos > 2) {  // perf-tests/sample/sample.cpp:163-166
+        std::cout << "ERROR: Incorrect number of parameters" << std::endl;
+        return 1;
+    }
+    try {
+        return spherical_wave(N, iters, prec, dump);
+    } catch (const std::exception& e) {
+        std::cerr << e.what() << std::endl;
+        return 1;
+    }
+}

@@ -238,6 +238,12 @@ class FDTD:
         _lib.check(self._L.fdtd_compare(self._h, ref._h, out))
         return [tuple(out[3 * c:3 * c + 3]) for c in range(6)]
 
+    def checksum(self):
+        """Order-independent 64-bit checksums of the stored bits of Ex..Bz."""
+        out = (ctypes.c_ulonglong * 6)()
+        _lib.check(self._L.fdtd_checksum(self._h, out))
+        return list(out)
+
     def set_scales(self, scales) -> None:
         arr = (ctypes.c_double * 6)(*[float(s) for s in scales])
         _lib.check(self._L.fdtd_set_scales(self._h, arr))

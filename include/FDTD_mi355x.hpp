@@ -1,0 +1,247 @@
+// C++ drop-in for the reference's FDTD class, on libfdtd_mi355x.so.
+//
+// Replaces (same public surface, same error types and messages):
+//   FDTD_openmp::FDTD   include/FDTD/FDTD.h:8-41,  src/FDTD/FDTD.cpp:3-157
+//   FDTD_kokkos::FDTD   include/FDTD_kokkos/FDTD_kokkos.h:7-36, src/FDTD_kokkos/FDTD_kokkos.cpp:5-104
+// and the shared types of include/FP.h:3, include/Constants.h:4-12,
+// include/Enums.h:4-7, include/Structures.h:7-39.
+//
+// A caller switches by including this header instead of "FDTD.h" /
+// "FDTD_kokkos.h" and naming FDTD_mi355x::FDTD.  Differences a caller can see:
+//  * get_field() returns a host mirror (FDTD_mi355x::Field) of the device
+//    array.  f[idx] and f(idx) read (downloading lazily) and assign; E/B
+//    writes are uploaded before the next step, single-cell J writes are
+//    sent to the device as sparse currents (the drivers' box source,
+//    perf-tests/sample/sample.cpp:67-81).
+//  * The constructor takes an optional storage precision (FP64 default).
+//  * update_fields(int n) advances n steps in one call (fused sweeps).
+//  * Only single-threaded use per object, like the reference.
+//
+// Header-only: it needs the C-ABI (fdtd_mi355x.h) and links against
+// libfdtd_mi355x.so; no HIP header is required by callers.
+#pragma once
+
+#include <cstddef>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fdtd_mi355x.h"
+
+typedef double FP;  // include/FP.h:3
+
+namespace FDTD_const {  // include/Constants.h:4-12
+const double C = 3e10;
+const double R = 1e-12;
+const double EPS0 = 1.0;
+const double MU0 = EPS0;
+const double N = 4.0;
+const double PI = 3.14159265358;
+}  // namespace FDTD_const
+
+namespace FDTD_enums {  // include/Enums.h:4-7
+enum class Component { EX, EY, EZ, BX, BY, BZ, JX, JY, JZ };
+enum class Axis { X, Y, Z };
+}  // namespace FDTD_enums
+
+namespace FDTD_struct {  // include/Structures.h:7-39
+struct SelectedFields {
+    FDTD_enums::Component selected_E;
+    FDTD_enums::Component selected_B;
+};
+
+struct CurrentParameters {
+    int period;
+    int m;
+    FP dt;
+    int iterations;
+    FP period_x = static_cast<FP>(m) * FDTD_const::C;
+    FP period_y = static_cast<FP>(m) * FDTD_const::C;
+    FP period_z = static_cast<FP>(m) * FDTD_const::C;
+};
+
+struct Parameters {
+    int Ni;
+    int Nj;
+    int Nk;
+    FP ax;
+    FP bx;
+    FP ay;
+    FP by;
+    FP az;
+    FP bz;
+    FP dx;
+    FP dy;
+    FP dz;
+};
+}  // namespace FDTD_struct
+
+namespace FDTD_mi355x {
+
+using namespace FDTD_enums;
+using namespace FDTD_struct;
+
+enum class Precision { FP64 = FDTD_FP64, FP32 = FDTD_FP32, FP16 = FDTD_FP16 };
+
+// Converts a C-ABI status into the reference's exception types.
+inline void check_status(int rc) {
+    if (rc == FDTD_OK) return;
+    const std::string msg = fdtd_last_error();
+    if (rc == FDTD_ERR_INVALID_PARAMETERS) throw std::invalid_argument(msg);
+    if (rc == FDTD_ERR_INVALID_COMPONENT) throw std::logic_error(msg);
+    throw std::runtime_error(msg);
+}
+
+class FDTD;
+
+// Host mirror of one device field (the reference's Field&).
+class Field {
+public:
+    class Ref {
+    public:
+        Ref(Field* f, std::size_t i) : f_(f), i_(i) {}
+        operator double() const { return f_->get(i_); }
+        Ref& operator=(double v) {
+            f_->set(i_, v);
+            return *this;
+        }
+        Ref& operator=(const Ref& o) { return *this = static_cast<double>(o); }
+        Ref& operator+=(double v) { return *this = static_cast<double>(*this) + v; }
+        Ref& operator-=(double v) { return *this = static_cast<double>(*this) - v; }
+        Ref& operator*=(double v) { return *this = static_cast<double>(*this) * v; }
+
+    private:
+        Field* f_;
+        std::size_t i_;
+    };
+
+    Field(FDTD* owner, int comp) : owner_(owner), comp_(comp) {}
+
+    Ref operator[](std::size_t i) { return Ref(this, i); }
+    Ref operator()(std::size_t i) { return Ref(this, i); }
+    double operator[](std::size_t i) const { return const_cast<Field*>(this)->get(i); }
+    double operator()(std::size_t i) const { return const_cast<Field*>(this)->get(i); }
+    std::size_t size() const;
+
+    // Mirror pointer for bulk access; the field is treated as modified.
+    double* data() {
+        pull();
+        dirty_ = true;
+        return host_.data();
+    }
+
+    inline double get(std::size_t i);
+    inline void set(std::size_t i, double v);
+    inline void pull();
+    inline void push();
+    void invalidate() {
+        if (!dirty_) valid_ = false;
+    }
+    void drop() {
+        valid_ = false;
+        dirty_ = false;
+    }
+
+private:
+    FDTD* owner_;
+    int comp_;
+    std::vector<double> host_;
+    bool valid_ = false;
+    bool dirty_ = false;
+};
+
+class FDTD {
+public:
+    // FDTD(Parameters, FP dt) -- src/FDTD/FDTD.cpp:3-61.
+    FDTD(Parameters parameters, FP dt, Precision precision = Precision::FP64)
+        : parameters_(parameters), dt_(dt) {
+        fdtd_parameters p{parameters.Ni, parameters.Nj, parameters.Nk, parameters.ax,
+                          parameters.bx, parameters.ay, parameters.by, parameters.az,
+                          parameters.bz, parameters.dx, parameters.dy, parameters.dz};
+        check_status(fdtd_create(&p, dt, static_cast<int>(precision), &grid_));
+        cells_ = static_cast<std::size_t>(parameters.Ni) * parameters.Nj * parameters.Nk;
+        fields_.reserve(9);
+        for (int c = 0; c < 9; ++c) fields_.emplace_back(this, c);
+    }
+    FDTD(const FDTD&) = delete;
+    FDTD& operator=(const FDTD&) = delete;
+    virtual ~FDTD() { fdtd_destroy(grid_); }
+
+    // Field& get_field(Component) -- src/FDTD/FDTD.cpp:138-151.
+    Field& get_field(Component c) {
+        const int id = static_cast<int>(c);
+        if (id < 0 || id > 8) throw std::logic_error("ERROR: Invalid field component");
+        return fields_[id];
+    }
+
+    // virtual void update_fields() -- src/FDTD/FDTD.cpp:153-157.
+    virtual void update_fields() { update_fields(1); }
+
+    // n steps in one call.
+    void update_fields(int nsteps) {
+        flush();
+        check_status(fdtd_update_fields(grid_, nsteps));
+        for (int c = 0; c < 6; ++c) fields_[c].invalidate();
+    }
+
+    // void zeroed_currents() -- src/FDTD/FDTD.cpp:132-136.
+    void zeroed_currents() {
+        for (int c = 6; c < 9; ++c) fields_[c].drop();
+        check_status(fdtd_zero_currents(grid_));
+    }
+
+    std::size_t cells() const { return cells_; }
+    fdtd_grid* handle() { return grid_; }
+    const Parameters& parameters() const { return parameters_; }
+    FP dt() const { return dt_; }
+
+    void synchronize() { check_status(fdtd_synchronize(grid_)); }
+
+private:
+    friend class Field;
+
+    void flush() {
+        for (auto& f : fields_) f.push();
+    }
+
+    Parameters parameters_;
+    FP dt_;
+    fdtd_grid* grid_ = nullptr;
+    std::size_t cells_ = 0;
+    std::vector<Field> fields_;
+};
+
+inline std::size_t Field::size() const { return owner_->cells(); }
+
+inline void Field::pull() {
+    if (valid_) return;
+    host_.resize(owner_->cells());
+    check_status(fdtd_download(owner_->handle(), comp_, host_.data(), host_.size()));
+    valid_ = true;
+}
+
+inline void Field::push() {
+    if (!dirty_) return;
+    check_status(fdtd_upload(owner_->handle(), comp_, host_.data(), host_.size()));
+    dirty_ = false;
+}
+
+inline double Field::get(std::size_t i) {
+    pull();
+    return host_.at(i);
+}
+
+inline void Field::set(std::size_t i, double v) {
+    if (comp_ >= FDTD_JX && !dirty_) {
+        if (i >= owner_->cells()) throw std::out_of_range("field index out of range");
+        const long long idx = static_cast<long long>(i);
+        check_status(fdtd_set_current_values(owner_->handle(), 1, &idx, &comp_, &v));
+        if (valid_) host_[i] = v;
+        return;
+    }
+    pull();
+    host_.at(i) = v;
+    dirty_ = true;
+}
+
+}  // namespace FDTD_mi355x

@@ -135,6 +135,11 @@ int fdtd_max_abs(fdtd_grid* grid, double out[6]);
  * out[3c+2] = max |t-r|. */
 int fdtd_compare(fdtd_grid* test, fdtd_grid* ref, double out[18]);
 
+/* Order-independent 64-bit checksum of the stored bit patterns of each of
+ * Ex..Bz (sum mod 2^64 of a hash of (bits, index)): equal arrays give equal
+ * checksums, so bit-exactness can be checked on the device at any size. */
+int fdtd_checksum(fdtd_grid* grid, unsigned long long out[6]);
+
 /* FP16 storage scales (stored = scale * physical), powers of two; the
  * current contents are rescaled exactly.  fdtd_autoscale picks for each
  * component the largest power of two that keeps max|stored| <= 2^target_log2

@@ -79,6 +79,8 @@ def ref_lib():
         L.ref_destroy.argtypes = [ctypes.c_void_p]
         L.ref_set.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_longlong]
         L.ref_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_longlong]
+        L.ref_set_value.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong,
+                                    ctypes.c_double]
         L.ref_update_fields.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ref_zeroed_currents.argtypes = [ctypes.c_void_p]
         _ref = L
@@ -184,8 +186,119 @@ class RefGrid:
             raise ValueError(f"ref_get failed rc={rc}")
         return out
 
+    def set_value(self, comp, idx, value):
+        rc = ref_lib().ref_set_value(self.h, comp, idx, value)
+        if rc != 0:
+            raise ValueError(f"ref_set_value failed rc={rc}")
+
     def update_fields(self, nsteps=1):
         ref_lib().ref_update_fields(self.h, nsteps)
 
     def zeroed_currents(self):
         ref_lib().ref_zeroed_currents(self.h)
+
+
+# ---------------------------------------------------------------------------
+# Independent numpy restatement (pins the C oracle's wrap and indexing).
+#
+# Arrays are viewed as [k, j, i] (index = i + j*Ni + k*Ni*Nj); the periodic
+# neighbours of include/FDTD/FDTD.h:24-31 become np.roll along an axis.  The
+# expression order is the reference's, so results are bit-identical to a
+# correct C implementation (numpy does no FMA contraction).
+
+def roll_update_fields(E, B, J, shape, coefs, nsteps=1):
+    """nsteps of B/2, E, B/2 on float64 arrays (lists of 3 flat arrays, in place)."""
+    Ni, Nj, Nk = shape
+    cEx, cEy, cEz, cBx, cBy, cBz, cur = coefs
+    v = lambda a: a.reshape(Nk, Nj, Ni)  # noqa: E731
+    Ex, Ey, Ez = (v(a) for a in E)
+    Bx, By, Bz = (v(a) for a in B)
+    Jx, Jy, Jz = (v(a) for a in J)
+    nxt = lambda a, ax: np.roll(a, -1, axis=ax)  # noqa: E731  a[.. +1 ..]
+    prv = lambda a, ax: np.roll(a, 1, axis=ax)  # noqa: E731   a[.. -1 ..]
+    I_, J_, K_ = 2, 1, 0
+
+    def half_b():
+        dbx = cBz * (nxt(Ey, K_) - Ey) - cBy * (nxt(Ez, J_) - Ez)
+        dby = cBx * (nxt(Ez, I_) - Ez) - cBz * (nxt(Ex, K_) - Ex)
+        dbz = cBy * (nxt(Ex, J_) - Ex) - cBx * (nxt(Ey, I_) - Ey)
+        Bx[...] = Bx + dbx
+        By[...] = By + dby
+        Bz[...] = Bz + dbz
+
+    for _ in range(nsteps):
+        half_b()
+        ex = Ex + ((cur * Jx + cEy * (Bz - prv(Bz, J_))) - cEz * (By - prv(By, K_)))
+        ey = Ey + ((cur * Jy + cEz * (Bx - prv(Bx, K_))) - cEx * (Bz - prv(Bz, I_)))
+        ez = Ez + ((cur * Jz + cEx * (By - prv(By, I_))) - cEy * (Bx - prv(Bx, J_)))
+        Ex[...] = ex
+        Ey[...] = ey
+        Ez[...] = ez
+        half_b()
+
+
+# ---------------------------------------------------------------------------
+# The reference drivers' spherical-wave harness (perf-tests/sample/sample.cpp:14-91)
+
+_libm = None
+
+
+def _m():
+    global _libm
+    if _libm is None:
+        _libm = ctypes.CDLL("libm.so.6")
+        for fn in ("sin", "cos"):
+            getattr(_libm, fn).argtypes = [ctypes.c_double]
+            getattr(_libm, fn).restype = ctypes.c_double
+        _libm.pow.argtypes = [ctypes.c_double, ctypes.c_double]
+        _libm.pow.restype = ctypes.c_double
+    return _libm
+
+
+def sample_current(x, y, z, t, T, Tx, Ty, Tz):
+    """cur_func of sample.cpp:25-31, evaluated with the C library's libm."""
+    m = _m()
+    return (m.sin(2.0 * PI * t / T) * m.pow(m.cos(2.0 * PI * x / Tx), 2.0)
+            * m.pow(m.cos(2.0 * PI * y / Ty), 2.0) * m.pow(m.cos(2.0 * PI * z / Tz), 2.0))
+
+
+def sample_box(n):
+    """Current box, parameters and dt of spherical_wave(n, it): sample.cpp:15-63."""
+    period, mm, dt = 8, 4, 0.2
+    T = float(period)
+    Tx = Ty = Tz = float(mm) * C
+    iterations = int(float(period) / dt)
+    d = C
+    boundary = float(n) / 2.0 * d
+    ax = -boundary
+    start = int(np.floor((-Tx / 4.0 - ax) / d))
+    stop = int(np.floor((Tx / 4.0 - ax) / d))
+    return dict(T=T, Tx=Tx, Ty=Ty, Tz=Tz, iterations=iterations, d=d, dt=dt, start=start,
+                stop=stop, boundary=boundary)
+
+
+def spherical_wave(n, it):
+    """Run the harness on the oracle; returns [Ex..Bz] after `it` steps.
+
+    J is written to Jx = Jy = Jz on the box before each of the first
+    min(40, it) steps, then zeroed -- the pattern for which the reference's
+    OpenMP and Kokkos E updates coincide."""
+    p = sample_box(n)
+    g = OracleGrid(n, n, n, p["d"], p["d"], p["d"], p["dt"])
+    cur_time = min(p["iterations"], it)
+    for t in range(cur_time):
+        for k in range(p["start"], p["stop"]):
+            for j in range(p["start"], p["stop"]):
+                for i in range(p["start"], p["stop"]):
+                    idx = i + j * n + k * n * n
+                    val = sample_current(float(i) * p["d"], float(j) * p["d"], float(k) * p["d"],
+                                         float(t + 1) * p["dt"], p["T"], p["Tx"], p["Ty"], p["Tz"])
+                    g.J[0][idx] = val
+                    g.J[1][idx] = val
+                    g.J[2][idx] = val
+        g.update_fields(1)
+    for a in g.J:
+        a[:] = 0.0
+    if it > cur_time:
+        g.update_fields(it - cur_time)
+    return g.E + g.B

@@ -6,7 +6,8 @@
 //
 // Wrapped interface: include/FDTD/FDTD.h:8-41
 //   FDTD(Parameters, FP dt)            -> ref_create
-//   Field& get_field(Component)        -> ref_get / ref_set (component ordinals of include/Enums.h:5)
+//   Field& get_field(Component)        -> ref_get / ref_set / ref_set_value
+//                                         (component ordinals of include/Enums.h:5)
 //   virtual void update_fields()       -> ref_update_fields (called nsteps times)
 //   void zeroed_currents()             -> ref_zeroed_currents
 #include <cstring>
@@ -46,6 +47,17 @@ int ref_get(void* h, int comp, double* dst, long long n) {
         auto& f = static_cast<FDTD*>(h)->get_field(static_cast<FDTD_enums::Component>(comp));
         if ((long long)f.size() != n) return -2;
         std::memcpy(dst, f.data(), sizeof(double) * n);
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+int ref_set_value(void* h, int comp, long long idx, double value) {
+    try {
+        auto& f = static_cast<FDTD*>(h)->get_field(static_cast<FDTD_enums::Component>(comp));
+        if (idx < 0 || idx >= (long long)f.size()) return -2;
+        f[idx] = value;
         return 0;
     } catch (const std::exception&) {
         return -1;

@@ -1,0 +1,232 @@
+// Unit tests of the C++ drop-in class FDTD_mi355x::FDTD, restating the
+// reference's gtest suite (unit-tests/test_FDTD_method.cpp:71-203 and
+// unit-tests/test_FDTD_method_kokkos.cpp:71-201): second-order convergence
+// of a plane wave along each axis, err(N)/err(2N) = 4 +- 0.1, plus the
+// class's error behaviour (src/FDTD/FDTD.cpp:5-7, :149).
+//
+// Usage: test_fdtd_mi355x [fp64|fp32]   (exit status 0 = all passed)
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "FDTD_mi355x.hpp"
+
+using namespace FDTD_mi355x;
+
+namespace {
+
+int g_failures = 0;
+Precision g_precision = Precision::FP64;
+
+#define EXPECT_TRUE(cond)                                                        \
+    do {                                                                         \
+        if (!(cond)) {                                                           \
+            std::printf("  FAILED %s:%d: %s\n", __FILE__, __LINE__, #cond);      \
+            ++g_failures;                                                        \
+        }                                                                        \
+    } while (0)
+
+#define EXPECT_NEAR(a, b, tol)                                                             \
+    do {                                                                                   \
+        const double a_ = (a), b_ = (b);                                                   \
+        if (!(std::fabs(a_ - b_) <= (tol))) {                                              \
+            std::printf("  FAILED %s:%d: %s = %.6f, expected %.6f +- %g\n", __FILE__,      \
+                        __LINE__, #a, a_, b_, (double)(tol));                              \
+            ++g_failures;                                                                  \
+        }                                                                                  \
+    } while (0)
+
+const FP kTime = 5e-13;  // unit-tests/test_FDTD_method.cpp:7
+
+// Initial profile and exact solution of the travelling wave
+// (unit-tests/test_FDTD_method.cpp:9-16).
+FP initial(FP x, FP a, FP b) { return std::sin(2.0 * FDTD_const::PI * (x - a) / (b - a)); }
+FP exact(FP x, FP t, FP a, FP b) {
+    return std::sin(2.0 * FDTD_const::PI * (x - a - FDTD_const::C * t) / (b - a));
+}
+
+// Propagation axis and sign of a (E, B) pair: the wave moves toward +axis
+// when E = sign * f and B = f (src/FDTD/test_FDTD.cpp:53-80).
+struct Wave {
+    Axis axis;
+    FP sign;
+};
+
+Wave wave_of(Component e, Component b) {
+    if (e == Component::EY && b == Component::BZ) return {Axis::X, 1.0};
+    if (e == Component::EZ && b == Component::BY) return {Axis::X, -1.0};
+    if (e == Component::EX && b == Component::BZ) return {Axis::Y, -1.0};
+    if (e == Component::EZ && b == Component::BX) return {Axis::Y, 1.0};
+    if (e == Component::EX && b == Component::BY) return {Axis::Z, 1.0};
+    if (e == Component::EY && b == Component::BX) return {Axis::Z, -1.0};
+    throw std::logic_error("ERROR: invalid selected fields");
+}
+
+// Fill E at grid points and B half a cell further along the axis
+// (src/FDTD/test_FDTD.cpp:5-51), writing through get_field()[index].
+void fill(FDTD& m, const Parameters& p, SelectedFields sel) {
+    const Wave w = wave_of(sel.selected_E, sel.selected_B);
+    Field& E = m.get_field(sel.selected_E);
+    Field& B = m.get_field(sel.selected_B);
+    for (int k = 0; k < p.Nk; ++k)
+        for (int j = 0; j < p.Nj; ++j)
+            for (int i = 0; i < p.Ni; ++i) {
+                const int index = i + j * p.Ni + k * p.Ni * p.Nj;
+                FP x, d, a, b;
+                if (w.axis == Axis::X) { x = i * p.dx; d = p.dx; a = p.ax; b = p.bx; }
+                else if (w.axis == Axis::Y) { x = j * p.dy; d = p.dy; a = p.ay; b = p.by; }
+                else { x = k * p.dz; d = p.dz; a = p.az; b = p.bz; }
+                E[index] = w.sign * initial(x, a, b);
+                B[index] = initial(x + d / 2.0, a, b);
+            }
+}
+
+// Max |error| along the grid line through the origin
+// (src/FDTD/test_FDTD.cpp:89-130); B is sampled half a cell along the axis.
+FP line_error(FDTD& m, const Parameters& p, SelectedFields sel, Component c, FP t) {
+    const Wave w = wave_of(sel.selected_E, sel.selected_B);
+    const bool is_b = static_cast<int>(c) > static_cast<int>(Component::EZ);
+    const FP sign = is_b ? 1.0 : w.sign;
+    Field& f = m.get_field(c);
+    FP err = 0.0;
+    const int n = w.axis == Axis::X ? p.Ni : (w.axis == Axis::Y ? p.Nj : p.Nk);
+    const FP d = w.axis == Axis::X ? p.dx : (w.axis == Axis::Y ? p.dy : p.dz);
+    const FP a = w.axis == Axis::X ? p.ax : (w.axis == Axis::Y ? p.ay : p.az);
+    const FP b = w.axis == Axis::X ? p.bx : (w.axis == Axis::Y ? p.by : p.bz);
+    FP x = is_b ? d / 2.0 : 0.0;
+    for (int q = 0; q < n; ++q, x += d) {
+        const int index = w.axis == Axis::X ? q : (w.axis == Axis::Y ? q * p.Ni : q * p.Ni * p.Nj);
+        err = std::max(err, std::fabs(sign * f[index] - exact(x, t, a, b)));
+    }
+    return err;
+}
+
+FP run_once(Component c, SelectedFields sel, int Ni, int Nj, int Nk, int iters) {
+    Parameters p{Ni, Nj, Nk, 0.0, 1.0, 0.0, 2.0, 0.0, 3.0,
+                 1.0 / Ni, 2.0 / Nj, 3.0 / Nk};
+    FDTD m(p, kTime / iters, g_precision);
+    fill(m, p, sel);
+    for (int t = 0; t < iters; ++t) m.update_fields();
+    return line_error(m, p, sel, c, kTime);
+}
+
+// unit-tests/test_FDTD_method.cpp:18-69: ratio of errors at N (16 steps)
+// and 2N (32 steps).
+FP convergence_ratio(Component c, SelectedFields sel, int Ni, int Nj, int Nk) {
+    return run_once(c, sel, Ni, Nj, Nk, 16) / run_once(c, sel, 2 * Ni, 2 * Nj, 2 * Nk, 32);
+}
+
+struct Case {
+    const char* name;
+    Component test, e, b;
+    int Ni, Nj, Nk;
+};
+
+const Case kCases[] = {
+    {"x_axis_EY", Component::EY, Component::EY, Component::BZ, 16, 8, 4},
+    {"x_axis_BZ", Component::BZ, Component::EY, Component::BZ, 16, 8, 4},
+    {"x_axis_EZ", Component::EZ, Component::EZ, Component::BY, 16, 8, 4},
+    {"x_axis_BY", Component::BY, Component::EZ, Component::BY, 16, 8, 4},
+    {"y_axis_EX", Component::EX, Component::EX, Component::BZ, 8, 16, 4},
+    {"y_axis_BZ", Component::BZ, Component::EX, Component::BZ, 8, 16, 4},
+    {"y_axis_EZ", Component::EZ, Component::EZ, Component::BX, 8, 16, 4},
+    {"y_axis_BX", Component::BX, Component::EZ, Component::BX, 8, 16, 4},
+    {"z_axis_EX", Component::EX, Component::EX, Component::BY, 4, 8, 16},
+    {"z_axis_BY", Component::BY, Component::EX, Component::BY, 4, 8, 16},
+    {"z_axis_EY", Component::EY, Component::EY, Component::BX, 4, 8, 16},
+    {"z_axis_BX", Component::BX, Component::EY, Component::BX, 4, 8, 16},
+};
+
+void test_errors() {
+    Parameters bad{0, 4, 4, 0, 1, 0, 1, 0, 1, 1, 1, 1};
+    bool thrown = false;
+    try {
+        FDTD m(bad, 1e-12, g_precision);
+    } catch (const std::invalid_argument& e) {
+        thrown = std::string(e.what()) == "ERROR: invalid parameters";
+    }
+    EXPECT_TRUE(thrown);
+    Parameters ok{4, 4, 4, 0, 1, 0, 1, 0, 1, 1, 1, 1};
+    thrown = false;
+    try {
+        FDTD m(ok, -1.0, g_precision);
+    } catch (const std::invalid_argument&) {
+        thrown = true;
+    }
+    EXPECT_TRUE(thrown);
+    FDTD m(ok, 1e-12, g_precision);
+    thrown = false;
+    try {
+        m.get_field(static_cast<Component>(9));
+    } catch (const std::logic_error& e) {
+        thrown = std::string(e.what()) == "ERROR: Invalid field component";
+    }
+    EXPECT_TRUE(thrown);
+}
+
+void test_currents() {
+    // J writes read back, persist across steps, and zeroed_currents clears them.
+    Parameters p{8, 6, 4, 0, 1, 0, 1, 0, 1, 1, 1, 1};
+    FDTD m(p, 1e-12, g_precision);
+    m.get_field(Component::JZ)[17] = 0.5;
+    m.get_field(Component::JX)(3) = -2.0;
+    EXPECT_TRUE(m.get_field(Component::JZ)[17] == 0.5);
+    EXPECT_TRUE(m.get_field(Component::JX)[3] == -2.0);
+    m.update_fields();
+    EXPECT_TRUE(m.get_field(Component::JZ)[17] == 0.5);
+    // Ez at the source cell after one step from zero: cur * J exactly.
+    const FP cur = -4.0 * FDTD_const::PI * 1e-12;
+    if (g_precision == Precision::FP64) EXPECT_TRUE(m.get_field(Component::EZ)[17] == cur * 0.5);
+    m.zeroed_currents();
+    EXPECT_TRUE(m.get_field(Component::JZ)[17] == 0.0);
+    EXPECT_TRUE(m.get_field(Component::JX)[3] == 0.0);
+}
+
+void test_multi_step_equivalence() {
+    // update_fields(n) == n calls of update_fields().
+    Parameters p{16, 12, 10, 0, 1, 0, 1, 0, 1, 1.0, 1.1, 0.9};
+    FDTD a(p, 5e-12, g_precision), b(p, 5e-12, g_precision);
+    for (int c = 0; c < 6; ++c) {
+        Field& fa = a.get_field(static_cast<Component>(c));
+        Field& fb = b.get_field(static_cast<Component>(c));
+        for (int i = 0; i < (int)a.cells(); ++i) {
+            const double v = std::sin(0.37 * i + c) * (c < 3 ? 1.0 : 0.5);
+            fa[i] = v;
+            fb[i] = v;
+        }
+    }
+    for (int s = 0; s < 5; ++s) a.update_fields();
+    b.update_fields(5);
+    bool same = true;
+    for (int c = 0; c < 6; ++c)
+        for (int i = 0; i < (int)a.cells(); ++i)
+            same = same && (double)a.get_field(static_cast<Component>(c))[i] ==
+                               (double)b.get_field(static_cast<Component>(c))[i];
+    EXPECT_TRUE(same);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc > 1 && std::strcmp(argv[1], "fp32") == 0) g_precision = Precision::FP32;
+    const char* prec_name = g_precision == Precision::FP64 ? "fp64" : "fp32";
+    int ran = 0;
+    for (const Case& c : kCases) {
+        const int before = g_failures;
+        const FP ratio = convergence_ratio(c.test, SelectedFields{c.e, c.b}, c.Ni, c.Nj, c.Nk);
+        EXPECT_NEAR(ratio, 4.0, 0.1);
+        std::printf("[%s] Convergence.%s ratio=%.4f %s\n", prec_name, c.name, ratio,
+                    g_failures == before ? "OK" : "FAIL");
+        ++ran;
+    }
+    test_errors();
+    test_currents();
+    test_multi_step_equivalence();
+    ran += 3;
+    std::printf("%d tests, %d failures\n", ran, g_failures);
+    return g_failures == 0 ? 0 : 1;
+}

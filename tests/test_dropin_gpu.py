@@ -1,0 +1,67 @@
+"""The reference's unit tests and driver, restated against the drop-in API.
+
+* unit-tests/test_FDTD_method.cpp:71-203 (and the Kokkos copy): 12
+  convergence-order assertions, err(N)/err(2N) = 4 +- 0.1, on the Python
+  mirror and on the C++ class FDTD_mi355x::FDTD (tests/cpp).
+* perf-tests/sample/sample.cpp: the C++ sample driver on the drop-in class,
+  its Ex dump bit-identical to the reference class's run (golden SHA-256).
+"""
+import hashlib
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from fdtd_method_amd import FDTD, Parameters, Precision
+from test_oracle import CONVERGENCE, run_convergence
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_stepper(precision):
+    def step(shape, d, dt, st, iters):
+        g = FDTD(Parameters(*shape, 0.0, 1.0, 0.0, 2.0, 0.0, 3.0, *d), dt, precision)
+        for c in range(6):
+            g.upload(c, st[c])
+        for _ in range(iters):
+            g.update_fields()  # one reference update_fields() per call
+        out = g.fields()
+        g.close()
+        return out
+    return step
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.FP32])
+@pytest.mark.parametrize("name,comp,e,b,shape", CONVERGENCE, ids=[c[0] for c in CONVERGENCE])
+def test_convergence(name, comp, e, b, shape, precision):
+    ratio = run_convergence(gpu_stepper(precision), comp, e, b, shape)
+    assert abs(ratio - 4.0) <= 0.1, ratio
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_cpp_unit_tests(prec):
+    exe = os.path.join(ROOT, "tests", "cpp", "test_fdtd_mi355x")
+    res = subprocess.run([exe, prec], capture_output=True, text=True, timeout=300)
+    print(res.stdout[-3000:])
+    assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-2000:]
+    assert "15 tests, 0 failures" in res.stdout
+
+
+def test_cpp_sample_driver_matches_reference(tmp_path, golden_dir):
+    rec = json.load(open(os.path.join(golden_dir, "ref_sample_32_100.json")))
+    exe = os.path.join(ROOT, "fdtd_method_amd", "bin", "sample")
+    dump = tmp_path / "ex.bin"
+    res = subprocess.run([exe, str(rec["n"]), str(rec["iterations"]), "--dump", str(dump)],
+                         capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    assert res.stdout.startswith("Execution time: ")
+    ex = np.fromfile(dump, dtype="<f8")
+    assert hashlib.sha256(ex.tobytes()).hexdigest() == rec["sha256"][0]
+    lines = res.stdout.strip().splitlines()[1:11]
+    printed = [[float(x) for x in ln.split()] for ln in lines]
+    np.testing.assert_allclose(printed, rec["ex_slice"], atol=5e-6)
+    bad = subprocess.run([exe, "32"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "ERROR: Incorrect number of parameters" in bad.stdout

@@ -1,0 +1,61 @@
+"""Library checks that need no GPU: the C-ABI library loads, exports every
+symbol include/fdtd_mi355x.h declares, and rejects invalid parameters with
+the reference's messages before touching the device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+from fdtd_method_amd import FDTD, Parameters, _lib
+
+HEADER = os.path.join(ROOT, "include", "fdtd_mi355x.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fdtd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_what_the_bindings_list():
+    assert declared_functions() == sorted(_lib.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    missing = [f for f in declared_functions() if not hasattr(L, f)]
+    assert not missing, missing
+
+
+def test_library_is_gfx950_code():
+    """The shared object carries a gfx950 code object (hipcc --offload-arch=gfx950)."""
+    data = open(_lib.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_invalid_parameters_without_gpu():
+    """src/FDTD/FDTD.cpp:5-7: N <= 0 or dt <= 0 -> invalid_argument."""
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    for dims, dt in (((0, 4, 4), 1e-12), ((4, -1, 4), 1e-12), ((4, 4, 4), 0.0), ((4, 4, 4), -1.0)):
+        p = _lib.Parameters(*dims, 0, 1, 0, 1, 0, 1, 1, 1, 1)
+        rc = L.fdtd_create(ctypes.byref(p), dt, 0, ctypes.byref(h))
+        assert rc == _lib.ERR_INVALID_PARAMETERS
+        assert L.fdtd_last_error() == b"ERROR: invalid parameters"
+    with pytest.raises(ValueError, match="ERROR: invalid parameters"):
+        FDTD(Parameters(4, 4, 0), 1e-12)
+
+
+def test_null_arguments_are_rejected():
+    L = _lib.load()
+    assert L.fdtd_create(None, 1.0, 0, None) == _lib.ERR_INVALID_ARGUMENT
+    assert L.fdtd_update_fields(None, 1) == _lib.ERR_INVALID_ARGUMENT
+    assert L.fdtd_destroy(None) == _lib.OK
+
+
+def test_cpp_programs_are_built():
+    for path in (os.path.join(ROOT, "fdtd_method_amd", "bin", "sample"),
+                 os.path.join(ROOT, "tests", "cpp", "test_fdtd_mi355x")):
+        assert os.access(path, os.X_OK), f"{path} missing: run __graft_entry__.build()"
