@@ -1,0 +1,249 @@
+#!/usr/bin/env python3
+"""Benchmark of the Yee time step on one MI355X.
+
+Workload (the headline configuration): a 512^3 periodic vacuum grid,
+dx = dy = dz = 1, dt = 0.5 dx / (c sqrt 3), a Gaussian-modulated sinusoid
+Jz point source at the centre (period 20 dt, envelope sigma 40 dt, peak at
+120 dt, amplitude 1, sampled at t = (n+1) dt), fields starting at zero,
+K steps (default 1000).  A step is one reference update_fields()
+(B/2, E, B/2) over the whole grid.
+
+Reported:
+  value         FP32 fused path, Gcell-updates/s (wall clock over K steps,
+                synchronised on both sides)
+  roofline      the FP32 fused sweep kernel: algorithmic 48 B/cell-update
+                (read and write E and B) / its average launch time from HIP
+                events on the grid's stream, against 8 TB/s; traffic = HBM
+                bytes per launch from the committed rocprofv3 counter runs
+  paths         FP64 fused, FP16-storage fused and the reference-structured
+                FP64 three-sweep path on the same workload
+  errors_vs_fp64  relative L2 and max-abs error of FP32 and FP16 after K
+                steps against the FP64 run, per component
+  cpu_baseline  the reference's own OpenMP class (compiled from its
+                sources) on this host's cores, bounded sample
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
+python bench.py --profile fp32|fp64|fp16   (only that path: for rocprofv3)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from fdtd_method_amd import FDTD, Parameters, Precision, calibration_copy  # noqa: E402
+from fdtd_method_amd import _lib  # noqa: E402
+
+C = 3e10
+PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s + %HBM roofline, FP32/FP16 vs FP64 GPU at 512³, L2 err vs FP64"
+WORD = {Precision.FP64: 8, Precision.FP32: 4, Precision.FP16: 2}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_sweep_fp32_512_traffic.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_grid(n, precision, autotune=True):
+    dt = 0.5 / (C * np.sqrt(3.0))
+    g = FDTD(Parameters(n, n, n), dt, precision)
+    src = dict(i=n // 2, j=n // 2, k=n // 2, comp=2, kind=1, amp=1.0, period=20 * dt,
+               t0=120 * dt, sigma=40 * dt, phase=0.0)
+    g.set_point_sources([src])
+    if precision == Precision.FP16:
+        # fields start at zero: seed the scales from one step's injection
+        # |cur * J| ~ 4 pi dt, then rescale the running state every 20 steps
+        g.autoscale(target_log2=10, magnitude_floor=4.0 * np.pi * dt * 1e-2)
+        g.set_autoscale_interval(20, 10)
+    if autotune:
+        shape, ms = g.autotune(reps=4)
+        log(f"autotune {precision.name}: V={shape[0]} TY={shape[1]} kchunk={shape[2]} "
+            f"({ms:.3f} ms/sweep)")
+    return g
+
+
+def run_path(g, steps, warmup, path=_lib.PATH_FUSED):
+    """Warm up, reset, then time `steps` steps: wall clock and HIP events."""
+    if warmup:
+        g.time_update_fields(warmup, path)
+        g.zero_fields()
+        if g.precision == Precision.FP16:
+            g.set_scales([1.0] * 6)
+            g.autoscale(target_log2=10, magnitude_floor=4.0 * np.pi * g.dt * 1e-2)
+    g.synchronize()
+    t0 = time.perf_counter()
+    ms_total, ms_main = g.time_update_fields(steps, path)
+    g.synchronize()
+    wall = time.perf_counter() - t0
+    cells = float(g.cells)
+    return dict(ms_per_step=1e3 * wall / steps, gcells=cells * steps / wall / 1e9,
+                event_ms_total=ms_total, launch_ms=ms_main / steps)
+
+
+def errors(test, ref):
+    per = test.compare(ref)
+    names = ["Ex", "Ey", "Ez", "Bx", "By", "Bz"]
+    comp = {}
+    for nm, (sd, sr, mx) in zip(names, per):
+        comp[nm] = dict(rel_l2=float(np.sqrt(sd / sr)) if sr > 0 else None, max_abs=mx)
+    tot = float(np.sqrt(sum(p[0] for p in per) / sum(p[1] for p in per)))
+    scale = max(float(np.sqrt(p[1] / test.cells)) for p in per)
+    return dict(rel_l2=tot, max_abs=max(p[2] for p in per), rms_ref=scale, per_component=comp)
+
+
+def energy(g):
+    s = g.sum_squares()
+    return sum(s) / (8.0 * np.pi)
+
+
+def cpu_baseline(n):
+    try:
+        threads = min(16, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        threads = min(16, os.cpu_count() or 1)
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    cmd = [sys.executable, "-m", "oracle.cpu_bench", "--n", str(n), "--seconds", "12"]
+    log("cpu baseline:", " ".join(cmd), f"(OMP_NUM_THREADS={threads})")
+    res = subprocess.run(cmd, capture_output=True, text=True, cwd=ROOT, env=env, timeout=600)
+    if res.returncode != 0:
+        log(res.stderr[-2000:])
+        return None
+    rec = json.loads(res.stdout.strip().splitlines()[-1])
+    return {k: rec[k] for k in ("value", "unit", "cores", "kind", "sample")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--three-sweep-steps", type=int, default=200)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile", choices=["fp32", "fp64", "fp16"], default=None)
+    ap.add_argument("--tuning", default=None,
+                    help="V,TY,kchunk launch shape for --profile (default: autotune)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): this build has no "
+              "multi-GPU path; see the sharding design in DESIGN.md.", file=sys.stderr)
+        sys.exit(2)
+    n, K, W = a.n, a.steps, a.warmup
+
+    if a.profile:
+        prec = {"fp32": Precision.FP32, "fp64": Precision.FP64, "fp16": Precision.FP16}[a.profile]
+        g = make_grid(n, prec, autotune=a.tuning is None)
+        if a.tuning:
+            g.set_tuning(*[int(x) for x in a.tuning.split(",")])
+        r = run_path(g, K, W)
+        nbytes = 1 << 30
+        ms_copy = calibration_copy(nbytes, 5)
+        print(json.dumps(dict(profile=a.profile, n=n, steps=K, tuning=g.get_tuning(), **r,
+                              calibration_copy=dict(bytes=nbytes, ms=ms_copy,
+                                                    gbs=2 * nbytes / ms_copy / 1e6))),
+              flush=True)
+        return
+
+    log(f"512^3 workload: n={n} steps={K} warmup={W}")
+    g32 = make_grid(n, Precision.FP32)
+    r32 = run_path(g32, K, W)
+    log(f"fp32 fused: {r32['gcells']:.1f} Gcell/s, launch {r32['launch_ms']:.3f} ms")
+    g64 = make_grid(n, Precision.FP64)
+    r64 = run_path(g64, K, W)
+    log(f"fp64 fused: {r64['gcells']:.1f} Gcell/s, launch {r64['launch_ms']:.3f} ms")
+    g16 = make_grid(n, Precision.FP16)
+    r16 = run_path(g16, K, W)
+    log(f"fp16 fused: {r16['gcells']:.1f} Gcell/s, launch {r16['launch_ms']:.3f} ms")
+    e32 = errors(g32, g64)
+    e16 = errors(g16, g64)
+    en = dict(fp64=energy(g64), fp32=energy(g32), fp16=energy(g16))
+    log(f"errors vs fp64: fp32 relL2={e32['rel_l2']:.3e}  fp16 relL2={e16['rel_l2']:.3e}")
+    g16.close()
+    g32_tune = g32.get_tuning()
+    g32.close()
+    # reference-structured FP64 path (three in-place sweeps) on the same grid
+    K3 = min(K, a.three_sweep_steps)
+    g64.zero_fields()
+    r3 = run_path(g64, K3, 2, path=_lib.PATH_3SWEEP)
+    log(f"fp64 3-sweep: {r3['gcells']:.1f} Gcell/s")
+    g64_tune = g64.get_tuning()
+    g64.close()
+
+    cells = float(n) ** 3
+    achieved = 12 * WORD[Precision.FP32] * cells / (r32["launch_ms"] * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        t = json.load(open(TRAFFIC_FILE))
+        if t.get("n") == n:
+            traffic = t.get("hbm_bytes_per_launch")
+
+    def path_rec(r, prec, tune=None, bytes_per_cell=None):
+        bpc = bytes_per_cell or 12 * WORD[prec]
+        gbs = bpc * cells / (r["launch_ms"] * 1e-3) / 1e9
+        rec = dict(gcells=r["gcells"], ms_per_step=r["ms_per_step"], launch_ms=r["launch_ms"],
+                   alg_bytes_per_cell=bpc, alg_gbs=gbs, frac_of_peak=gbs / PEAK_GBS)
+        if tune:
+            rec["launch_shape"] = dict(V=tune[0], TY=tune[1], kchunk=tune[2])
+        return rec
+
+    cpu = None if a.no_cpu_baseline else cpu_baseline(n)
+    out = {
+        "metric": METRIC,
+        "value": r32["gcells"],
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": r32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{n}^3 periodic vacuum Yee grid, dx=1, dt=0.5dx/(c*sqrt3), "
+                         "Gaussian-modulated sinusoid Jz point source at the centre "
+                         "(period 20dt, sigma 40dt, peak 120dt), zero initial fields, "
+                         f"{K} steps"),
+            "grid": [n, n, n],
+            "path": "fp32 storage and compute, fused E-then-double-half-B sweep",
+            "launch_shape": dict(V=g32_tune[0], TY=g32_tune[1], kchunk=g32_tune[2]),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "fdtd::fused_sweep<float,float,float,V,true>",
+            "alg_bytes_per_launch": 48.0 * cells,
+            "launch_ms": r32["launch_ms"],
+        },
+        "cpu_baseline": cpu,
+        "paths": {
+            "fp32_fused": path_rec(r32, Precision.FP32, g32_tune),
+            "fp64_fused": path_rec(r64, Precision.FP64, g64_tune),
+            "fp16_fused": path_rec(r16, Precision.FP16),
+            "fp64_3sweep": path_rec(r3, Precision.FP64, bytes_per_cell=240),
+        },
+        "speedup_fp32_vs_fp64": r32["gcells"] / r64["gcells"],
+        "speedup_fp32_vs_fp64_3sweep": r32["gcells"] / r3["gcells"],
+        "errors_vs_fp64": {"fp32": e32, "fp16": e16},
+        "energy": en,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

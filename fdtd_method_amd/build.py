@@ -25,7 +25,7 @@ HIP_SOURCES = ["fdtd_runtime.hip", "fdtd_batch.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-command-line-argument",
+    "-Wno-unused-command-line-argument", "-Wl,--no-undefined",
 ]
 
 

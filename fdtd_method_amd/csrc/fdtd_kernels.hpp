@@ -121,15 +121,18 @@ plain_update_B(Fields<S> f, Coefs<C> c, int Ni, int Nj, int Nk) {
     f.b[2][idx] = static_cast<S>(C(f.b[2][idx]) + dbz);
 }
 
-// Source values of step n: vals[q] = J of source q at t = (n+1)*dt, in FP64.
-// One thread per source; the sweeps then only add precomputed values.
-__global__ void __launch_bounds__(64)
-eval_sources(const PointSource* __restrict__ src, int nsrc, const long long* __restrict__ step_dev,
-             long long step, double dt, double* __restrict__ vals) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nsrc) return;
-    const long long n = step_dev ? *step_dev : step;
-    vals[q] = source_value(src[q], static_cast<double>(n + 1) * dt);
+// Source values of steps n0 .. n0+nsteps-1: vals[s*nsrc + q] = J of source q
+// at t = (n0+s+1)*dt, in FP64 (n0 read from step_dev when it is non-null).
+// The sweeps then only add precomputed values.
+__global__ void __launch_bounds__(256)
+eval_sources(const PointSource* __restrict__ src, int nsrc, int nsteps,
+             const long long* __restrict__ step_dev, long long step, double dt,
+             double* __restrict__ vals) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nsrc * nsteps) return;
+    const int s = t / nsrc, q = t - s * nsrc;
+    const long long n = (step_dev ? *step_dev : step) + s;
+    vals[t] = source_value(src[q], static_cast<double>(n + 1) * dt);
 }
 
 // Adds the J term of every source located in cells (ib..ib+V-1, j, k) of the

@@ -28,6 +28,7 @@ constexpr double kPiRef = 3.14159265358; // include/Constants.h:14
 constexpr int kMaxSparseCurrents = 64;   // beyond this J is kept dense
 constexpr size_t kLdsLimit = 160 * 1024; // gfx950 LDS per workgroup
 constexpr int kReduceBlocks = 1024;
+constexpr int kValTable = 256;            // steps of source values per table fill
 
 thread_local std::string g_last_error;
 
@@ -79,12 +80,14 @@ struct fdtd_grid {
     std::map<long long, double> sparse_j;  // key = index*3 + comp
     std::vector<fdtd::PointSource> waveforms;
     fdtd::PointSource* d_src = nullptr;
-    double* d_vals = nullptr;  // source values of the step being launched
+    double* d_vals = nullptr;  // source values, kValTable steps x nsrc
     int d_src_cap = 0;
     int nsrc = 0;
     bool src_dirty = true;
     Tuning tune;
     bool fused_ok = false;
+    int autoscale_interval = 0;  // FP16: rescale every this many sweeps (0 = off)
+    int autoscale_log2 = 12;
     double* d_part = nullptr;
     double* d_out = nullptr;
     hipEvent_t ev[3] = {};
@@ -152,21 +155,23 @@ bool shape_valid(const fdtd_grid* g, int V, int TY, int kchunk) {
     return true;
 }
 
-// z-chunk length for a tile height: the per-XCD tile count should fill
-// the XCD's 32 CUs in whole rounds (no tail), the grid should have at least
-// 256 tiles, and longer chunks recompute fewer boundary planes.
+// z-chunk length for a tile height: balanced chunks (lengths differ by at
+// most one plane), at least two tiles per CU where the grid allows it, the
+// per-XCD tile count filling its 32 CUs in whole rounds, and otherwise
+// longer chunks (fewer recomputed boundary planes).
 int default_kchunk(int Nj, int Nk, int TY) {
     const int ntile_y = (Nj + TY - 1) / TY;
     int best_k = Nk;
     double best_score = -1.0;
-    for (int k = std::min(Nk, 8); k <= Nk; ++k) {
-        const int nchunk = (Nk + k - 1) / k;
-        if (k != Nk && (nchunk - 1) * k >= Nk) continue;
+    for (int nchunk = 1; nchunk <= Nk; ++nchunk) {
+        const int k = (Nk + nchunk - 1) / nchunk;
+        if ((Nk + k - 1) / k != nchunk) continue;  // not a distinct balanced split
         const int ntiles = ntile_y * nchunk;
         const int per_xcd = (ntiles + 7) / 8;
         const int rounds = (per_xcd + 31) / 32;
         const double fill = double(ntiles) / (rounds * 256.0);
-        const double score = fill * (1.0 - 1.0 / (k + 1.0));
+        const double spread = std::min(1.0, ntiles / 512.0);
+        const double score = fill * spread * (1.0 - 1.0 / (k + 1.0));
         if (score > best_score + 1e-9) {
             best_score = score;
             best_k = k;
@@ -252,7 +257,7 @@ int flush_sources(fdtd_grid* g) {
         g->d_vals = nullptr;
         const int cap = std::max<int>(16, (int)all.size());
         HIP_TRY(hipMalloc(&g->d_src, sizeof(fdtd::PointSource) * cap));
-        HIP_TRY(hipMalloc(&g->d_vals, sizeof(double) * cap));
+        HIP_TRY(hipMalloc(&g->d_vals, sizeof(double) * cap * kValTable));
         g->d_src_cap = cap;
     }
     if (!all.empty())
@@ -264,11 +269,14 @@ int flush_sources(fdtd_grid* g) {
     return FDTD_OK;
 }
 
-// J values of all point sources for step `step` (t = (step+1)*dt).
-int eval_sources(fdtd_grid* g, long long step) {
+// J values of all point sources for steps step .. step+nsteps-1
+// (t = (n+1)*dt) into the value table, nsteps <= kValTable.
+int eval_sources(fdtd_grid* g, long long step, int nsteps) {
     if (g->nsrc == 0) return FDTD_OK;
-    hipLaunchKernelGGL(fdtd::eval_sources, dim3((g->nsrc + 63) / 64), dim3(64), 0, g->stream,
-                       g->d_src, g->nsrc, (const long long*)nullptr, step, g->dt, g->d_vals);
+    const int n = g->nsrc * nsteps;
+    hipLaunchKernelGGL(fdtd::eval_sources, dim3((n + 255) / 256), dim3(256), 0, g->stream,
+                       g->d_src, g->nsrc, nsteps, (const long long*)nullptr, step, g->dt,
+                       g->d_vals);
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
@@ -295,7 +303,7 @@ int launch_plain_E(fdtd_grid* g, long long step) {
     const JT* j[3] = {nullptr, nullptr, nullptr};
     if (g->j_dense_active)
         for (int d = 0; d < 3; ++d) j[d] = static_cast<const JT*>(g->J[d]);
-    if (int rc = eval_sources(g, step)) return rc;
+    if (int rc = eval_sources(g, step, 1)) return rc;
     hipLaunchKernelGGL((fdtd::plain_update_E<S, C, JT>), grid, block, 0, g->stream,
                        fields_of<S>(g, g->cur), j[0], j[1], j[2], g->d_src, g->d_vals, g->nsrc,
                        make_coefs<C>(g), g->Ni, g->Nj, g->Nk);
@@ -322,7 +330,7 @@ int plain_E(fdtd_grid* g, long long step) {
 // ---- fused sweep --------------------------------------------------------------
 
 template <typename S, int V, bool STEADY>
-int launch_fused_v(fdtd_grid* g, const Tuning& t, long long step, int in, int out) {
+int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out) {
     using C = typename fdtd::ComputeOf<S>::type;
     using JT = C;
     auto kern = fdtd::fused_sweep<S, C, JT, V, STEADY>;
@@ -332,7 +340,7 @@ int launch_fused_v(fdtd_grid* g, const Tuning& t, long long step, int in, int ou
     for (int d = 0; d < 3; ++d)
         a.j[d] = g->j_dense_active ? static_cast<const JT*>(g->J[d]) : nullptr;
     a.src = g->d_src;
-    a.vals = g->d_vals;
+    a.vals = g->d_vals ? g->d_vals + (size_t)slot * g->nsrc : nullptr;
     a.nsrc = g->nsrc;
     a.pad = 0;
     a.c = make_coefs<C>(g);
@@ -344,47 +352,55 @@ int launch_fused_v(fdtd_grid* g, const Tuning& t, long long step, int in, int ou
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
     });
     const int threads = (t.TY + 1) * (g->Ni / V);
-    if (int rc = eval_sources(g, step)) return rc;
     hipLaunchKernelGGL(kern, dim3(8 * a.g.per_xcd), dim3(threads), lds, g->stream, a);
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
 
 template <typename S, bool STEADY>
-int launch_fused_s(fdtd_grid* g, const Tuning& t, long long step, int in, int out) {
+int launch_fused_s(fdtd_grid* g, const Tuning& t, int slot, int in, int out) {
     switch (t.V) {
-        case 1: return launch_fused_v<S, 1, STEADY>(g, t, step, in, out);
-        case 2: return launch_fused_v<S, 2, STEADY>(g, t, step, in, out);
-        case 4: return launch_fused_v<S, 4, STEADY>(g, t, step, in, out);
+        case 1: return launch_fused_v<S, 1, STEADY>(g, t, slot, in, out);
+        case 2: return launch_fused_v<S, 2, STEADY>(g, t, slot, in, out);
+        case 4: return launch_fused_v<S, 4, STEADY>(g, t, slot, in, out);
         case 8:
-            if constexpr (sizeof(S) <= 4) return launch_fused_v<S, 8, STEADY>(g, t, step, in, out);
+            if constexpr (sizeof(S) <= 4) return launch_fused_v<S, 8, STEADY>(g, t, slot, in, out);
             [[fallthrough]];
         default: return fail(FDTD_ERR_UNSUPPORTED, "unsupported V=%d", t.V);
     }
 }
 
-int launch_fused(fdtd_grid* g, const Tuning& t, bool steady, long long step, int in, int out) {
+// One fused sweep; its source values are row `slot` of the value table.
+int launch_fused(fdtd_grid* g, const Tuning& t, bool steady, int slot, int in, int out) {
     switch (g->precision) {
         case FDTD_FP64:
-            return steady ? launch_fused_s<double, true>(g, t, step, in, out)
-                          : launch_fused_s<double, false>(g, t, step, in, out);
+            return steady ? launch_fused_s<double, true>(g, t, slot, in, out)
+                          : launch_fused_s<double, false>(g, t, slot, in, out);
         case FDTD_FP32:
-            return steady ? launch_fused_s<float, true>(g, t, step, in, out)
-                          : launch_fused_s<float, false>(g, t, step, in, out);
+            return steady ? launch_fused_s<float, true>(g, t, slot, in, out)
+                          : launch_fused_s<float, false>(g, t, slot, in, out);
         default:
-            return steady ? launch_fused_s<_Float16, true>(g, t, step, in, out)
-                          : launch_fused_s<_Float16, false>(g, t, step, in, out);
+            return steady ? launch_fused_s<_Float16, true>(g, t, slot, in, out)
+                          : launch_fused_s<_Float16, false>(g, t, slot, in, out);
     }
 }
 
+int rescale_to_target(fdtd_grid* g, bool current_only);
+
 // n steps: B half step in place, then n fused sweeps between the buffers.
+// With FP16 autoscaling, the state between two sweeps, (E_n, B_{n+1/2}),
+// is rescaled by exact powers of two every autoscale_interval sweeps.
 int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
     int rc = plain_B(g);
     if (rc) return rc;
     if (mid) HIP_TRY(hipEventRecord(mid, g->stream));
     for (int s = 0; s < nsteps; ++s) {
+        if (g->autoscale_interval > 0 && s > 0 && s % g->autoscale_interval == 0)
+            if ((rc = rescale_to_target(g, true))) return rc;
+        if (s % kValTable == 0)
+            if ((rc = eval_sources(g, g->step + s, std::min(kValTable, nsteps - s)))) return rc;
         const int in = g->cur, out = 1 - g->cur;
-        rc = launch_fused(g, g->tune, s + 1 < nsteps, g->step + s, in, out);
+        rc = launch_fused(g, g->tune, s + 1 < nsteps, s % kValTable, in, out);
         if (rc) return rc;
         g->cur = out;
     }
@@ -884,22 +900,86 @@ int fdtd_get_scales(const fdtd_grid* g, double scales[6]) {
     return FDTD_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Power-of-two scales putting max|stored| in [2^(target-1), 2^target).
+void target_scales(const fdtd_grid* g, const double mx[6], int target_log2, double floor_mag,
+                   double s[6]) {
+    for (int c = 0; c < 6; ++c) {
+        const double m = std::max(mx[c], floor_mag);
+        if (!(m > 0)) { s[c] = g->scales[c]; continue; }
+        int e;
+        std::frexp(m, &e);  // 2^(e-1) <= m < 2^e
+        const int k = std::max(-100, std::min(100, target_log2 - e));
+        s[c] = std::ldexp(1.0, k);
+    }
+}
+
+int apply_scales(fdtd_grid* g, const double s[6], bool current_only) {
+    for (int c = 0; c < 6; ++c) {
+        const double f = s[c] / g->scales[c];
+        if (f != 1.0)
+            for (int b = 0; b < 2; ++b) {
+                if (current_only && b != g->cur) continue;
+                void* p = c < 3 ? g->E[b][c] : g->B[b][c - 3];
+                hipLaunchKernelGGL(fdtd::scale_array<_Float16>, dim3(grid_blocks(g->N)), dim3(256), 0,
+                                   g->stream, static_cast<_Float16*>(p), g->N, (float)f);
+            }
+        g->scales[c] = s[c];
+    }
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+// Rescale the current FP16 state so every component's max sits just below
+// 2^autoscale_log2 (components that are all zero keep their scale).
+int rescale_to_target(fdtd_grid* g, bool current_only) {
+    double mx[6], s[6];
+    if (int rc = fdtd_max_abs(g, mx)) return rc;
+    target_scales(g, mx, g->autoscale_log2, 0.0, s);
+    return apply_scales(g, s, current_only);
+}
+
+}  // namespace
+
+extern "C" {
+
 int fdtd_autoscale(fdtd_grid* g, int target_log2, double floor_mag) {
     if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
     if (g->precision != FDTD_FP16) return FDTD_OK;
     double mx[6];
     if (int rc = fdtd_max_abs(g, mx)) return rc;
     double s[6];
-    for (int c = 0; c < 6; ++c) {
-        const double m = std::max(mx[c], floor_mag);
-        if (!(m > 0)) { s[c] = g->scales[c]; continue; }
-        int e;
-        std::frexp(m, &e);  // m < 2^e
-        int k = target_log2 - e;
-        k = std::max(-60, std::min(60, k));
-        s[c] = std::ldexp(1.0, k);
-    }
+    target_scales(g, mx, target_log2, floor_mag, s);
     return fdtd_set_scales(g, s);
+}
+
+int fdtd_set_autoscale_interval(fdtd_grid* g, int interval, int target_log2) {
+    if (!g || interval < 0 || target_log2 < 1 || target_log2 > 15)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (g->precision != FDTD_FP16 && interval > 0)
+        return fail(FDTD_ERR_UNSUPPORTED, "autoscaling applies to FP16 grids only");
+    g->autoscale_interval = interval;
+    g->autoscale_log2 = target_log2;
+    return FDTD_OK;
+}
+
+int fdtd_zero_fields(fdtd_grid* g) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    const size_t bytes = g->N * g->esize + 16;
+    for (int b = 0; b < 2; ++b)
+        for (int d = 0; d < 3; ++d) {
+            if (int rc = zero_fill(g->E[b][d], bytes, g->stream)) return rc;
+            if (int rc = zero_fill(g->B[b][d], bytes, g->stream)) return rc;
+        }
+    g->j_dense_active = false;
+    if (!g->sparse_j.empty()) g->src_dirty = true;
+    g->sparse_j.clear();
+    g->step = 0;
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return FDTD_OK;
 }
 
 int fdtd_set_tuning(fdtd_grid* g, int V, int TY, int kchunk) {
@@ -977,10 +1057,11 @@ int fdtd_autotune(fdtd_grid* g, int reps, int out[3], float* best_ms) {
                 Tuning t{V, TY, kc};
                 if (!shape_valid(g, V, TY, kc)) continue;
                 // warm-up then timed reps
-                if (int rc = launch_fused(g, t, true, g->step, g->cur, 1 - g->cur)) return rc;
+                if (int rc = eval_sources(g, g->step, 1)) return rc;
+                if (int rc = launch_fused(g, t, true, 0, g->cur, 1 - g->cur)) return rc;
                 HIP_TRY(hipEventRecord(g->ev[0], g->stream));
                 for (int r = 0; r < reps; ++r)
-                    if (int rc = launch_fused(g, t, true, g->step, (r & 1) ? 1 - g->cur : g->cur,
+                    if (int rc = launch_fused(g, t, true, 0, (r & 1) ? 1 - g->cur : g->cur,
                                               (r & 1) ? g->cur : 1 - g->cur))
                         return rc;
                 HIP_TRY(hipEventRecord(g->ev[1], g->stream));

@@ -256,6 +256,16 @@ class FDTD:
     def autoscale(self, target_log2: int = 12, magnitude_floor: float = 0.0) -> None:
         _lib.check(self._L.fdtd_autoscale(self._h, int(target_log2), float(magnitude_floor)))
 
+    def set_autoscale_interval(self, interval: int, target_log2: int = 12) -> None:
+        _lib.check(self._L.fdtd_set_autoscale_interval(self._h, int(interval), int(target_log2)))
+
+    def zero_fields(self) -> None:
+        """E, B, J := 0 and step := 0 on the device (sources are kept)."""
+        for f in self._fields.values():
+            f._host = None
+            f._dirty = False
+        _lib.check(self._L.fdtd_zero_fields(self._h))
+
     def set_tuning(self, V=0, TY=0, kchunk=0) -> None:
         _lib.check(self._L.fdtd_set_tuning(self._h, int(V), int(TY), int(kchunk)))
 

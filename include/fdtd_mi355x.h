@@ -147,6 +147,13 @@ int fdtd_checksum(fdtd_grid* grid, unsigned long long out[6]);
 int fdtd_set_scales(fdtd_grid* grid, const double scales[6]);
 int fdtd_get_scales(const fdtd_grid* grid, double scales[6]);
 int fdtd_autoscale(fdtd_grid* grid, int target_log2, double magnitude_floor);
+/* FP16: during fdtd_update_fields, every `interval` sweeps rescale the
+ * running state so each component's max|stored| lies in
+ * [2^(target_log2-1), 2^target_log2) (interval 0 = off). */
+int fdtd_set_autoscale_interval(fdtd_grid* grid, int interval, int target_log2);
+
+/* Reset: E, B and J to zero, step counter to 0 (sources are kept). */
+int fdtd_zero_fields(fdtd_grid* grid);
 
 /* Fused-sweep launch shape: V cells per lane along x, TY rows per tile,
  * kchunk planes per z chunk (0 = keep current).  fdtd_autotune times the
