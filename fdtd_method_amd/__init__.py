@@ -7,6 +7,7 @@ host-side mirror of the reference interface.
 from .structures import (C, PI, Axis, Component, CurrentParameters, Parameters, Precision,
                          SelectedFields)
 from .fdtd import FDTD, Field, calibration_copy
+from .batch import Batch, plane_wave_batch
 
 __all__ = ["C", "PI", "Axis", "Component", "CurrentParameters", "Parameters", "Precision",
-           "SelectedFields", "FDTD", "Field", "calibration_copy"]
+           "SelectedFields", "FDTD", "Field", "calibration_copy", "Batch", "plane_wave_batch"]

@@ -32,6 +32,7 @@ EXPORTED = [
     "fdtd_get_tuning", "fdtd_autotune", "fdtd_time_update_fields", "fdtd_calibration_copy",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_update_fields",
     "fdtd_batch_update_fields_graph", "fdtd_batch_synchronize", "fdtd_batch_time",
+    "fdtd_batch_set_shape", "fdtd_batch_get_shape", "fdtd_batch_autotune",
 ]
 
 
@@ -112,6 +113,10 @@ def load():
         "fdtd_batch_synchronize": ([_vp], _i),
         "fdtd_batch_time": ([_vp, _i, ctypes.POINTER(ctypes.c_float),
                              ctypes.POINTER(ctypes.c_float)], _i),
+        "fdtd_batch_set_shape": ([_vp, _i, _i, ctypes.c_longlong], _i),
+        "fdtd_batch_get_shape": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _i),
+        "fdtd_batch_autotune": ([_vp, _i, ctypes.POINTER(ctypes.c_longlong),
+                                 ctypes.POINTER(ctypes.c_float)], _i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

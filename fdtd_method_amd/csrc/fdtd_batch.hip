@@ -1,18 +1,599 @@
-// Batched solves (placeholder until the persistent batched sweep lands).
+// Batched solves: many independent Yee grids of uneven sizes advanced by one
+// persistent launch per step (batch_fused_sweep), optionally as a captured
+// HIP graph replayed with a device-side step counter.
+//
+// Semantics per grid are those of fdtd_update_fields (the reference's
+// update_fields(), src/FDTD/FDTD.cpp:153-157, n times): one B half step,
+// then n fused sweeps.  All grids of a batch share one precision and one
+// step count.
 #include <hip/hip_runtime.h>
 
-#include "fdtd_mi355x.h"
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "fdtd_internal.hpp"
+
+using namespace fdtd::rt;
+
+namespace {
+
+constexpr int kBatchLdsCap = 80 * 1024;  // per workgroup: two per CU
+
+struct BatchShape {
+    int V = 4;                  // cells per lane along x
+    int NT = 512;               // threads per workgroup
+    long long target = 65536;   // cells per tile (sets each grid's z-chunk)
+};
+
+struct GridPlan {
+    int TY = 1, kchunk = 1, ntile_y = 1, nchunk = 1;
+};
+
+}  // namespace
+
+struct fdtd_batch {
+    std::vector<fdtd_grid*> grids;
+    int precision = FDTD_FP32;
+    hipStream_t stream = nullptr;
+    BatchShape shape;
+    std::vector<GridPlan> plan;
+    size_t lds_bytes = 0;
+    int nwork = 0;
+    int nblocks = 0;
+    void* d_grids = nullptr;
+    size_t grid_desc_bytes = 0;
+    std::vector<unsigned char> h_grids;
+    int4* d_work = nullptr;
+    unsigned int* d_counters = nullptr;
+    fdtd::PointSource* d_src = nullptr;
+    double* d_vals = nullptr;
+    int total_src = 0;
+    int src_cap = 0;
+    long long* d_step = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    hipEvent_t ev[4] = {};
+};
+
+namespace {
+
+int batch_v_candidates(int precision, int* out) {
+    if (precision == FDTD_FP64) { out[0] = 2; out[1] = 1; return 2; }
+    if (precision == FDTD_FP32) { out[0] = 4; out[1] = 2; out[2] = 1; return 3; }
+    out[0] = 4; out[1] = 8; out[2] = 2; out[3] = 1;
+    return 4;
+}
+
+// Per-grid tile height and z-chunk for a batch launch shape; false if a
+// grid cannot be tiled with it.
+bool make_plan(const fdtd_batch* b, const BatchShape& sh, std::vector<GridPlan>* plan,
+               size_t* lds) {
+    const int cs = (int)compute_size(b->precision);
+    if (sh.NT > fdtd::fused_max_threads(sh.V, cs)) return false;
+    plan->assign(b->grids.size(), GridPlan());
+    *lds = 0;
+    for (size_t q = 0; q < b->grids.size(); ++q) {
+        const fdtd_grid* g = b->grids[q];
+        if (g->Ni % sh.V) return false;
+        const int lanes = g->Ni / sh.V;
+        int ty = std::min(g->Nj, sh.NT / lanes - 1);
+        while (ty >= 1 && fdtd::fused_lds_bytes(ty, g->Ni, cs) > (size_t)kBatchLdsCap) --ty;
+        if (ty < 1) return false;
+        GridPlan& p = (*plan)[q];
+        p.TY = ty;
+        p.ntile_y = (g->Nj + ty - 1) / ty;
+        long long k = sh.target / ((long long)ty * g->Ni);
+        k = std::max<long long>(4, std::min<long long>(k, g->Nk));
+        p.nchunk = (int)((g->Nk + k - 1) / k);
+        p.kchunk = (g->Nk + p.nchunk - 1) / p.nchunk;
+        *lds = std::max(*lds, fdtd::fused_lds_bytes(ty, g->Ni, cs));
+    }
+    return true;
+}
+
+// Work list: every tile of every grid, most expensive first (cost ~ cells
+// of the tile including its halo row and plane).
+std::vector<int4> make_work(const fdtd_batch* b, const std::vector<GridPlan>& plan) {
+    std::vector<int4> w;
+    for (size_t q = 0; q < b->grids.size(); ++q) {
+        const fdtd_grid* g = b->grids[q];
+        const GridPlan& p = plan[q];
+        for (int c = 0; c < p.nchunk; ++c)
+            for (int t = 0; t < p.ntile_y; ++t) {
+                const int ty = std::min(p.TY, g->Nj - t * p.TY);
+                const int kc = std::min(p.kchunk, g->Nk - c * p.kchunk);
+                w.push_back(make_int4((int)q, t, c, (ty + 1) * g->Ni * (kc + 1)));
+            }
+    }
+    std::stable_sort(w.begin(), w.end(), [](const int4& a, const int4& c) { return a.w > c.w; });
+    return w;
+}
+
+template <typename S>
+void fill_descriptors(fdtd_batch* b, const std::vector<int>& src_off) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    b->h_grids.resize(sizeof(fdtd::BatchGrid<S, C>) * b->grids.size());
+    auto* d = reinterpret_cast<fdtd::BatchGrid<S, C>*>(b->h_grids.data());
+    for (size_t q = 0; q < b->grids.size(); ++q) {
+        fdtd_grid* g = b->grids[q];
+        const GridPlan& p = b->plan[q];
+        fdtd::BatchGrid<S, C> bg;
+        bg.buf[0] = fields_of<S>(g, g->cur);
+        bg.buf[1] = fields_of<S>(g, 1 - g->cur);
+        bg.src = b->d_src ? b->d_src + src_off[q] : nullptr;
+        bg.src_off = src_off[q];
+        bg.nsrc = src_off[q + 1] - src_off[q];
+        bg.c = make_coefs<C>(g);
+        bg.g.Ni = g->Ni;
+        bg.g.Nj = g->Nj;
+        bg.g.Nk = g->Nk;
+        bg.g.TY = p.TY;
+        bg.g.kchunk = p.kchunk;
+        bg.g.ntile_y = p.ntile_y;
+        bg.g.ntiles = p.ntile_y * p.nchunk;
+        bg.g.per_xcd = 0;
+        d[q] = bg;
+    }
+}
+
+template <typename S, int V, bool STEADY>
+int occupancy_v(const fdtd_batch* b, int nt, size_t lds, int* blocks) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    auto kern = fdtd::batch_fused_sweep<S, C, V, STEADY>;
+    auto last = fdtd::batch_fused_sweep<S, C, V, !STEADY>;
+    if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
+    if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(last), lds)) return rc;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, kern, nt, lds));
+    return FDTD_OK;
+}
+
+template <typename S, int V, bool STEADY>
+int launch_batch_v(fdtd_batch* b, int parity, const double* vals_row) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    fdtd::BatchArgs<S, C> a;
+    a.grids = static_cast<const fdtd::BatchGrid<S, C>*>(b->d_grids);
+    a.work = b->d_work;
+    a.nwork = b->nwork;
+    a.parity = parity;
+    a.vals_row = vals_row;
+    a.counters = b->d_counters;
+    hipLaunchKernelGGL((fdtd::batch_fused_sweep<S, C, V, STEADY>), dim3(b->nblocks),
+                       dim3(b->shape.NT), b->lds_bytes, b->stream, a);
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+template <typename S, bool STEADY>
+int launch_batch_s(fdtd_batch* b, int parity, const double* vals_row) {
+    switch (b->shape.V) {
+        case 1: return launch_batch_v<S, 1, STEADY>(b, parity, vals_row);
+        case 2: return launch_batch_v<S, 2, STEADY>(b, parity, vals_row);
+        case 4: return launch_batch_v<S, 4, STEADY>(b, parity, vals_row);
+        case 8:
+            if constexpr (sizeof(S) <= 4) return launch_batch_v<S, 8, STEADY>(b, parity, vals_row);
+            [[fallthrough]];
+        default: return fail(FDTD_ERR_UNSUPPORTED, "unsupported V=%d", b->shape.V);
+    }
+}
+
+int launch_batch(fdtd_batch* b, bool steady, int parity, const double* vals_row) {
+    switch (b->precision) {
+        case FDTD_FP64:
+            return steady ? launch_batch_s<double, true>(b, parity, vals_row)
+                          : launch_batch_s<double, false>(b, parity, vals_row);
+        case FDTD_FP32:
+            return steady ? launch_batch_s<float, true>(b, parity, vals_row)
+                          : launch_batch_s<float, false>(b, parity, vals_row);
+        default:
+            return steady ? launch_batch_s<_Float16, true>(b, parity, vals_row)
+                          : launch_batch_s<_Float16, false>(b, parity, vals_row);
+    }
+}
+
+template <typename S>
+int occupancy(const fdtd_batch* b, int V, int nt, size_t lds, int* blocks) {
+    switch (V) {
+        case 1: return occupancy_v<S, 1, true>(b, nt, lds, blocks);
+        case 2: return occupancy_v<S, 2, true>(b, nt, lds, blocks);
+        case 4: return occupancy_v<S, 4, true>(b, nt, lds, blocks);
+        case 8:
+            if constexpr (sizeof(S) <= 4) return occupancy_v<S, 8, true>(b, nt, lds, blocks);
+            [[fallthrough]];
+        default: return fail(FDTD_ERR_UNSUPPORTED, "unsupported V=%d", V);
+    }
+}
+
+// Apply a launch shape: per-grid plans, work list, persistent grid size.
+int apply_shape(fdtd_batch* b, const BatchShape& sh) {
+    std::vector<GridPlan> plan;
+    size_t lds = 0;
+    if (!make_plan(b, sh, &plan, &lds))
+        return fail(FDTD_ERR_UNSUPPORTED, "batch shape V=%d NT=%d does not tile every grid",
+                    sh.V, sh.NT);
+    int per_cu = 0;
+    int rc = b->precision == FDTD_FP64 ? occupancy<double>(b, sh.V, sh.NT, lds, &per_cu)
+           : b->precision == FDTD_FP32 ? occupancy<float>(b, sh.V, sh.NT, lds, &per_cu)
+                                       : occupancy<_Float16>(b, sh.V, sh.NT, lds, &per_cu);
+    if (rc) return rc;
+    if (per_cu < 1) return fail(FDTD_ERR_UNSUPPORTED, "batch shape does not fit a CU");
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const std::vector<int4> work = make_work(b, plan);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->d_work) HIP_TRY(hipFree(b->d_work));
+    b->d_work = nullptr;
+    HIP_TRY(hipMalloc(&b->d_work, sizeof(int4) * work.size()));
+    HIP_TRY(hipMemcpy(b->d_work, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice));
+    b->shape = sh;
+    b->plan = plan;
+    b->lds_bytes = lds;
+    b->nwork = (int)work.size();
+    b->nblocks = std::max(1, std::min(b->nwork, per_cu * cus));
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        b->graph_steps = 0;
+    }
+    return FDTD_OK;
+}
+
+// Sources of every grid into one device list (and value table), then the
+// grid descriptors for the grids' current buffers.
+int prepare(fdtd_batch* b) {
+    std::vector<fdtd::PointSource> all;
+    std::vector<int> off(b->grids.size() + 1, 0);
+    for (size_t q = 0; q < b->grids.size(); ++q) {
+        const std::vector<fdtd::PointSource> s = host_sources(b->grids[q]);
+        off[q] = (int)all.size();
+        all.insert(all.end(), s.begin(), s.end());
+        if (b->grids[q]->j_dense_active)
+            return fail(FDTD_ERR_UNSUPPORTED, "batched grids take point sources, not dense J");
+        HIP_TRY(hipStreamSynchronize(b->grids[q]->stream));
+    }
+    off[b->grids.size()] = (int)all.size();
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if ((int)all.size() > b->src_cap) {
+        if (b->d_src) HIP_TRY(hipFree(b->d_src));
+        if (b->d_vals) HIP_TRY(hipFree(b->d_vals));
+        b->d_src = nullptr;
+        b->d_vals = nullptr;
+        b->src_cap = std::max<int>(64, (int)all.size());
+        HIP_TRY(hipMalloc(&b->d_src, sizeof(fdtd::PointSource) * b->src_cap));
+        HIP_TRY(hipMalloc(&b->d_vals, sizeof(double) * b->src_cap * kValTable));
+        if (b->graph) {
+            (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+            b->graph_steps = 0;
+        }
+    }
+    if (!all.empty())
+        HIP_TRY(hipMemcpy(b->d_src, all.data(), sizeof(fdtd::PointSource) * all.size(),
+                          hipMemcpyHostToDevice));
+    b->total_src = (int)all.size();
+    switch (b->precision) {
+        case FDTD_FP64: fill_descriptors<double>(b, off); break;
+        case FDTD_FP32: fill_descriptors<float>(b, off); break;
+        default: fill_descriptors<_Float16>(b, off);
+    }
+    if (b->h_grids.size() != b->grid_desc_bytes) {
+        if (b->d_grids) HIP_TRY(hipFree(b->d_grids));
+        HIP_TRY(hipMalloc(&b->d_grids, b->h_grids.size()));
+        b->grid_desc_bytes = b->h_grids.size();
+    }
+    HIP_TRY(hipMemcpy(b->d_grids, b->h_grids.data(), b->h_grids.size(), hipMemcpyHostToDevice));
+    return FDTD_OK;
+}
+
+template <typename S>
+int prologue_B_s(fdtd_batch* b) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    const auto* desc = static_cast<const fdtd::BatchGrid<S, C>*>(b->d_grids);
+    for (size_t q = 0; q < b->grids.size(); ++q) {
+        const fdtd_grid* g = b->grids[q];
+        const dim3 block(64, 4, 1);
+        const dim3 grid((g->Ni + 63) / 64, (g->Nj + 3) / 4, g->Nk);
+        hipLaunchKernelGGL((fdtd::batch_plain_update_B<S, C>), grid, block, 0, b->stream, desc,
+                           (int)q);
+    }
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+int prologue_B(fdtd_batch* b) {
+    switch (b->precision) {
+        case FDTD_FP64: return prologue_B_s<double>(b);
+        case FDTD_FP32: return prologue_B_s<float>(b);
+        default: return prologue_B_s<_Float16>(b);
+    }
+}
+
+// Enqueue n steps on the batch stream: B/2 prologue of every grid, then n
+// batched sweeps.  Source values come from step0 (host) or, when
+// step_dev is non-null, from the device counter, which is advanced at the
+// end (graph replays then continue in time).
+int enqueue_steps(fdtd_batch* b, int nsteps, long long step0, bool device_step) {
+    if (int rc = prologue_B(b)) return rc;
+    for (int s = 0; s < nsteps; ++s) {
+        if (b->total_src > 0 && s % kValTable == 0) {
+            const int chunk = std::min(kValTable, nsteps - s);
+            const int n = b->total_src * chunk;
+            // values of steps step0+s ...; with a device counter the kernel
+            // reads the run's first step and this launch adds s to it
+            hipLaunchKernelGGL(fdtd::eval_sources, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                               b->d_src, b->total_src, chunk,
+                               device_step ? (const long long*)b->d_step : nullptr,
+                               device_step ? 0LL : step0 + s, b->grids[0]->dt, b->d_vals);
+            HIP_TRY(hipGetLastError());
+            if (device_step) {
+                hipLaunchKernelGGL(fdtd::advance_step, dim3(1), dim3(1), 0, b->stream, b->d_step,
+                                   (long long)chunk);
+                HIP_TRY(hipGetLastError());
+            }
+        }
+        const double* row = b->total_src > 0 ? b->d_vals + (size_t)(s % kValTable) * b->total_src
+                                             : nullptr;
+        if (int rc = launch_batch(b, s + 1 < nsteps, s & 1, row)) return rc;
+    }
+    return FDTD_OK;
+}
+
+void finish_steps(fdtd_batch* b, int nsteps) {
+    for (fdtd_grid* g : b->grids) {
+        g->cur = (g->cur + nsteps) & 1;
+        g->step += nsteps;
+    }
+}
+
+// The batch evaluates all waveform sources in one table with one dt, at
+// the common step count of its grids.
+int check_steps(fdtd_batch* b) {
+    for (fdtd_grid* g : b->grids)
+        if (g->step != b->grids[0]->step)
+            return fail(FDTD_ERR_UNSUPPORTED, "grids of a batch must be at the same step");
+    for (fdtd_grid* g : b->grids)
+        if (!g->waveforms.empty() && g->dt != b->grids[0]->dt)
+            return fail(FDTD_ERR_UNSUPPORTED, "grids with waveform sources must share one dt");
+    return FDTD_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
-int fdtd_batch_create(int, fdtd_grid* const*, fdtd_batch** out) {
-    if (out) *out = nullptr;
-    return FDTD_ERR_UNSUPPORTED;
+int fdtd_batch_create(int n, fdtd_grid* const* grids, fdtd_batch** out) {
+    if (!out || n <= 0 || !grids) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    *out = nullptr;
+    for (int q = 0; q < n; ++q) {
+        if (!grids[q]) return fail(FDTD_ERR_INVALID_ARGUMENT, "null grid %d", q);
+        if (grids[q]->precision != grids[0]->precision)
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "grids of a batch must share one precision");
+        for (int r = 0; r < q; ++r)
+            if (grids[r] == grids[q]) return fail(FDTD_ERR_INVALID_ARGUMENT, "grid %d repeated", q);
+    }
+    fdtd_batch* b = new fdtd_batch();
+    b->grids.assign(grids, grids + n);
+    b->precision = grids[0]->precision;
+    auto cleanup = [&](int rc) {
+        fdtd_batch_destroy(b);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(FDTD_ERR_HIP, "hipStreamCreate failed"));
+    for (auto& e : b->ev)
+        if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(FDTD_ERR_HIP, "hipEventCreate"));
+    if (hipMalloc(&b->d_counters, 4 * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc(&b->d_step, sizeof(long long)) != hipSuccess)
+        return cleanup(fail(FDTD_ERR_HIP, "out of device memory"));
+    if (hipMemset(b->d_counters, 0, 4 * sizeof(unsigned)) != hipSuccess)
+        return cleanup(fail(FDTD_ERR_HIP, "hipMemset failed"));
+    // default shape: widest valid V, 512 threads, 64 Ki cells per tile
+    int vc[4];
+    const int nv = batch_v_candidates(b->precision, vc);
+    int rc = FDTD_ERR_UNSUPPORTED;
+    for (int q = 0; q < nv && rc != FDTD_OK; ++q)
+        for (int nt : {512, 1024, 256})
+            if ((rc = apply_shape(b, BatchShape{vc[q], nt, 65536})) == FDTD_OK) break;
+    if (rc) return cleanup(fail(FDTD_ERR_UNSUPPORTED, "no batch launch shape tiles every grid"));
+    *out = b;
+    return FDTD_OK;
 }
-int fdtd_batch_destroy(fdtd_batch*) { return FDTD_OK; }
-int fdtd_batch_update_fields(fdtd_batch*, int) { return FDTD_ERR_UNSUPPORTED; }
-int fdtd_batch_update_fields_graph(fdtd_batch*, int, int) { return FDTD_ERR_UNSUPPORTED; }
-int fdtd_batch_synchronize(fdtd_batch*) { return FDTD_ERR_UNSUPPORTED; }
-int fdtd_batch_time(fdtd_batch*, int, float*, float*) { return FDTD_ERR_UNSUPPORTED; }
+
+int fdtd_batch_destroy(fdtd_batch* b) {
+    if (!b) return FDTD_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->d_grids) (void)hipFree(b->d_grids);
+    if (b->d_work) (void)hipFree(b->d_work);
+    if (b->d_counters) (void)hipFree(b->d_counters);
+    if (b->d_src) (void)hipFree(b->d_src);
+    if (b->d_vals) (void)hipFree(b->d_vals);
+    if (b->d_step) (void)hipFree(b->d_step);
+    for (auto& e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return FDTD_OK;
+}
+
+int fdtd_batch_update_fields(fdtd_batch* b, int nsteps) {
+    if (!b || nsteps < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (nsteps == 0) return FDTD_OK;
+    if (int rc = check_steps(b)) return rc;
+    if (int rc = prepare(b)) return rc;
+    if (int rc = enqueue_steps(b, nsteps, b->grids[0]->step, false)) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    finish_steps(b, nsteps);
+    return FDTD_OK;
+}
+
+int fdtd_batch_update_fields_graph(fdtd_batch* b, int nsteps, int replays) {
+    if (!b || nsteps <= 0 || replays < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (int rc = check_steps(b)) return rc;
+    if (int rc = prepare(b)) return rc;
+    if (!b->graph || b->graph_steps != nsteps) {
+        if (b->graph) {
+            HIP_TRY(hipGraphExecDestroy(b->graph));
+            b->graph = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_steps(b, nsteps, 0, true);
+        hipError_t e = hipStreamEndCapture(b->stream, &graph);
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(FDTD_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(FDTD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+        b->graph_steps = nsteps;
+    }
+    const long long step0 = b->grids[0]->step;
+    HIP_TRY(hipMemcpy(b->d_step, &step0, sizeof step0, hipMemcpyHostToDevice));
+    for (int r = 0; r < replays; ++r) {
+        if (r > 0) {
+            finish_steps(b, nsteps);
+            // an odd step count swaps every grid's buffers: re-point the
+            // descriptors (prepare() waits for the previous replay first)
+            if (nsteps & 1)
+                if (int rc = prepare(b)) return rc;
+        }
+        HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (replays > 0) finish_steps(b, nsteps);
+    return FDTD_OK;
+}
+
+int fdtd_batch_synchronize(fdtd_batch* b) {
+    if (!b) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return FDTD_OK;
+}
+
+int fdtd_batch_set_shape(fdtd_batch* b, int V, int threads, long long tile_cells) {
+    if (!b || V <= 0 || threads <= 0 || tile_cells <= 0)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    return apply_shape(b, BatchShape{V, threads, tile_cells});
+}
+
+int fdtd_batch_get_shape(const fdtd_batch* b, long long out[4]) {
+    if (!b || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    out[0] = b->shape.V;
+    out[1] = b->shape.NT;
+    out[2] = b->shape.target;
+    out[3] = b->nwork;
+    return FDTD_OK;
+}
+
+int fdtd_batch_time(fdtd_batch* b, int nsteps, float* ms_batched, float* ms_sequential) {
+    if (!b || nsteps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (int rc = check_steps(b)) return rc;
+    if (int rc = prepare(b)) return rc;
+    HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+    if (int rc = enqueue_steps(b, nsteps, b->grids[0]->step, false)) return rc;
+    HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+    HIP_TRY(hipEventSynchronize(b->ev[1]));
+    finish_steps(b, nsteps);
+    float t = 0;
+    HIP_TRY(hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
+    if (ms_batched) *ms_batched = t / nsteps;
+    if (ms_sequential) {
+        // the same steps as one fdtd_update_fields per grid, on one stream
+        for (fdtd_grid* g : b->grids) {
+            hipStream_t own = g->stream;
+            g->stream = b->stream;
+            const int rc = flush_sources(g);
+            g->stream = own;
+            if (rc) return rc;
+        }
+        HIP_TRY(hipEventRecord(b->ev[2], b->stream));
+        for (fdtd_grid* g : b->grids) {
+            hipStream_t own = g->stream;
+            g->stream = b->stream;
+            const int rc = fdtd_update_fields(g, nsteps);
+            g->stream = own;
+            if (rc) return rc;
+        }
+        HIP_TRY(hipEventRecord(b->ev[3], b->stream));
+        HIP_TRY(hipEventSynchronize(b->ev[3]));
+        HIP_TRY(hipEventElapsedTime(&t, b->ev[2], b->ev[3]));
+        *ms_sequential = t / nsteps;
+    }
+    return FDTD_OK;
+}
+
+int fdtd_batch_autotune(fdtd_batch* b, int reps, long long out[4], float* best_ms) {
+    if (!b || reps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (int rc = check_steps(b)) return rc;
+    if (int rc = prepare(b)) return rc;
+    // Time steady sweeps on scratch copies of every grid's state.
+    std::vector<void*> saved;
+    for (fdtd_grid* g : b->grids)
+        for (int c = 0; c < 6; ++c) {
+            void* p = nullptr;
+            const size_t bytes = g->N * g->esize;
+            HIP_TRY(hipMalloc(&p, bytes));
+            void* src = c < 3 ? g->E[g->cur][c] : g->B[g->cur][c - 3];
+            HIP_TRY(hipMemcpyAsync(p, src, bytes, hipMemcpyDeviceToDevice, b->stream));
+            saved.push_back(p);
+        }
+    const BatchShape orig = b->shape;
+    BatchShape best = orig;
+    float best_t = 1e30f;
+    int vc[4];
+    const int nv = batch_v_candidates(b->precision, vc);
+    for (int q = 0; q < nv; ++q)
+        for (int nt : {256, 512, 768, 1024})
+            for (long long tc : {16384LL, 32768LL, 65536LL, 131072LL}) {
+                if (apply_shape(b, BatchShape{vc[q], nt, tc}) != FDTD_OK) continue;
+                if (int rc = prepare(b)) return rc;
+                if (b->total_src > 0) {
+                    hipLaunchKernelGGL(fdtd::eval_sources, dim3((b->total_src + 255) / 256),
+                                       dim3(256), 0, b->stream, b->d_src, b->total_src, 1,
+                                       (const long long*)nullptr, b->grids[0]->step,
+                                       b->grids[0]->dt, b->d_vals);
+                }
+                const double* row = b->total_src > 0 ? b->d_vals : nullptr;
+                if (int rc = launch_batch(b, true, 0, row)) return rc;
+                HIP_TRY(hipEventRecord(b->ev[0], b->stream));
+                for (int r = 0; r < reps; ++r)
+                    if (int rc = launch_batch(b, true, r & 1, row)) return rc;
+                HIP_TRY(hipEventRecord(b->ev[1], b->stream));
+                HIP_TRY(hipEventSynchronize(b->ev[1]));
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+                ms /= reps;
+                if (ms < best_t) {
+                    best_t = ms;
+                    best = b->shape;
+                }
+            }
+    size_t k = 0;
+    for (fdtd_grid* g : b->grids)
+        for (int c = 0; c < 6; ++c) {
+            void* dst = c < 3 ? g->E[g->cur][c] : g->B[g->cur][c - 3];
+            HIP_TRY(hipMemcpyAsync(dst, saved[k++], g->N * g->esize, hipMemcpyDeviceToDevice,
+                                   b->stream));
+        }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (void* p : saved) HIP_TRY(hipFree(p));
+    if (best_t >= 1e30f) {
+        (void)apply_shape(b, orig);
+        return fail(FDTD_ERR_UNSUPPORTED, "no batch launch shape fits");
+    }
+    if (int rc = apply_shape(b, best)) return rc;
+    if (out) {
+        out[0] = best.V;
+        out[1] = best.NT;
+        out[2] = best.target;
+        out[3] = b->nwork;
+    }
+    if (best_ms) *best_ms = best_t;
+    return FDTD_OK;
+}
 
 }  // extern "C"

@@ -1,0 +1,136 @@
+// Internal interface of libfdtd_mi355x.so shared by its translation units
+// (fdtd_runtime.hip: single grids and the C-ABI; fdtd_batch.hip: batches).
+// Not installed; callers use include/fdtd_mi355x.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fdtd_kernels.hpp"
+#include "fdtd_mi355x.h"
+
+namespace fdtd::rt {
+
+
+constexpr double kC = 3e10;              // include/Constants.h:9
+constexpr double kPiRef = 3.14159265358; // include/Constants.h:14
+constexpr int kMaxSparseCurrents = 64;   // beyond this J is kept dense
+constexpr size_t kLdsLimit = 160 * 1024; // gfx950 LDS per workgroup
+constexpr int kReduceBlocks = 1024;
+constexpr int kValTable = 256;            // steps of source values per table fill
+
+
+// Records a message for fdtd_last_error() and returns code.
+int fail(int code, const char* fmt, ...);
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(FDTD_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+struct Tuning {
+    int V = 1, TY = 1, kchunk = 1;
+};
+
+inline size_t storage_size(int precision) {
+    return precision == FDTD_FP64 ? 8 : (precision == FDTD_FP32 ? 4 : 2);
+}
+inline size_t compute_size(int precision) { return precision == FDTD_FP64 ? 8 : 4; }
+inline size_t current_size(int precision) { return precision == FDTD_FP64 ? 8 : 4; }
+
+
+}  // namespace fdtd::rt
+
+struct fdtd_grid {
+    fdtd_parameters params{};
+    int Ni = 0, Nj = 0, Nk = 0;
+    size_t N = 0;
+    double dt = 0;
+    int precision = FDTD_FP64;
+    size_t esize = 8;
+    void* E[2][3] = {};
+    void* B[2][3] = {};
+    int cur = 0;
+    void* J[3] = {};
+    bool j_dense_active = false;
+    double* staging = nullptr;
+    double scales[6] = {1, 1, 1, 1, 1, 1};
+    hipStream_t stream = nullptr;
+    long long step = 0;
+    std::map<long long, double> sparse_j;  // key = index*3 + comp
+    std::vector<fdtd::PointSource> waveforms;
+    fdtd::PointSource* d_src = nullptr;
+    double* d_vals = nullptr;  // source values, kValTable steps x nsrc
+    int d_src_cap = 0;
+    int nsrc = 0;
+    bool src_dirty = true;
+    fdtd::rt::Tuning tune;
+    bool fused_ok = false;
+    int autoscale_interval = 0;  // FP16: rescale every this many sweeps (0 = off)
+    int autoscale_log2 = 12;
+    double* d_part = nullptr;
+    double* d_out = nullptr;
+    hipEvent_t ev[3] = {};
+};
+
+namespace fdtd::rt {
+
+// Coefficients, src/FDTD/FDTD.cpp:43-53 (FP64, reference order), then the
+// storage-scale ratios (exact powers of two) and a final rounding to C.
+template <typename C>
+fdtd::Coefs<C> make_coefs(const fdtd_grid* g) {
+    const double cdt = kC * g->dt;
+    const double cEx = cdt / g->params.dx, cEy = cdt / g->params.dy, cEz = cdt / g->params.dz;
+    const double cBx = cdt / (2.0 * g->params.dx), cBy = cdt / (2.0 * g->params.dy),
+                 cBz = cdt / (2.0 * g->params.dz);
+    const double cur = -4.0 * kPiRef * g->dt;
+    const double* s = g->scales;  // Ex Ey Ez Bx By Bz
+    fdtd::Coefs<C> c;
+    c.ex_bz = C(cEy * (s[0] / s[5]));
+    c.ex_by = C(cEz * (s[0] / s[4]));
+    c.ey_bx = C(cEz * (s[1] / s[3]));
+    c.ey_bz = C(cEx * (s[1] / s[5]));
+    c.ez_by = C(cEx * (s[2] / s[4]));
+    c.ez_bx = C(cEy * (s[2] / s[3]));
+    c.jx = C(cur * s[0]);
+    c.jy = C(cur * s[1]);
+    c.jz = C(cur * s[2]);
+    c.bx_ey = C(cBz * (s[3] / s[1]));
+    c.bx_ez = C(cBy * (s[3] / s[2]));
+    c.by_ez = C(cBx * (s[4] / s[2]));
+    c.by_ex = C(cBz * (s[4] / s[0]));
+    c.bz_ex = C(cBy * (s[5] / s[0]));
+    c.bz_ey = C(cBx * (s[5] / s[1]));
+    return c;
+}
+
+template <typename S>
+fdtd::Fields<S> fields_of(fdtd_grid* g, int buf) {
+    fdtd::Fields<S> f;
+    for (int d = 0; d < 3; ++d) {
+        f.e[d] = static_cast<S*>(g->E[buf][d]);
+        f.b[d] = static_cast<S*>(g->B[buf][d]);
+    }
+    return f;
+}
+
+// Allows `kernel` to be launched with `bytes` of dynamic LDS (needed above
+// 64 KiB); the largest value set so far is remembered per kernel.
+int allow_dynamic_lds(const void* kernel, size_t bytes);
+
+int grid_blocks(size_t n);
+int max_threads(const fdtd_grid* g, int V);
+int default_kchunk(int Nj, int Nk, int TY);
+// Sparse J values (as kind-0 sources) followed by the waveform sources.
+std::vector<fdtd::PointSource> host_sources(const fdtd_grid* g);
+// Uploads the grid's source list to its device copy when it changed.
+int flush_sources(fdtd_grid* g);
+int plain_B(fdtd_grid* g);
+int zero_fill(void* p, size_t bytes, hipStream_t s);
+
+}  // namespace fdtd::rt

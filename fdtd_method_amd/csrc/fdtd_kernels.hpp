@@ -101,12 +101,8 @@ template <typename S> struct Fields {
 // Plain three-sweep kernels (one thread per cell, in place).
 
 template <typename S, typename C>
-__global__ void __launch_bounds__(256)
-plain_update_B(Fields<S> f, Coefs<C> c, int Ni, int Nj, int Nk) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= Ni || j >= Nj) return;
+__device__ __forceinline__ void plain_B_cell(const Fields<S>& f, const Coefs<C>& c, int Ni, int Nj,
+                                             int Nk, int i, int j, int k) {
     const int plane = Ni * Nj;
     const int idx = i + j * Ni + k * plane;
     const int i_next = wrap(i + 1, Ni) + j * Ni + k * plane;
@@ -121,10 +117,20 @@ plain_update_B(Fields<S> f, Coefs<C> c, int Ni, int Nj, int Nk) {
     f.b[2][idx] = static_cast<S>(C(f.b[2][idx]) + dbz);
 }
 
+template <typename S, typename C>
+__global__ void __launch_bounds__(256)
+plain_update_B(Fields<S> f, Coefs<C> c, int Ni, int Nj, int Nk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= Ni || j >= Nj) return;
+    plain_B_cell<S, C>(f, c, Ni, Nj, Nk, i, j, k);
+}
+
 // Source values of steps n0 .. n0+nsteps-1: vals[s*nsrc + q] = J of source q
 // at t = (n0+s+1)*dt, in FP64 (n0 read from step_dev when it is non-null).
 // The sweeps then only add precomputed values.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 eval_sources(const PointSource* __restrict__ src, int nsrc, int nsteps,
              const long long* __restrict__ step_dev, long long step, double dt,
              double* __restrict__ vals) {
@@ -442,6 +448,130 @@ fused_sweep(SweepArgs<S, C, JT> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Batched sweep: many independent grids, one persistent launch per step.
+//
+// The tiles of all grids form one work list (most expensive first); each
+// workgroup takes the next tile from a global counter until the list is
+// exhausted, then the last workgroup to finish resets the counters for the
+// next launch.  Every tile runs the same fused_tile as a single grid.
+
+template <typename S, typename C> struct BatchGrid {
+    Fields<S> buf[2];        // buf[p]: the grid's buffer pair used at parity p
+    const PointSource* src;  // this grid's sources
+    int src_off;             // offset of its source values in a value-table row
+    int nsrc;
+    Coefs<C> c;
+    SweepGeom g;             // per-grid tile height and z-chunk
+};
+
+template <typename S, typename C> struct BatchArgs {
+    const BatchGrid<S, C>* grids;
+    const int4* work;        // (grid, tile_y, chunk, cost), most expensive first
+    int nwork;
+    int parity;              // the sweep reads buf[parity], writes buf[1-parity]
+    const double* vals_row;  // source values of this step (null: no sources)
+    unsigned int* counters;  // [0] next work item, [1] finished workgroups
+};
+
+// Values that are the same for every lane, moved into scalar registers
+// (readfirstlane).  Loads of the batch descriptors inside the work loop
+// cannot be proven unclobbered, so the compiler keeps them as vector loads;
+// these helpers make the results scalar again.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ float uni(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uni(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b));
+    const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                static_cast<unsigned int>(lo));
+}
+template <typename P> __device__ __forceinline__ P* uni(P* v) {
+    const unsigned long long b = reinterpret_cast<unsigned long long>(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+    return reinterpret_cast<P*>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// Sweep arguments of one grid of a batch at a given parity, in scalar
+// registers (the parity selects buffers by address, not by indexing a
+// local array, which would go to scratch memory).
+template <typename S, typename C>
+__device__ __forceinline__ SweepArgs<S, C, C> load_batch_grid(const BatchGrid<S, C>* p, int parity,
+                                                              const double* vals_row) {
+    SweepArgs<S, C, C> a;
+    const Fields<S>* in = p->buf + parity;
+    const Fields<S>* out = p->buf + (1 - parity);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        a.in.e[d] = uni(in->e[d]);
+        a.in.b[d] = uni(in->b[d]);
+        a.out.e[d] = uni(out->e[d]);
+        a.out.b[d] = uni(out->b[d]);
+        a.j[d] = nullptr;
+    }
+    a.src = uni(p->src);
+    a.vals = vals_row ? vals_row + uni(p->src_off) : nullptr;
+    a.nsrc = uni(p->nsrc);
+    a.pad = 0;
+    const Coefs<C>& c = p->c;
+    a.c.ex_bz = uni(c.ex_bz); a.c.ex_by = uni(c.ex_by); a.c.ey_bx = uni(c.ey_bx);
+    a.c.ey_bz = uni(c.ey_bz); a.c.ez_by = uni(c.ez_by); a.c.ez_bx = uni(c.ez_bx);
+    a.c.jx = uni(c.jx); a.c.jy = uni(c.jy); a.c.jz = uni(c.jz);
+    a.c.bx_ey = uni(c.bx_ey); a.c.bx_ez = uni(c.bx_ez); a.c.by_ez = uni(c.by_ez);
+    a.c.by_ex = uni(c.by_ex); a.c.bz_ex = uni(c.bz_ex); a.c.bz_ey = uni(c.bz_ey);
+    const SweepGeom& q = p->g;
+    a.g.Ni = uni(q.Ni); a.g.Nj = uni(q.Nj); a.g.Nk = uni(q.Nk); a.g.TY = uni(q.TY);
+    a.g.kchunk = uni(q.kchunk); a.g.ntile_y = uni(q.ntile_y); a.g.ntiles = uni(q.ntiles);
+    a.g.per_xcd = uni(q.per_xcd);
+    return a;
+}
+
+template <typename S, typename C, int V, bool STEADY>
+__global__ void __launch_bounds__(fused_max_threads(V, sizeof(C)))
+batch_fused_sweep(BatchArgs<S, C> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ int item;
+    for (;;) {
+        if (threadIdx.x == 0) item = static_cast<int>(atomicAdd(&a.counters[0], 1u));
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(item);
+        __syncthreads();
+        if (t >= a.nwork) break;
+        const int4 wv = a.work[t];
+        const int4 w = make_int4(uni(wv.x), uni(wv.y), uni(wv.z), 0);
+        const SweepArgs<S, C, C> sa = load_batch_grid(a.grids + w.x, a.parity, a.vals_row);
+        fused_tile<S, C, C, V, STEADY>(sa, w.y, w.z, reinterpret_cast<C*>(lds_raw));
+    }
+    if (threadIdx.x == 0) {
+        const unsigned done = atomicAdd(&a.counters[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&a.counters[0], 0u);
+            atomicExch(&a.counters[1], 0u);
+        }
+    }
+}
+
+// B half step of grid q of a batch, reading its current buffers (buf[0])
+// and coefficients from the descriptor table, so that a captured launch
+// follows the descriptors when they are re-pointed between graph replays.
+template <typename S, typename C>
+__global__ void __launch_bounds__(256)
+batch_plain_update_B(const BatchGrid<S, C>* __restrict__ grids, int q) {
+    const BatchGrid<S, C>* p = grids + q;
+    const int Ni = p->g.Ni, Nj = p->g.Nj, Nk = p->g.Nk;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= Ni || j >= Nj || k >= Nk) return;
+    const Fields<S> f = p->buf[0];
+    const Coefs<C> c = p->c;
+    plain_B_cell<S, C>(f, c, Ni, Nj, Nk, i, j, k);
+}
+
+// ---------------------------------------------------------------------------
 // Diagnostics (FP64 accumulation, deterministic two-pass reductions).
 
 constexpr int kReduceThreads = 256;
@@ -522,7 +652,7 @@ diff_partial(const S* __restrict__ a, double inv_scale, const double* __restrict
 
 // Final pass over `nblk` partial records of width w; column c is a sum
 // unless (max_mask >> c) & 1.  Deterministic: one block, fixed order.
-__global__ void __launch_bounds__(kReduceThreads)
+static __global__ void __launch_bounds__(kReduceThreads)
 reduce_partials(const double* __restrict__ part, int nblk, int w, unsigned max_mask,
                 double* __restrict__ out) {
     __shared__ double sh[kReduceThreads];
@@ -566,14 +696,14 @@ scale_array(S* __restrict__ x, size_t n, float factor) {
 
 // Fill with zeros (used instead of hipMemsetAsync so that step sequences can
 // be captured in HIP graphs).
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 zero_bytes(uint4* __restrict__ p, size_t n16) {
     for (size_t i = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; i < n16;
          i += static_cast<size_t>(gridDim.x) * 256)
         p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
 
-__global__ void advance_step(long long* step, long long by) { *step += by; }
+static __global__ void advance_step(long long* step, long long by) { *step += by; }
 
 // Order-independent checksum of the bit patterns of an array: the sum mod
 // 2^64 of a 64-bit mix of (bits, index).  Integer addition makes the result
@@ -604,7 +734,7 @@ checksum_partial(const S* __restrict__ x, size_t n, unsigned long long* __restri
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
-__global__ void __launch_bounds__(kReduceThreads)
+static __global__ void __launch_bounds__(kReduceThreads)
 checksum_final(const unsigned long long* __restrict__ part, int nblk,
                unsigned long long* __restrict__ out) {
     __shared__ unsigned long long sh[kReduceThreads];
@@ -616,7 +746,7 @@ checksum_final(const unsigned long long* __restrict__ part, int nblk,
 
 // Streaming copy of a known byte count, 16 B per lane: used to calibrate the
 // HBM byte counters against an exactly known traffic.
-__global__ void __launch_bounds__(256)
+static __global__ void __launch_bounds__(256)
 copy_calibration(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
     for (size_t i = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; i < n16;
          i += static_cast<size_t>(gridDim.x) * 256)

@@ -18,17 +18,9 @@
 #include <string>
 #include <vector>
 
-#include "fdtd_kernels.hpp"
-#include "fdtd_mi355x.h"
+#include "fdtd_internal.hpp"
 
-namespace {
-
-constexpr double kC = 3e10;              // include/Constants.h:9
-constexpr double kPiRef = 3.14159265358; // include/Constants.h:14
-constexpr int kMaxSparseCurrents = 64;   // beyond this J is kept dense
-constexpr size_t kLdsLimit = 160 * 1024; // gfx950 LDS per workgroup
-constexpr int kReduceBlocks = 1024;
-constexpr int kValTable = 256;            // steps of source values per table fill
+namespace fdtd::rt {
 
 thread_local std::string g_last_error;
 
@@ -42,87 +34,26 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                  \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(FDTD_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
+}  // namespace fdtd::rt
 
-struct Tuning {
-    int V = 1, TY = 1, kchunk = 1;
-};
+namespace fdtd::rt {
 
-size_t storage_size(int precision) {
-    return precision == FDTD_FP64 ? 8 : (precision == FDTD_FP32 ? 4 : 2);
-}
-size_t compute_size(int precision) { return precision == FDTD_FP64 ? 8 : 4; }
-size_t current_size(int precision) { return precision == FDTD_FP64 ? 8 : 4; }
 
-}  // namespace
-
-struct fdtd_grid {
-    fdtd_parameters params{};
-    int Ni = 0, Nj = 0, Nk = 0;
-    size_t N = 0;
-    double dt = 0;
-    int precision = FDTD_FP64;
-    size_t esize = 8;
-    void* E[2][3] = {};
-    void* B[2][3] = {};
-    int cur = 0;
-    void* J[3] = {};
-    bool j_dense_active = false;
-    double* staging = nullptr;
-    double scales[6] = {1, 1, 1, 1, 1, 1};
-    hipStream_t stream = nullptr;
-    long long step = 0;
-    std::map<long long, double> sparse_j;  // key = index*3 + comp
-    std::vector<fdtd::PointSource> waveforms;
-    fdtd::PointSource* d_src = nullptr;
-    double* d_vals = nullptr;  // source values, kValTable steps x nsrc
-    int d_src_cap = 0;
-    int nsrc = 0;
-    bool src_dirty = true;
-    Tuning tune;
-    bool fused_ok = false;
-    int autoscale_interval = 0;  // FP16: rescale every this many sweeps (0 = off)
-    int autoscale_log2 = 12;
-    double* d_part = nullptr;
-    double* d_out = nullptr;
-    hipEvent_t ev[3] = {};
-};
-
-namespace {
-
-// -------------------------------------------------------------------------
-// Coefficients, src/FDTD/FDTD.cpp:43-53 (FP64, reference order), then the
-// storage-scale ratios (exact powers of two) and a final rounding to C.
-template <typename C>
-fdtd::Coefs<C> make_coefs(const fdtd_grid* g) {
-    const double cdt = kC * g->dt;
-    const double cEx = cdt / g->params.dx, cEy = cdt / g->params.dy, cEz = cdt / g->params.dz;
-    const double cBx = cdt / (2.0 * g->params.dx), cBy = cdt / (2.0 * g->params.dy),
-                 cBz = cdt / (2.0 * g->params.dz);
-    const double cur = -4.0 * kPiRef * g->dt;
-    const double* s = g->scales;  // Ex Ey Ez Bx By Bz
-    fdtd::Coefs<C> c;
-    c.ex_bz = C(cEy * (s[0] / s[5]));
-    c.ex_by = C(cEz * (s[0] / s[4]));
-    c.ey_bx = C(cEz * (s[1] / s[3]));
-    c.ey_bz = C(cEx * (s[1] / s[5]));
-    c.ez_by = C(cEx * (s[2] / s[4]));
-    c.ez_bx = C(cEy * (s[2] / s[3]));
-    c.jx = C(cur * s[0]);
-    c.jy = C(cur * s[1]);
-    c.jz = C(cur * s[2]);
-    c.bx_ey = C(cBz * (s[3] / s[1]));
-    c.bx_ez = C(cBy * (s[3] / s[2]));
-    c.by_ez = C(cBx * (s[4] / s[2]));
-    c.by_ex = C(cBz * (s[4] / s[0]));
-    c.bz_ex = C(cBy * (s[5] / s[0]));
-    c.bz_ey = C(cBx * (s[5] / s[1]));
-    return c;
+int allow_dynamic_lds(const void* kernel, size_t bytes) {
+    static std::mutex mu;
+    static std::map<const void*, size_t> allowed;
+    if (bytes <= 64 * 1024) return FDTD_OK;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& cur = allowed[kernel];
+    if (bytes <= cur) return FDTD_OK;
+    const hipError_t e =
+        hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(FDTD_ERR_HIP, "cannot allow %zu bytes of LDS: %s", bytes, hipGetErrorString(e));
+    }
+    cur = bytes;
+    return FDTD_OK;
 }
 
 int grid_blocks(size_t n) {
@@ -222,20 +153,8 @@ fdtd::SweepGeom make_geom(const fdtd_grid* g, const Tuning& t) {
     return s;
 }
 
-template <typename S>
-fdtd::Fields<S> fields_of(fdtd_grid* g, int buf) {
-    fdtd::Fields<S> f;
-    for (int d = 0; d < 3; ++d) {
-        f.e[d] = static_cast<S*>(g->E[buf][d]);
-        f.b[d] = static_cast<S*>(g->B[buf][d]);
-    }
-    return f;
-}
 
-// Push the combined source list (sparse J values as kind-0 sources, then
-// waveform sources) to the device when it changed.
-int flush_sources(fdtd_grid* g) {
-    if (!g->src_dirty) return FDTD_OK;
+std::vector<fdtd::PointSource> host_sources(const fdtd_grid* g) {
     std::vector<fdtd::PointSource> all;
     all.reserve(g->sparse_j.size() + g->waveforms.size());
     for (const auto& kv : g->sparse_j) {
@@ -250,6 +169,13 @@ int flush_sources(fdtd_grid* g) {
         all.push_back(s);
     }
     all.insert(all.end(), g->waveforms.begin(), g->waveforms.end());
+    return all;
+}
+
+// Push the combined source list to the device when it changed.
+int flush_sources(fdtd_grid* g) {
+    if (!g->src_dirty) return FDTD_OK;
+    const std::vector<fdtd::PointSource> all = host_sources(g);
     if ((int)all.size() > g->d_src_cap) {
         if (g->d_src) HIP_TRY(hipFree(g->d_src));
         if (g->d_vals) HIP_TRY(hipFree(g->d_vals));
@@ -346,11 +272,7 @@ int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out) {
     a.c = make_coefs<C>(g);
     a.g = make_geom(g, t);
     const size_t lds = fdtd::fused_lds_bytes(t.TY, g->Ni, sizeof(C));
-    static std::once_flag once;
-    std::call_once(once, [&] {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsLimit);
-    });
+    if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
     const int threads = (t.TY + 1) * (g->Ni / V);
     hipLaunchKernelGGL(kern, dim3(8 * a.g.per_xcd), dim3(threads), lds, g->stream, a);
     HIP_TRY(hipGetLastError());
@@ -502,11 +424,13 @@ void* field_ptr(fdtd_grid* g, int c) {
     return c < 3 ? g->E[g->cur][c] : g->B[g->cur][c - 3];
 }
 
-}  // namespace
+}  // namespace fdtd::rt
 
 // =============================================================================
 // C-ABI
 // =============================================================================
+
+using namespace fdtd::rt;
 
 extern "C" {
 
@@ -902,7 +826,7 @@ int fdtd_get_scales(const fdtd_grid* g, double scales[6]) {
 
 }  // extern "C"
 
-namespace {
+namespace fdtd::rt {
 
 // Power-of-two scales putting max|stored| in [2^(target-1), 2^target).
 void target_scales(const fdtd_grid* g, const double mx[6], int target_log2, double floor_mag,
@@ -942,7 +866,7 @@ int rescale_to_target(fdtd_grid* g, bool current_only) {
     return apply_scales(g, s, current_only);
 }
 
-}  // namespace
+}  // namespace fdtd::rt
 
 extern "C" {
 

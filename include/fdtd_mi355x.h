@@ -172,7 +172,10 @@ int fdtd_time_update_fields(fdtd_grid* grid, int nsteps, int path, float* ms_tot
 int fdtd_calibration_copy(size_t bytes, int iters, float* ms);
 
 /* ---- Batched solves: many independent grids advanced by one persistent
- * launch per step.  All grids must share one precision. ---- */
+ * launch per step (a work queue over every grid's tiles).  All grids must
+ * share one precision and one step count; grids with waveform sources one
+ * dt; currents must be point sources or sparse values (no dense J).  The
+ * grids stay usable individually between batch calls. ---- */
 int fdtd_batch_create(int n, fdtd_grid* const* grids, fdtd_batch** out);
 int fdtd_batch_destroy(fdtd_batch* batch);
 /* nsteps steps of every grid (same semantics as fdtd_update_fields). */
@@ -181,8 +184,16 @@ int fdtd_batch_update_fields(fdtd_batch* batch, int nsteps);
 int fdtd_batch_update_fields_graph(fdtd_batch* batch, int nsteps, int replays);
 int fdtd_batch_synchronize(fdtd_batch* batch);
 /* ms per step of the batched launch and of one launch per grid, timed with
- * HIP events over nsteps steps each (fields advance by 2*nsteps). */
+ * HIP events over nsteps steps each (fields advance by 2*nsteps when
+ * ms_sequential is requested, else by nsteps). */
 int fdtd_batch_time(fdtd_batch* batch, int nsteps, float* ms_batched, float* ms_sequential);
+/* Launch shape of the batched sweep: V cells per lane, threads per
+ * workgroup, target cells per tile (sets each grid's z-chunk).
+ * get: out = {V, threads, tile_cells, work items}.  The autotuner times the
+ * candidate shapes on scratch copies of the state and keeps the fastest. */
+int fdtd_batch_set_shape(fdtd_batch* batch, int V, int threads, long long tile_cells);
+int fdtd_batch_get_shape(const fdtd_batch* batch, long long out[4]);
+int fdtd_batch_autotune(fdtd_batch* batch, int reps, long long out[4], float* best_ms);
 
 #ifdef __cplusplus
 }
