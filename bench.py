@@ -24,6 +24,17 @@ Reported:
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
 python bench.py --profile fp32|fp64|fp16   (only that path: for rocprofv3)
+
+Other workloads (not the headline; each prints one JSON line):
+python bench.py --workload batch [--grids 256] [--steps 500]
+    256 independent FP32 grids, each dimension from {32,48,...,160} (seeded),
+    one plane wave (wavelength 8-32 cells) and one Gaussian-modulated Jz
+    source each: one persistent launch per step, HIP-graph captured,
+    autotuned shape; aggregate Gcell/s, and ms/step against one launch per grid
+python bench.py --workload precision [--n 768] [--steps 2000]
+    768^3, superposition of 8 seeded plane waves plus the centre Jz source:
+    FP64, FP32 and FP16 (per-component power-of-two scales) runs, per
+    component error and energy drift against FP64
 """
 from __future__ import annotations
 
@@ -122,6 +133,103 @@ def cpu_baseline(n):
     return {k: rec[k] for k in ("value", "unit", "cores", "kind", "sample")}
 
 
+def bench_batch(a):
+    from fdtd_method_amd import Batch, plane_wave_batch
+    K = a.steps
+    t0 = time.perf_counter()
+    grids, desc = plane_wave_batch(n_grids=a.grids, seed=a.seed)
+    ref, _ = plane_wave_batch(n_grids=min(8, a.grids), seed=a.seed)
+    log(f"batch: {len(grids)} grids, {sum(g.cells for g in grids) / 1e6:.1f} M cells, "
+        f"built in {time.perf_counter() - t0:.1f} s")
+    b = Batch(grids)
+    shape, ms_tune = b.autotune(reps=3)
+    log(f"batch autotune: {shape} ({ms_tune:.3f} ms/sweep)")
+    # parity on the first grids: batched graph replays == single-grid runs
+    b.update_fields_graph(10, 2)
+    for g in ref:
+        g.update_fields(20)
+    same = all(r.checksum() == g.checksum() for r, g in zip(ref, grids))
+    for g in ref:
+        g.close()
+    for q, (g, d) in enumerate(zip(grids, desc)):
+        g.zero_fields()
+        g.add_plane_wave(d["m"], d["amp"], d["phase"],
+                         np.random.default_rng(a.seed * 1000 + q).standard_normal(3))
+    b.update_fields_graph(K, 0)  # capture only
+    t0 = time.perf_counter()
+    b.update_fields_graph(K, 1)
+    wall = time.perf_counter() - t0
+    cells = float(b.cells)
+    ms_b, ms_s = b.time(min(K, 50))
+    out = {
+        "metric": "aggregate Gcell-updates/s, batched persistent launch per step",
+        "value": cells * K / wall / 1e9, "unit": "Gcell-updates/s", "n_gpus": 1, "steps": K,
+        "ms_per_step": 1e3 * wall / K, "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": (f"{len(grids)} independent periodic FP32 grids, dims from "
+                                "{32,48,...,160} (seed %d), one plane wave and one "
+                                "Gaussian-modulated Jz each, %d steps, graph-captured" % (a.seed, K)),
+                   "cells": cells, "shape": shape},
+        "alg_gbs": 48.0 * cells / (1e-3 * 1e3 * wall / K) / 1e9,
+        "batched_ms_per_step": ms_b, "sequential_ms_per_step": ms_s,
+        "speedup_vs_one_launch_per_grid": ms_s / ms_b,
+        "bit_identical_to_single_grid_runs": same,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_precision(a):
+    n, K = a.n, a.steps
+    dt = 0.5 / (C * np.sqrt(3.0))
+    rng = np.random.default_rng(a.seed)
+    waves = []
+    while len(waves) < 8:
+        m = [int(x) for x in rng.integers(-8, 9, 3)]
+        if not any(m):
+            continue
+        waves.append(dict(m=m, amp=float(rng.uniform(0.1, 1.0)),
+                          phase=float(rng.uniform(0, 2 * np.pi)),
+                          pol=[float(x) for x in rng.standard_normal(3)]))
+    src = dict(i=n // 2, j=n // 2, k=n // 2, comp=2, kind=1, amp=1.0, period=20 * dt,
+               t0=120 * dt, sigma=40 * dt)
+    res, grids = {}, {}
+    for prec in (Precision.FP64, Precision.FP32, Precision.FP16):
+        g = FDTD(Parameters(n, n, n), dt, prec)
+        if prec == Precision.FP16:
+            g.set_scales([2.0 ** 10] * 6)  # sum of 8 amplitudes <= 8 -> max ~2^13
+        for w in waves:
+            g.add_plane_wave(w["m"], w["amp"], w["phase"], w["pol"])
+        if prec == Precision.FP16:
+            g.autoscale(target_log2=12)
+            g.set_autoscale_interval(100, 12)
+        g.set_point_sources([src])
+        g.autotune(reps=3)
+        e0 = energy(g)
+        g.synchronize()
+        t0 = time.perf_counter()
+        ms_total, ms_main = g.time_update_fields(K)
+        wall = time.perf_counter() - t0
+        res[prec.name] = dict(gcells=float(n) ** 3 * K / wall / 1e9, ms_per_step=1e3 * wall / K,
+                              energy_start=e0, energy_end=energy(g))
+        grids[prec.name] = g
+        log(f"{prec.name}: {res[prec.name]['gcells']:.1f} Gcell/s")
+    e32 = errors(grids["FP32"], grids["FP64"])
+    e16 = errors(grids["FP16"], grids["FP64"])
+    e64 = res["FP64"]["energy_end"]
+    for nm in ("FP32", "FP16"):
+        res[nm]["energy_drift_vs_fp64"] = res[nm]["energy_end"] / e64 - 1.0
+    out = {
+        "metric": "relative L2 / max-abs error vs FP64 and Gcell-updates/s per precision",
+        "value": e32["rel_l2"], "unit": "relative L2 (FP32 vs FP64)", "n_gpus": 1, "steps": K,
+        "higher_is_better": False, "dtype": "f32/f16 vs f64", "data": "synthetic",
+        "config": {"workload": (f"{n}^3 periodic grid, 8 seeded plane waves (seed {a.seed}) "
+                                f"plus the centre Gaussian-modulated Jz, {K} steps"),
+                   "waves": waves},
+        "paths": res, "errors_vs_fp64": {"fp32": e32, "fp16": e16},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,12 +241,19 @@ def main():
     ap.add_argument("--profile", choices=["fp32", "fp64", "fp16"], default=None)
     ap.add_argument("--tuning", default=None,
                     help="V,TY,kchunk launch shape for --profile (default: autotune)")
+    ap.add_argument("--workload", choices=["point", "batch", "precision"], default="point")
+    ap.add_argument("--grids", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=2024)
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): this build has no "
               "multi-GPU path; see the sharding design in DESIGN.md.", file=sys.stderr)
         sys.exit(2)
     n, K, W = a.n, a.steps, a.warmup
+    if a.workload == "batch":
+        return bench_batch(a)
+    if a.workload == "precision":
+        return bench_precision(a)
 
     if a.profile:
         prec = {"fp32": Precision.FP32, "fp64": Precision.FP64, "fp16": Precision.FP16}[a.profile]

@@ -26,7 +26,7 @@ EXPORTED = [
     "fdtd_zero_currents", "fdtd_set_current_values", "fdtd_set_point_sources",
     "fdtd_update_fields", "fdtd_update_fields_3sweep", "fdtd_get_step", "fdtd_set_step",
     "fdtd_synchronize", "fdtd_get_info", "fdtd_sum_squares", "fdtd_max_abs", "fdtd_compare",
-    "fdtd_checksum",
+    "fdtd_checksum", "fdtd_add_plane_wave",
     "fdtd_set_scales", "fdtd_get_scales", "fdtd_autoscale", "fdtd_set_autoscale_interval",
     "fdtd_zero_fields", "fdtd_set_tuning",
     "fdtd_get_tuning", "fdtd_autotune", "fdtd_time_update_fields", "fdtd_calibration_copy",
@@ -95,6 +95,8 @@ def load():
         "fdtd_max_abs": ([_vp, _dp], _i),
         "fdtd_compare": ([_vp, _vp, _dp], _i),
         "fdtd_checksum": ([_vp, ctypes.POINTER(ctypes.c_ulonglong)], _i),
+        "fdtd_add_plane_wave": ([_vp, ctypes.POINTER(_i), ctypes.c_double, ctypes.c_double,
+                                 _dp], _i),
         "fdtd_set_scales": ([_vp, _dp], _i),
         "fdtd_get_scales": ([_vp, _dp], _i),
         "fdtd_autoscale": ([_vp, _i, ctypes.c_double], _i),

@@ -87,17 +87,16 @@ class Batch:
         return sum(g.cells for g in self.grids)
 
 
-def yee_plane_wave(shape, m, amp, phase, pol_seed):
-    """E and B of a plane wave exp(i k.r) with integer wave numbers m (so it
-    is periodic on the grid), sampled at the Yee positions of each component
-    (Ex at (i+1/2,j,k), Ey at (i,j+1/2,k), Ez at (i,j,k+1/2), Bx at
-    (i,j+1/2,k+1/2), By at (i+1/2,j,k+1/2), Bz at (i+1/2,j+1/2,k)); dx = 1.
-    Gaussian units: B = khat x E."""
+def yee_plane_wave(shape, m, amp, phase, pol):
+    """Host (numpy) form of FDTD.add_plane_wave: E = amp*e*cos(k.r+phase),
+    B = khat x E, with integer wave numbers m (periodic on the grid), each
+    component at its Yee position (Ex at (i+1/2,j,k), Ey at (i,j+1/2,k), Ez
+    at (i,j,k+1/2), Bx at (i,j+1/2,k+1/2), By at (i+1/2,j,k+1/2), Bz at
+    (i+1/2,j+1/2,k)); dx = 1, Gaussian units."""
     Ni, Nj, Nk = shape
     k = 2.0 * np.pi * np.array([m[0] / Ni, m[1] / Nj, m[2] / Nk])
     khat = k / np.linalg.norm(k)
-    rng = np.random.default_rng(pol_seed)
-    v = rng.standard_normal(3)
+    v = np.asarray(pol, dtype=float)
     e = v - np.dot(v, khat) * khat
     e /= np.linalg.norm(e)
     b = np.cross(khat, e)
@@ -116,7 +115,8 @@ def plane_wave_batch(n_grids=256, seed=2024, dims=(32, 160, 16), precision=Preci
     """The seeded many-grid workload: each dimension uniform in
     {dims[0], dims[0]+dims[2], ..., dims[1]}, dx = 1, dt = 0.5/(c sqrt 3),
     one plane wave (random integer wave vector with wavelength in lam cells,
-    amplitude U[0.1, 1], phase U[0, 2 pi)) and one Gaussian-modulated Jz
+    amplitude U[0.1, 1], phase U[0, 2 pi), random polarization), generated
+    on the device, and one Gaussian-modulated Jz
     point source at a random cell (period U[16, 40] dt, sigma 40 dt, peak
     120 dt).  Returns (grids, description)."""
     rng = np.random.default_rng(seed)
@@ -135,8 +135,8 @@ def plane_wave_batch(n_grids=256, seed=2024, dims=(32, 160, 16), precision=Preci
         amp = float(rng.uniform(0.1, 1.0))
         phase = float(rng.uniform(0.0, 2.0 * np.pi))
         g = FDTD(Parameters(*shape), dt, precision)
-        for c, a in enumerate(yee_plane_wave(shape, m, amp, phase, seed * 1000 + q)):
-            g.upload(c, a)
+        pol = np.random.default_rng(seed * 1000 + q).standard_normal(3)
+        g.add_plane_wave(m, amp, phase, pol)
         src = None
         if with_sources:
             src = dict(i=int(rng.integers(shape[0])), j=int(rng.integers(shape[1])),

@@ -685,6 +685,36 @@ convert_to_f64(const S* __restrict__ src, double* __restrict__ dst, size_t n, do
         dst[i] = static_cast<double>(src[i]) * inv_scale;
 }
 
+// Adds a plane wave amp*pol*cos(k.r + phase) to E and (khat x E) to B,
+// each component sampled at its Yee position (Ex at (i+1/2,j,k), Ey at
+// (i,j+1/2,k), Ez at (i,j,k+1/2), Bx at (i,j+1/2,k+1/2), By at
+// (i+1/2,j,k+1/2), Bz at (i+1/2,j+1/2,k)); k in radians per cell, values in
+// FP64, stored with the component's scale.
+template <typename S>
+__global__ void __launch_bounds__(256)
+add_plane_wave(Fields<S> f, int Ni, int Nj, int Nk, double kx, double ky, double kz, double phase,
+               double ex, double ey, double ez, double bx, double by, double bz,
+               double s0, double s1, double s2, double s3, double s4, double s5) {
+    const size_t n = static_cast<size_t>(Ni) * Nj * Nk;
+    for (size_t idx = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; idx < n;
+         idx += static_cast<size_t>(gridDim.x) * 256) {
+        const double i = static_cast<double>(idx % Ni);
+        const double j = static_cast<double>((idx / Ni) % Nj);
+        const double k = static_cast<double>(idx / (static_cast<size_t>(Ni) * Nj));
+        const double amp[6] = {ex * s0, ey * s1, ez * s2, bx * s3, by * s4, bz * s5};
+        const double ox[6] = {0.5, 0.0, 0.0, 0.0, 0.5, 0.5};
+        const double oy[6] = {0.0, 0.5, 0.0, 0.5, 0.0, 0.5};
+        const double oz[6] = {0.0, 0.0, 0.5, 0.5, 0.5, 0.0};
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            if (amp[c] == 0.0) continue;
+            const double ph = kx * (i + ox[c]) + ky * (j + oy[c]) + kz * (k + oz[c]) + phase;
+            S* dst = c < 3 ? f.e[c] : f.b[c - 3];
+            dst[idx] = static_cast<S>(static_cast<double>(dst[idx]) + amp[c] * cos(ph));
+        }
+    }
+}
+
 // Multiply a storage array by an exact power of two (FP16 rescaling).
 template <typename S>
 __global__ void __launch_bounds__(256)

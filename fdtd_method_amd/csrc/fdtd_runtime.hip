@@ -67,7 +67,7 @@ int grid_blocks(size_t n) {
 int v_candidates(int precision, int* out) {
     if (precision == FDTD_FP64) { out[0] = 2; out[1] = 1; out[2] = 4; return 3; }
     if (precision == FDTD_FP32) { out[0] = 4; out[1] = 2; out[2] = 1; out[3] = 8; return 4; }
-    out[0] = 8; out[1] = 4; out[2] = 2; out[3] = 1;
+    out[0] = 4; out[1] = 2; out[2] = 8; out[3] = 1;
     return 4;
 }
 
@@ -785,6 +785,46 @@ int fdtd_checksum(fdtd_grid* g, unsigned long long out[6]) {
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipMemcpyAsync(out, res, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return FDTD_OK;
+}
+
+int fdtd_add_plane_wave(fdtd_grid* g, const int m[3], double amplitude, double phase,
+                        const double polarization[3]) {
+    if (!g || !m || !polarization) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (!m[0] && !m[1] && !m[2]) return fail(FDTD_ERR_INVALID_ARGUMENT, "zero wave vector");
+    const double kx = 2.0 * M_PI * m[0] / g->Ni, ky = 2.0 * M_PI * m[1] / g->Nj,
+                 kz = 2.0 * M_PI * m[2] / g->Nk;
+    const double kn = std::sqrt(kx * kx + ky * ky + kz * kz);
+    const double kh[3] = {kx / kn, ky / kn, kz / kn};
+    double e[3] = {polarization[0], polarization[1], polarization[2]};
+    const double dot = e[0] * kh[0] + e[1] * kh[1] + e[2] * kh[2];
+    for (int d = 0; d < 3; ++d) e[d] -= dot * kh[d];
+    const double en = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    if (!(en > 1e-12)) return fail(FDTD_ERR_INVALID_ARGUMENT, "polarization parallel to k");
+    for (int d = 0; d < 3; ++d) e[d] = amplitude * e[d] / en;
+    const double b[3] = {kh[1] * e[2] - kh[2] * e[1], kh[2] * e[0] - kh[0] * e[2],
+                         kh[0] * e[1] - kh[1] * e[0]};
+    const double* sc = g->scales;
+    const dim3 grid(grid_blocks(g->N)), block(256);
+    switch (g->precision) {
+        case FDTD_FP64:
+            hipLaunchKernelGGL(fdtd::add_plane_wave<double>, grid, block, 0, g->stream,
+                               fields_of<double>(g, g->cur), g->Ni, g->Nj, g->Nk, kx, ky, kz, phase,
+                               e[0], e[1], e[2], b[0], b[1], b[2], sc[0], sc[1], sc[2], sc[3], sc[4], sc[5]);
+            break;
+        case FDTD_FP32:
+            hipLaunchKernelGGL(fdtd::add_plane_wave<float>, grid, block, 0, g->stream,
+                               fields_of<float>(g, g->cur), g->Ni, g->Nj, g->Nk, kx, ky, kz, phase,
+                               e[0], e[1], e[2], b[0], b[1], b[2], sc[0], sc[1], sc[2], sc[3], sc[4], sc[5]);
+            break;
+        default:
+            hipLaunchKernelGGL(fdtd::add_plane_wave<_Float16>, grid, block, 0, g->stream,
+                               fields_of<_Float16>(g, g->cur), g->Ni, g->Nj, g->Nk, kx, ky, kz,
+                               phase, e[0], e[1], e[2], b[0], b[1], b[2], sc[0], sc[1], sc[2],
+                               sc[3], sc[4], sc[5]);
+    }
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(g->stream));
     return FDTD_OK;
 }

@@ -238,6 +238,15 @@ class FDTD:
         _lib.check(self._L.fdtd_compare(self._h, ref._h, out))
         return [tuple(out[3 * c:3 * c + 3]) for c in range(6)]
 
+    def add_plane_wave(self, m, amplitude, phase, polarization) -> None:
+        """Add amplitude*e*cos(k.r+phase) to E and khat x E to B (device-side,
+        Yee positions; k = 2 pi (m0/Ni, m1/Nj, m2/Nk) per cell)."""
+        self._flush()
+        mm = (ctypes.c_int * 3)(*[int(x) for x in m])
+        pol = (ctypes.c_double * 3)(*[float(x) for x in polarization])
+        _lib.check(self._L.fdtd_add_plane_wave(self._h, mm, float(amplitude), float(phase), pol))
+        self._invalidate()
+
     def checksum(self):
         """Order-independent 64-bit checksums of the stored bits of Ex..Bz."""
         out = (ctypes.c_ulonglong * 6)()

@@ -135,6 +135,15 @@ int fdtd_max_abs(fdtd_grid* grid, double out[6]);
  * out[3c+2] = max |t-r|. */
 int fdtd_compare(fdtd_grid* test, fdtd_grid* ref, double out[18]);
 
+/* Initial conditions: add a plane wave with integer wave numbers m (so it is
+ * periodic on the grid; k = 2 pi (m0/Ni, m1/Nj, m2/Nk) per cell), E =
+ * amplitude * e * cos(k.r + phase) with e the polarization made
+ * perpendicular to k, and B = khat x E (Gaussian units), each component at
+ * its Yee position.  Generalises the reference tests' axis-aligned wave
+ * (src/FDTD/test_FDTD.cpp:5-51). */
+int fdtd_add_plane_wave(fdtd_grid* grid, const int m[3], double amplitude, double phase,
+                        const double polarization[3]);
+
 /* Order-independent 64-bit checksum of the stored bit patterns of each of
  * Ex..Bz (sum mod 2^64 of a hash of (bits, index)): equal arrays give equal
  * checksums, so bit-exactness can be checked on the device at any size. */
