@@ -31,6 +31,10 @@ python bench.py --workload batch [--grids 256] [--steps 500]
     one plane wave (wavelength 8-32 cells) and one Gaussian-modulated Jz
     source each: one persistent launch per step, HIP-graph captured,
     autotuned shape; aggregate Gcell/s, and ms/step against one launch per grid
+python bench.py --workload blocked [--dims 1024,1024,256] [--steps 4000]
+    FP32 with 4 seeded Jz sources of distinct periods: the full-row sweep
+    against the autotuned best of tiled/temporally blocked passes
+    (L steps per pass over HBM); effective roofline, error vs FP64
 python bench.py --workload precision [--n 768] [--steps 2000]
     768^3, superposition of 8 seeded plane waves plus the centre Jz source:
     FP64, FP32 and FP16 (per-component power-of-two scales) runs, per
@@ -230,6 +234,79 @@ def bench_precision(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_blocked(a):
+    Ni, Nj, Nk = (int(x) for x in a.dims.split(","))
+    K = a.steps
+    dt = 0.5 / (C * np.sqrt(3.0))
+    rng = np.random.default_rng(a.seed)
+    periods = rng.permutation(np.arange(16, 41))[:4]
+    srcs = [dict(i=int(rng.integers(Ni)), j=int(rng.integers(Nj)), k=int(rng.integers(Nk)),
+                 comp=2, kind=1, amp=1.0, period=float(p) * dt, t0=0.0, sigma=0.0)
+            for p in periods]
+    cells = float(Ni) * Nj * Nk
+
+    def make(prec):
+        g = FDTD(Parameters(Ni, Nj, Nk), dt, prec)
+        g.set_point_sources(srcs)
+        return g
+
+    g = make(Precision.FP32)
+    g.autotune(reps=3)
+    full_shape = g.get_tuning()
+    # full-row fused sweep over the whole run
+    g.time_update_fields(4)
+    g.zero_fields()
+    t0 = time.perf_counter()
+    _, ms_full = g.time_update_fields(K)
+    wall_full = time.perf_counter() - t0
+    ms_full /= K
+    # blocked passes: short timing of candidate configurations, then the run
+    g.zero_fields()
+    best = None
+    for cand in [(4, 1, 32, 32), (3, 1, 32, 32), (4, 1, 64, 16), (3, 1, 64, 16), (2, 2, 64, 16),
+                 (2, 1, 32, 32), (4, 2, 32, 16)]:
+        try:
+            g.set_time_blocking(*cand, 0)
+        except Exception:
+            continue
+        L = cand[0]
+        g.time_update_fields(2 * L)
+        _, m = g.time_update_fields(12 * L)
+        m /= 12 * L
+        log(f"  blocking {g.get_time_blocking()}: {m:.3f} ms/step")
+        if best is None or m < best[0]:
+            best = (m, cand)
+    g.set_time_blocking(*best[1], 0)
+    cfg = g.get_time_blocking()
+    g.zero_fields()
+    g.synchronize()
+    t0 = time.perf_counter()
+    _, ms_main = g.time_update_fields(K)
+    wall = time.perf_counter() - t0
+    log(f"full-row {ms_full:.3f} ms/step, blocked {cfg} {ms_main / K:.3f} ms/step")
+    ref = make(Precision.FP64)
+    ref.autotune(reps=2)
+    ref.time_update_fields(K)
+    err = errors(g, ref)
+    eff = 48.0 * cells / (ms_main / K * 1e-3) / 1e9
+    out = {
+        "metric": "Gcell-updates/s and effective %HBM roofline with temporal blocking",
+        "value": cells * K / wall / 1e9, "unit": "Gcell-updates/s", "n_gpus": 1, "steps": K,
+        "ms_per_step": 1e3 * wall / K, "higher_is_better": True, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": (f"{Ni}x{Nj}x{Nk} periodic FP32 grid, 4 Jz point sources "
+                                f"(periods {[int(p) for p in periods]} dt, seed {a.seed}), {K} steps"),
+                   "blocking": cfg, "full_row_shape": full_shape},
+        "effective_gbs": eff, "effective_frac_of_8tbs": eff / PEAK_GBS,
+        "full_row_ms_per_step": ms_full, "blocked_ms_per_step": ms_main / K,
+        "full_row_gcells": cells * K / wall_full / 1e9,
+        "full_row_effective_frac_of_8tbs": 48.0 * cells / (ms_full * 1e-3) / 1e9 / PEAK_GBS,
+        "speedup_vs_full_row": ms_full / (ms_main / K),
+        "errors_vs_fp64": err,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,7 +318,9 @@ def main():
     ap.add_argument("--profile", choices=["fp32", "fp64", "fp16"], default=None)
     ap.add_argument("--tuning", default=None,
                     help="V,TY,kchunk launch shape for --profile (default: autotune)")
-    ap.add_argument("--workload", choices=["point", "batch", "precision"], default="point")
+    ap.add_argument("--workload", choices=["point", "batch", "precision", "blocked"],
+                    default="point")
+    ap.add_argument("--dims", default="1024,1024,256", help="grid of --workload blocked")
     ap.add_argument("--grids", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     a = ap.parse_args()
@@ -254,6 +333,8 @@ def main():
         return bench_batch(a)
     if a.workload == "precision":
         return bench_precision(a)
+    if a.workload == "blocked":
+        return bench_blocked(a)
 
     if a.profile:
         prec = {"fp32": Precision.FP32, "fp64": Precision.FP64, "fp16": Precision.FP16}[a.profile]

@@ -37,6 +37,13 @@ struct Tuning {
     int V = 1, TY = 1, kchunk = 1;
 };
 
+// Temporal blocking: L steps per pass (L < 2: off), V cells per lane,
+// lanes x TYg patch per workgroup (ghosts included), kchunk planes per
+// z chunk.
+struct BlockTuning {
+    int L = 0, V = 2, lanes = 64, TYg = 16, kchunk = 64;
+};
+
 inline size_t storage_size(int precision) {
     return precision == FDTD_FP64 ? 8 : (precision == FDTD_FP32 ? 4 : 2);
 }
@@ -70,6 +77,7 @@ struct fdtd_grid {
     int nsrc = 0;
     bool src_dirty = true;
     fdtd::rt::Tuning tune;
+    fdtd::rt::BlockTuning btune;
     bool fused_ok = false;
     int autoscale_interval = 0;  // FP16: rescale every this many sweeps (0 = off)
     int autoscale_log2 = 12;

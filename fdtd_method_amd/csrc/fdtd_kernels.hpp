@@ -448,6 +448,292 @@ fused_sweep(SweepArgs<S, C, JT> a) {
 }
 
 // ---------------------------------------------------------------------------
+// Temporally blocked sweep: L fused steps per pass over HBM.
+//
+// A workgroup covers a TXg x TYg patch of every plane (lanes x V columns,
+// TYg rows) whose outer L columns and rows on each side are ghost cells:
+// all L levels are computed on the whole patch and only the inner
+// TX x TY = (TXg-2L) x (TYg-2L) cells, valid after L levels, are stored
+// (overlapped tiling).  Along z the levels form a pipeline: at iteration kk
+// level l computes E_l at plane kk-l and B_l at plane kk-l-1, so every
+// level's k+1 neighbour is ready one iteration before it is needed.
+//
+//   E_l(p) = E_{l-1}(p) + curl B_{l-1}(p)        (E_{-1}, B_{-1}: the input)
+//   B_l(p) = B_{l-1}(p) + d + d, d = -c dt/2 curl E_l(p)   (d once for the
+//            last level of the last pass of a run)
+//
+// with the same per-cell expressions as fused_tile, so results are
+// bit-identical to L single fused sweeps.  Level l applies the currents of
+// its own step (source-value row l).  Tile origins are multiples of TX, so
+// a lane's V cells are contiguous in memory (V divides Ni and 2L).
+//
+// LDS: for every level, the B input planes (x/y neighbours of the E update)
+// and the E output planes (x/y neighbours of the B update): 2L x 3 planes
+// of TXg x TYg values of type C.
+
+struct BlockedGeom {
+    int Ni, Nj, Nk;
+    int TXg, TYg;     // patch covered by a workgroup, ghosts included
+    int TX, TY;       // stored cells per tile: TXg - 2L, TYg - 2L
+    int kchunk;
+    int ntx, nty;     // tiles along x and y
+    int ntiles;       // ntx * nty * nchunk
+    int per_xcd;
+};
+
+__host__ __device__ inline size_t blocked_lds_bytes(int L, int TXg, int TYg, int csize) {
+    return static_cast<size_t>(2 * L * 3) * TXg * TYg * csize;
+}
+
+// Largest workgroup of the blocked sweep: variants holding more state per
+// lane (V*L*csize >= 24 bytes) get 512 threads, hence more registers.
+__host__ __device__ constexpr int blocked_max_threads(int V, int L, int csize) {
+    return V * L * csize >= 24 ? 512 : 1024;
+}
+
+template <typename S, typename C>
+struct BlockedArgs {
+    Fields<S> in;
+    Fields<S> out;
+    const PointSource* src;
+    const double* vals;  // source values of the pass's first step; step l at vals + l*nsrc
+    int nsrc;
+    int pad;
+    Coefs<C> c;
+    BlockedGeom g;
+};
+
+__device__ __forceinline__ int wrap_mod(int i, int n) {
+    int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+template <typename S, typename C, int V, int L, bool FINAL>
+__device__ __forceinline__ void blocked_tile(const BlockedArgs<S, C>& a, int tx, int ty, int chunk,
+                                             C* __restrict__ lds) {
+    const BlockedGeom& G = a.g;
+    const int Ni = G.Ni, Nj = G.Nj, Nk = G.Nk, TXg = G.TXg, TYg = G.TYg;
+    const int lanes = TXg / V;
+    const int tid = threadIdx.x;
+    const int r = tid / lanes;                 // patch row
+    const int cl = (tid - r * lanes) * V;      // patch column of the lane's first cell
+    const bool active = r < TYg;
+    const int px0 = tx * G.TX;                 // patch column 0 <-> global column px0 mod Ni
+    const int gi0 = (px0 + cl) % Ni;           // global column of the first cell
+    const int qrow = ty * G.TY + r - L;        // output row position of this patch row
+    const int gj = wrap_mod(qrow, Nj);
+    const bool row_out = active && r >= L && r < L + G.TY && qrow < Nj;
+    bool col_out[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int c = cl + v;
+        col_out[v] = c >= L && c < L + G.TX && px0 + c - L < Ni;
+    }
+    const int k0 = chunk * G.kchunk;
+    const int k1 = min(k0 + G.kchunk, Nk);
+    const int plane = Ni * Nj;
+    const int row_off = gj * Ni + gi0;
+    const int P = TXg * TYg;
+    // neighbours inside the patch (clamped at its border: only ghost cells
+    // read clamped values, and no stored cell depends on them)
+    const int rm = r > 0 ? r - 1 : 0, rp = r + 1 < TYg ? r + 1 : TYg - 1;
+    const int cm = cl > 0 ? cl - 1 : 0, cp = cl + V < TXg ? cl + V : TXg - 1;
+    auto sB = [&](int l, int d) { return lds + (l * 6 + d) * P; };
+    auto sE = [&](int l, int d) { return lds + (l * 6 + 3 + d) * P; };
+    const Coefs<C> c = a.c;
+
+    C bP[L][3][V];  // B input of level l at its previous plane
+    C eP[L][3][V];  // E output of level l at the previous iteration
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < V; ++v) { bP[l][d][v] = C(0); eP[l][d][v] = C(0); }
+
+    const int kfirst = k0 - L + 1, klast = k1 + L - 1;
+    C en[3][V], bn[3][V];
+    auto load_plane = [&](int kk, C (&e)[3][V], C (&b)[3][V]) {
+        const int off = wrap_mod(kk, Nk) * plane + row_off;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            load_pack<S, C, V>(a.in.b[d] + off, b[d]);
+            load_pack<S, C, V>(a.in.e[d] + off, e[d]);
+        }
+    };
+    if (active) {
+        C e0[3][V];
+        load_plane(kfirst - 1, e0, bP[0]);
+        load_plane(kfirst, en, bn);
+        // the E planes are read (for ghost-plane results only) before their
+        // first write: start them at zero
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < V; ++v) sE(l, d)[r * TXg + cl + v] = C(0);
+    }
+
+    for (int kk = kfirst; kk <= klast; ++kk) {
+        if (active) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < V; ++v) sB(0, d)[r * TXg + cl + v] = bn[d][v];
+        }
+        __syncthreads();
+        C en2[3][V], bn2[3][V];
+        if (active && kk < klast) load_plane(kk + 1, en2, bn2);
+
+        C bin[3][V];   // B input of the current level at its E plane
+        C ein[3][V];   // E input of the current level
+        C enew[L][3][V];
+        C bnew[3][V];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < V; ++v) { bin[d][v] = bn[d][v]; ein[d][v] = en[d][v]; }
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const int pe = kk - l;  // this level's E plane; its B plane is pe - 1
+            C* Bz_ = sB(l, 2);
+            C* Bx_ = sB(l, 0);
+            C* By_ = sB(l, 1);
+            if (active) {
+                C t[3][V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const C bz_jm = Bz_[rm * TXg + cl + v];
+                    const C by_im = (v > 0) ? bin[1][v - 1] : By_[r * TXg + cm];
+                    t[0][v] = c.ex_bz * (bin[2][v] - bz_jm);
+                    t[1][v] = c.ey_bx * (bin[0][v] - bP[l][0][v]);
+                    t[2][v] = c.ez_by * (bin[1][v] - by_im);
+                }
+                if (a.nsrc > 0)
+                    add_point_sources<C, V>(t, c, a.src, a.vals + l * a.nsrc, a.nsrc, gi0, gj,
+                                            wrap_mod(pe, Nk));
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const C bx_jm = Bx_[rm * TXg + cl + v];
+                    const C bz_im = (v > 0) ? bin[2][v - 1] : Bz_[r * TXg + cm];
+                    enew[l][0][v] = ein[0][v] + (t[0][v] - c.ex_by * (bin[1][v] - bP[l][1][v]));
+                    enew[l][1][v] = ein[1][v] + (t[1][v] - c.ey_bz * (bin[2][v] - bz_im));
+                    enew[l][2][v] = ein[2][v] + (t[2][v] - c.ez_bx * (bin[0][v] - bx_jm));
+                }
+                // B_l at plane pe-1 from E_l(pe-1) (eP, LDS) and E_l(pe) (enew)
+                C* Ex_ = sE(l, 0);
+                C* Ey_ = sE(l, 1);
+                C* Ez_ = sE(l, 2);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const C ey_ip = (v < V - 1) ? eP[l][1][v + 1] : Ey_[r * TXg + cp];
+                    const C ez_ip = (v < V - 1) ? eP[l][2][v + 1] : Ez_[r * TXg + cp];
+                    const C ex_jp = Ex_[rp * TXg + cl + v];
+                    const C ez_jp = Ez_[rp * TXg + cl + v];
+                    const C dbx = c.bx_ey * (enew[l][1][v] - eP[l][1][v]) - c.bx_ez * (ez_jp - eP[l][2][v]);
+                    const C dby = c.by_ez * (ez_ip - eP[l][2][v]) - c.by_ex * (enew[l][0][v] - eP[l][0][v]);
+                    const C dbz = c.bz_ex * (ex_jp - eP[l][0][v]) - c.bz_ey * (ey_ip - eP[l][1][v]);
+                    bnew[0][v] = bP[l][0][v] + dbx;
+                    bnew[1][v] = bP[l][1][v] + dby;
+                    bnew[2][v] = bP[l][2][v] + dbz;
+                    if (!(FINAL && l == L - 1)) {
+                        bnew[0][v] = bnew[0][v] + dbx;
+                        bnew[1][v] = bnew[1][v] + dby;
+                        bnew[2][v] = bnew[2][v] + dbz;
+                    }
+                }
+            }
+            if (l < L - 1) {
+                // bnew = B_l(pe-1) is the next level's B input at its E plane
+                if (active) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int v = 0; v < V; ++v) sB(l + 1, d)[r * TXg + cl + v] = bnew[d][v];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        // level l+1 inputs: E_l(pe-1) = eP[l] (previous iteration), B_l(pe-1)
+                        ein[d][v] = eP[l][d][v];
+                        bP[l][d][v] = bin[d][v];   // level l's B input at pe: next iteration's previous plane
+                        bin[d][v] = bnew[d][v];
+                    }
+            } else {
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+#pragma unroll
+                    for (int v = 0; v < V; ++v) bP[l][d][v] = bin[d][v];
+            }
+        }
+        // stores of the last level: E_{L-1} at plane kk-L+1, B_{L-1} at kk-L
+        if (row_out) {
+            const int pe = kk - L + 1;
+            if (pe >= k0 && pe < k1) {
+                const int off = pe * plane + row_off;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    S* dst = a.out.e[d] + off;
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        if (col_out[v]) dst[v] = static_cast<S>(enew[L - 1][d][v]);
+                }
+            }
+            if (pe - 1 >= k0 && pe - 1 < k1) {
+                const int off = (pe - 1) * plane + row_off;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    S* dst = a.out.b[d] + off;
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        if (col_out[v]) dst[v] = static_cast<S>(bnew[d][v]);
+                }
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int l = 0; l < L; ++l)
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+#pragma unroll
+                    for (int v = 0; v < V; ++v) sE(l, d)[r * TXg + cl + v] = enew[l][d][v];
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int v = 0; v < V; ++v) eP[l][d][v] = enew[l][d][v];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                en[d][v] = en2[d][v];
+                bn[d][v] = bn2[d][v];
+            }
+    }
+}
+
+template <typename S, typename C, int V, int L, bool FINAL>
+__global__ void __launch_bounds__(blocked_max_threads(V, L, sizeof(C)))
+blocked_sweep(BlockedArgs<S, C> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int b = blockIdx.x;
+    const int t = (b & 7) * a.g.per_xcd + (b >> 3);
+    if (t >= a.g.ntiles) return;
+    const int txy = a.g.ntx * a.g.nty;
+    const int chunk = t / txy;
+    const int rem = t - chunk * txy;
+    const int ty = rem / a.g.ntx;
+    const int tx = rem - ty * a.g.ntx;
+    blocked_tile<S, C, V, L, FINAL>(a, tx, ty, chunk, reinterpret_cast<C*>(lds_raw));
+}
+
+// ---------------------------------------------------------------------------
 // Batched sweep: many independent grids, one persistent launch per step.
 //
 // The tiles of all grids form one work list (most expensive first); each

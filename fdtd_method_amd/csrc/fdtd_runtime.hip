@@ -209,6 +209,36 @@ int eval_sources(fdtd_grid* g, long long step, int nsteps) {
 
 // ---- plain kernels ----------------------------------------------------------
 
+fdtd::BlockedGeom make_blocked_geom(const fdtd_grid* g, const BlockTuning& t) {
+    fdtd::BlockedGeom G;
+    G.Ni = g->Ni;
+    G.Nj = g->Nj;
+    G.Nk = g->Nk;
+    G.TXg = t.lanes * t.V;
+    G.TYg = t.TYg;
+    G.TX = G.TXg - 2 * t.L;
+    G.TY = G.TYg - 2 * t.L;
+    G.kchunk = t.kchunk;
+    G.ntx = (g->Ni + G.TX - 1) / G.TX;
+    G.nty = (g->Nj + G.TY - 1) / G.TY;
+    const int nchunk = (g->Nk + t.kchunk - 1) / t.kchunk;
+    G.ntiles = G.ntx * G.nty * nchunk;
+    G.per_xcd = (G.ntiles + 7) / 8;
+    return G;
+}
+
+bool blocked_valid(const fdtd_grid* g, const BlockTuning& t) {
+    if (t.L < 1 || t.L > 4 || (t.V != 1 && t.V != 2)) return false;
+    if (g->Ni % t.V || (2 * t.L) % t.V) return false;
+    if (t.lanes <= 0 || t.TYg <= 0 || t.kchunk <= 0) return false;
+    if (t.lanes * t.V - 2 * t.L < 1 || t.TYg - 2 * t.L < 1) return false;
+    if (t.lanes * t.TYg > fdtd::blocked_max_threads(t.V, t.L, (int)compute_size(g->precision)))
+        return false;
+    if (fdtd::blocked_lds_bytes(t.L, t.lanes * t.V, t.TYg, (int)compute_size(g->precision)) > kLdsLimit)
+        return false;
+    return true;
+}
+
 template <typename S>
 int launch_plain_B(fdtd_grid* g) {
     using C = typename fdtd::ComputeOf<S>::type;
@@ -309,10 +339,102 @@ int launch_fused(fdtd_grid* g, const Tuning& t, bool steady, int slot, int in, i
 
 int rescale_to_target(fdtd_grid* g, bool current_only);
 
+// ---- temporally blocked sweeps ----------------------------------------------------
+
+template <typename S, int V, int L, bool FINAL>
+int launch_blocked_v(fdtd_grid* g, int slot, int in, int out) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    auto kern = fdtd::blocked_sweep<S, C, V, L, FINAL>;
+    const BlockTuning& t = g->btune;
+    fdtd::BlockedArgs<S, C> a;
+    a.in = fields_of<S>(g, in);
+    a.out = fields_of<S>(g, out);
+    a.src = g->d_src;
+    a.vals = g->d_vals ? g->d_vals + (size_t)slot * g->nsrc : nullptr;
+    a.nsrc = g->nsrc;
+    a.pad = 0;
+    a.c = make_coefs<C>(g);
+    a.g = make_blocked_geom(g, t);
+    const size_t lds = fdtd::blocked_lds_bytes(L, a.g.TXg, a.g.TYg, sizeof(C));
+    if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
+    hipLaunchKernelGGL(kern, dim3(8 * a.g.per_xcd), dim3(t.lanes * t.TYg), lds, g->stream, a);
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+template <typename S, int V, bool FINAL>
+int launch_blocked_l(fdtd_grid* g, int slot, int in, int out) {
+    switch (g->btune.L) {
+        case 1: return launch_blocked_v<S, V, 1, FINAL>(g, slot, in, out);
+        case 2: return launch_blocked_v<S, V, 2, FINAL>(g, slot, in, out);
+        case 3: return launch_blocked_v<S, V, 3, FINAL>(g, slot, in, out);
+        case 4: return launch_blocked_v<S, V, 4, FINAL>(g, slot, in, out);
+        default: return fail(FDTD_ERR_UNSUPPORTED, "unsupported blocking depth %d", g->btune.L);
+    }
+}
+
+template <typename S, bool FINAL>
+int launch_blocked_s(fdtd_grid* g, int slot, int in, int out) {
+    switch (g->btune.V) {
+        case 1: return launch_blocked_l<S, 1, FINAL>(g, slot, in, out);
+        case 2: return launch_blocked_l<S, 2, FINAL>(g, slot, in, out);
+        default: return fail(FDTD_ERR_UNSUPPORTED, "unsupported blocked V=%d", g->btune.V);
+    }
+}
+
+// One pass of btune.L fused steps; its source values are rows slot ..
+// slot+L-1 of the value table.
+int launch_blocked(fdtd_grid* g, bool final, int slot, int in, int out) {
+    switch (g->precision) {
+        case FDTD_FP64:
+            return final ? launch_blocked_s<double, true>(g, slot, in, out)
+                         : launch_blocked_s<double, false>(g, slot, in, out);
+        case FDTD_FP32:
+            return final ? launch_blocked_s<float, true>(g, slot, in, out)
+                         : launch_blocked_s<float, false>(g, slot, in, out);
+        default:
+            return final ? launch_blocked_s<_Float16, true>(g, slot, in, out)
+                         : launch_blocked_s<_Float16, false>(g, slot, in, out);
+    }
+}
+
+// n steps with temporal blocking: B/2, then n / L blocked passes, then the
+// remaining n % L steps as single fused sweeps.  Source tables are filled
+// in whole passes.
+int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
+    int rc = plain_B(g);
+    if (rc) return rc;
+    if (mid) HIP_TRY(hipEventRecord(mid, g->stream));
+    const int L = g->btune.L;
+    const int npass = nsteps / L, rem = nsteps % L;
+    const int chunk = (kValTable / L) * L;
+    int s = 0;
+    for (int p = 0; p < npass; ++p, s += L) {
+        if (g->autoscale_interval > 0 && s > 0 && (s / L) % std::max(1, g->autoscale_interval / L) == 0)
+            if ((rc = rescale_to_target(g, true))) return rc;
+        if (s % chunk == 0)
+            if ((rc = eval_sources(g, g->step + s, std::min(chunk, npass * L - s)))) return rc;
+        const int in = g->cur, out = 1 - g->cur;
+        rc = launch_blocked(g, rem == 0 && p + 1 == npass, s % chunk, in, out);
+        if (rc) return rc;
+        g->cur = out;
+    }
+    for (int q = 0; q < rem; ++q, ++s) {
+        if ((rc = eval_sources(g, g->step + s, 1))) return rc;
+        const int in = g->cur, out = 1 - g->cur;
+        rc = launch_fused(g, g->tune, q + 1 < rem, 0, in, out);
+        if (rc) return rc;
+        g->cur = out;
+    }
+    g->step += nsteps;
+    return FDTD_OK;
+}
+
 // n steps: B half step in place, then n fused sweeps between the buffers.
 // With FP16 autoscaling, the state between two sweeps, (E_n, B_{n+1/2}),
 // is rescaled by exact powers of two every autoscale_interval sweeps.
 int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
+    if (g->btune.L >= 1 && nsteps >= g->btune.L) return run_blocked(g, nsteps, mid);
     int rc = plain_B(g);
     if (rc) return rc;
     if (mid) HIP_TRY(hipEventRecord(mid, g->stream));
@@ -957,6 +1079,115 @@ int fdtd_set_tuning(fdtd_grid* g, int V, int TY, int kchunk) {
                     t.V, t.TY, t.kchunk, g->Ni);
     g->tune = t;
     g->fused_ok = true;
+    return FDTD_OK;
+}
+
+int fdtd_set_time_blocking(fdtd_grid* g, int L, int V, int lanes, int TYg, int kchunk) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (L <= 0) {
+        g->btune.L = 0;
+        return FDTD_OK;
+    }
+    BlockTuning t;
+    t.L = L;
+    t.V = V > 0 ? V : ((g->Ni % 2 == 0) ? 2 : 1);
+    t.lanes = lanes > 0 ? lanes : 64;
+    t.TYg = TYg > 0 ? TYg
+                    : fdtd::blocked_max_threads(t.V, L, (int)compute_size(g->precision)) / t.lanes;
+    t.kchunk = 1;
+    if (!blocked_valid(g, t))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "blocking L=%d V=%d lanes=%d TYg=%d invalid for this grid",
+                    t.L, t.V, t.lanes, t.TYg);
+    if (kchunk > 0) {
+        t.kchunk = kchunk;
+    } else {
+        // at least two tiles per CU when the grid allows, chunks of >= 16 planes
+        const int TX = t.lanes * t.V - 2 * L, TY = t.TYg - 2 * L;
+        const long long nxy = (long long)((g->Ni + TX - 1) / TX) * ((g->Nj + TY - 1) / TY);
+        int nchunk = (int)std::max<long long>(1, (512 + nxy - 1) / nxy);
+        nchunk = std::min(nchunk, std::max(1, g->Nk / 16));
+        t.kchunk = (g->Nk + nchunk - 1) / nchunk;
+    }
+    if (!blocked_valid(g, t))
+        return fail(FDTD_ERR_INVALID_ARGUMENT,
+                    "blocking L=%d V=%d lanes=%d TYg=%d kchunk=%d invalid for this grid", t.L, t.V,
+                    t.lanes, t.TYg, t.kchunk);
+    g->btune = t;
+    return FDTD_OK;
+}
+
+int fdtd_autotune_blocking(fdtd_grid* g, int reps, int out[5], float* best_ms) {
+    if (!g || reps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (int rc = flush_sources(g)) return rc;
+    // Candidate passes are timed on scratch copies of the state (restored
+    // afterwards); the full-row kernel's time per step is the reference.
+    std::vector<void*> saved;
+    const size_t bytes = g->N * g->esize;
+    for (int d = 0; d < 6; ++d) {
+        void* p = nullptr;
+        HIP_TRY(hipMalloc(&p, bytes));
+        HIP_TRY(hipMemcpyAsync(p, field_ptr(g, d), bytes, hipMemcpyDeviceToDevice, g->stream));
+        saved.push_back(p);
+    }
+    if (int rc = eval_sources(g, g->step, 4)) return rc;
+    auto time_passes = [&](bool blocked, float* ms_step) -> int {
+        const int per = blocked ? g->btune.L : 1;
+        int rc = blocked ? launch_blocked(g, false, 0, g->cur, 1 - g->cur)
+                         : launch_fused(g, g->tune, true, 0, g->cur, 1 - g->cur);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(g->ev[0], g->stream));
+        for (int r = 0; r < reps; ++r) {
+            const int in = (r & 1) ? 1 - g->cur : g->cur;
+            rc = blocked ? launch_blocked(g, false, 0, in, 1 - in)
+                         : launch_fused(g, g->tune, true, 0, in, 1 - in);
+            if (rc) return rc;
+        }
+        HIP_TRY(hipEventRecord(g->ev[1], g->stream));
+        HIP_TRY(hipEventSynchronize(g->ev[1]));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, g->ev[0], g->ev[1]));
+        *ms_step = ms / (reps * per);
+        return FDTD_OK;
+    };
+    const BlockTuning orig = g->btune;
+    BlockTuning best;  // L = 0: full-row kernel
+    float best_t = 1e30f;
+    if (g->fused_ok) {
+        float t;
+        if (int rc = time_passes(false, &t)) return rc;
+        best_t = t;
+    }
+    const int shapes[][3] = {{1, 32, 32}, {1, 64, 16}, {2, 64, 16}, {2, 32, 16}, {1, 64, 8},
+                             {2, 64, 8}, {1, 32, 16}, {2, 32, 32}};
+    for (int L = 1; L <= 4; ++L)
+        for (const auto& sh : shapes) {
+            if (fdtd_set_time_blocking(g, L, sh[0], sh[1], sh[2], 0) != FDTD_OK) continue;
+            float t;
+            if (int rc = time_passes(true, &t)) return rc;
+            if (t < best_t) {
+                best_t = t;
+                best = g->btune;
+            }
+        }
+    g_last_error.clear();
+    g->btune = best_t < 1e30f ? best : orig;
+    for (int d = 0; d < 6; ++d)
+        HIP_TRY(hipMemcpyAsync(field_ptr(g, d), saved[d], bytes, hipMemcpyDeviceToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    for (void* p : saved) HIP_TRY(hipFree(p));
+    if (best_t >= 1e30f) return fail(FDTD_ERR_UNSUPPORTED, "no launch shape fits this grid");
+    if (out) fdtd_get_time_blocking(g, out);
+    if (best_ms) *best_ms = best_t;
+    return FDTD_OK;
+}
+
+int fdtd_get_time_blocking(const fdtd_grid* g, int out[5]) {
+    if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    out[0] = g->btune.L;
+    out[1] = g->btune.V;
+    out[2] = g->btune.lanes;
+    out[3] = g->btune.TYg;
+    out[4] = g->btune.kchunk;
     return FDTD_OK;
 }
 

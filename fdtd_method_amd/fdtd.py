@@ -275,6 +275,25 @@ class FDTD:
             f._dirty = False
         _lib.check(self._L.fdtd_zero_fields(self._h))
 
+    def set_time_blocking(self, L: int, V: int = 0, lanes: int = 0, TYg: int = 0,
+                          kchunk: int = 0) -> None:
+        """L steps per pass over HBM (L = 0: full-row sweep, 1: tiled single
+        step); bit-identical results."""
+        _lib.check(self._L.fdtd_set_time_blocking(self._h, int(L), int(V), int(lanes), int(TYg),
+                                                  int(kchunk)))
+
+    def autotune_blocking(self, reps: int = 3):
+        """Fastest of the full-row sweep and tiled/blocked passes (ms per step)."""
+        out = (ctypes.c_int * 5)()
+        ms = ctypes.c_float()
+        _lib.check(self._L.fdtd_autotune_blocking(self._h, int(reps), out, ctypes.byref(ms)))
+        return dict(L=out[0], V=out[1], lanes=out[2], TYg=out[3], kchunk=out[4]), ms.value
+
+    def get_time_blocking(self):
+        out = (ctypes.c_int * 5)()
+        _lib.check(self._L.fdtd_get_time_blocking(self._h, out))
+        return dict(L=out[0], V=out[1], lanes=out[2], TYg=out[3], kchunk=out[4])
+
     def set_tuning(self, V=0, TY=0, kchunk=0) -> None:
         _lib.check(self._L.fdtd_set_tuning(self._h, int(V), int(TY), int(kchunk)))
 

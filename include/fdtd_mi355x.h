@@ -171,6 +171,19 @@ int fdtd_set_tuning(fdtd_grid* grid, int V, int TY, int kchunk);
 int fdtd_get_tuning(const fdtd_grid* grid, int out[3]);
 int fdtd_autotune(fdtd_grid* grid, int reps, int out[3], float* best_ms);
 
+/* Temporal blocking: fdtd_update_fields advances L steps per pass over HBM
+ * (L = 2..4) with overlapped (ghost-zone) x/y tiles of lanes*V columns by
+ * TYg rows (ghosts included) marching kchunk planes in z; L = 1 is the same
+ * tiled kernel one step per pass (for rows too wide for the full-row
+ * sweep); L = 0 returns to the full-row fused sweep.  0 picks a default for
+ * V, lanes, TYg, kchunk.  Results are bit-identical to unblocked steps.
+ * get: {L, V, lanes, TYg, kchunk}.  fdtd_autotune_blocking times the
+ * full-row sweep and tiled/blocked candidates on scratch copies of the
+ * state and keeps the fastest per step (best_ms: ms per step). */
+int fdtd_set_time_blocking(fdtd_grid* grid, int L, int V, int lanes, int TYg, int kchunk);
+int fdtd_get_time_blocking(const fdtd_grid* grid, int out[5]);
+int fdtd_autotune_blocking(fdtd_grid* grid, int reps, int out[5], float* best_ms);
+
 /* Timed run on the grid's stream with HIP events: ms_total covers the whole
  * call (including the B half step of the fused path), ms_main only the
  * nsteps sweep launches (fused) or step triples (3-sweep). */

@@ -408,3 +408,62 @@ def test_device_plane_wave_matches_numpy():
     e0 = sum(g.sum_squares())
     g.update_fields(50)
     assert abs(sum(g.sum_squares()) / e0 - 1.0) < 1e-2
+
+
+@pytest.mark.parametrize("prec", [Precision.FP64, Precision.FP32])
+@pytest.mark.parametrize("L,V,lanes,TYg,kchunk", [(1, 2, 32, 8, 3), (1, 1, 16, 6, 0),
+                                                  (2, 2, 64, 0, 0), (2, 1, 32, 12, 5), (3, 1, 64, 8, 0),
+                                                  (3, 2, 16, 10, 7), (4, 2, 32, 12, 4), (4, 1, 20, 12, 3)])
+def test_time_blocking_bit_exact(prec, L, V, lanes, TYg, kchunk):
+    """L steps per pass (ghost-zone tiles) give exactly the unblocked fields,
+    with waveform sources, remainders and wrap-around at the tile edges."""
+    shape = (40, 36, 22)
+    n = int(np.prod(shape))
+    dt = courant_dt(1, 1, 1, factor=0.5)
+    st = random_state(n, 30)
+    src = [dict(i=3, j=35, k=0, comp=2, kind=1, amp=1.0, period=16 * dt, t0=5 * dt, sigma=6 * dt),
+           dict(i=39, j=0, k=21, comp=0, kind=0, amp=0.3)]
+    a = gpu_grid(shape, (1, 1, 1), dt, st, prec)
+    b = gpu_grid(shape, (1, 1, 1), dt, st, prec)
+    for g in (a, b):
+        g.set_point_sources(src)
+    b.set_time_blocking(L, V, lanes, TYg, kchunk)
+    print("blocking", b.get_time_blocking(), flush=True)
+    for nsteps in (2 * L + 1, L, 3):
+        a.update_fields(nsteps)
+        b.update_fields(nsteps)
+        assert a.checksum() == b.checksum(), nsteps
+    if prec == Precision.FP64:
+        r = oracle_grid(shape, (1, 1, 1), dt, st)
+        r.run_point_sources(src, 2 * L + 1 + L + 3)
+        rel = rel_l2(b.fields(), r.E + r.B)
+        assert rel <= 1e-12
+
+
+def test_time_blocking_tiny_grids():
+    """Grids smaller than one tile: ghost cells wrap several times."""
+    for shape in [(4, 3, 5), (2, 2, 2), (6, 1, 3)]:
+        n = int(np.prod(shape))
+        dt = courant_dt(1, 1, 1)
+        st = random_state(n, 31)
+        a = gpu_grid(shape, (1, 1, 1), dt, st)
+        b = gpu_grid(shape, (1, 1, 1), dt, st)
+        b.set_time_blocking(2, 1, 8, 8, 1)
+        a.update_fields(6)
+        b.update_fields(6)
+        assert a.checksum() == b.checksum(), shape
+
+
+def test_autotune_blocking_keeps_state():
+    shape = (96, 40, 24)
+    n = int(np.prod(shape))
+    st = random_state(n, 33)
+    g = gpu_grid(shape, (1, 1, 1), courant_dt(1, 1, 1), st, Precision.FP32)
+    ref = gpu_grid(shape, (1, 1, 1), courant_dt(1, 1, 1), st, Precision.FP32)
+    before = g.checksum()
+    cfg, ms = g.autotune_blocking(reps=2)
+    assert ms > 0 and 0 <= cfg["L"] <= 4
+    assert g.checksum() == before
+    g.update_fields(9)
+    ref.update_fields(9)
+    assert g.checksum() == ref.checksum()
