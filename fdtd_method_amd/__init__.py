@@ -6,8 +6,8 @@ host-side mirror of the reference interface.
 """
 from .structures import (C, PI, Axis, Component, CurrentParameters, Parameters, Precision,
                          SelectedFields)
-from .fdtd import FDTD, Field, calibration_copy
+from .fdtd import FDTD, FDTD_PML, Field, calibration_copy, pml_profile
 from .batch import Batch, plane_wave_batch
 
 __all__ = ["C", "PI", "Axis", "Component", "CurrentParameters", "Parameters", "Precision",
-           "SelectedFields", "FDTD", "Field", "calibration_copy", "Batch", "plane_wave_batch"]
+           "SelectedFields", "FDTD", "FDTD_PML", "Field", "calibration_copy", "pml_profile", "Batch", "plane_wave_batch"]

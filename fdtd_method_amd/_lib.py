@@ -22,7 +22,8 @@ PATH_FUSED, PATH_3SWEEP = 0, 1
 
 # Every symbol include/fdtd_mi355x.h declares (checked by the CPU test suite).
 EXPORTED = [
-    "fdtd_last_error", "fdtd_create", "fdtd_destroy", "fdtd_upload", "fdtd_download",
+    "fdtd_last_error", "fdtd_create", "fdtd_destroy", "fdtd_create_pml", "fdtd_get_pml",
+    "fdtd_pml_download_split", "fdtd_pml_profile", "fdtd_upload", "fdtd_download",
     "fdtd_zero_currents", "fdtd_set_current_values", "fdtd_set_point_sources",
     "fdtd_update_fields", "fdtd_update_fields_3sweep", "fdtd_get_step", "fdtd_set_step",
     "fdtd_synchronize", "fdtd_get_info", "fdtd_sum_squares", "fdtd_max_abs", "fdtd_compare",
@@ -79,6 +80,12 @@ def load():
         "fdtd_last_error": ([], ctypes.c_char_p),
         "fdtd_create": ([ctypes.POINTER(Parameters), ctypes.c_double, _i, ctypes.POINTER(_vp)], _i),
         "fdtd_destroy": ([_vp], _i),
+        "fdtd_create_pml": ([ctypes.POINTER(Parameters), ctypes.c_double, ctypes.c_double, _i,
+                             ctypes.POINTER(_vp)], _i),
+        "fdtd_get_pml": ([_vp, ctypes.POINTER(_i), _dp], _i),
+        "fdtd_pml_download_split": ([_vp, _i, _dp, ctypes.c_size_t], _i),
+        "fdtd_pml_profile": ([_i, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                              _dp], _i),
         "fdtd_upload": ([_vp, _i, _dp, ctypes.c_size_t], _i),
         "fdtd_download": ([_vp, _i, _dp, ctypes.c_size_t], _i),
         "fdtd_zero_currents": ([_vp], _i),

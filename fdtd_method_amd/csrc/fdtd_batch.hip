@@ -369,6 +369,8 @@ int fdtd_batch_create(int n, fdtd_grid* const* grids, fdtd_batch** out) {
         if (!grids[q]) return fail(FDTD_ERR_INVALID_ARGUMENT, "null grid %d", q);
         if (grids[q]->precision != grids[0]->precision)
             return fail(FDTD_ERR_INVALID_ARGUMENT, "grids of a batch must share one precision");
+        if (grids[q]->pml)
+            return fail(FDTD_ERR_UNSUPPORTED, "grid %d has a PML layer; batches take periodic grids", q);
         for (int r = 0; r < q; ++r)
             if (grids[r] == grids[q]) return fail(FDTD_ERR_INVALID_ARGUMENT, "grid %d repeated", q);
     }

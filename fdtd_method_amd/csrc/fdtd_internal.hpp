@@ -15,12 +15,14 @@
 namespace fdtd::rt {
 
 
-constexpr double kC = 3e10;              // include/Constants.h:9
-constexpr double kPiRef = 3.14159265358; // include/Constants.h:14
+constexpr double kC = 3e10;              // include/Constants.h:6
+constexpr double kPiRef = 3.14159265358; // include/Constants.h:11
 constexpr int kMaxSparseCurrents = 64;   // beyond this J is kept dense
 constexpr size_t kLdsLimit = 160 * 1024; // gfx950 LDS per workgroup
 constexpr int kReduceBlocks = 1024;
 constexpr int kValTable = 256;            // steps of source values per table fill
+constexpr double kPmlR = 1e-12;           // include/Constants.h:7  (layer reflection target)
+constexpr double kPmlOrder = 4.0;         // include/Constants.h:10 (conductivity grading power)
 
 
 // Records a message for fdtd_last_error() and returns code.
@@ -84,6 +86,13 @@ struct fdtd_grid {
     double* d_part = nullptr;
     double* d_out = nullptr;
     hipEvent_t ev[3] = {};
+    // PML layer (include/FDTD/FDTD_PML.h): set when some layer width is > 0.
+    bool pml = false;
+    double pml_percent = 0;
+    int pml_p[3] = {0, 0, 0};
+    void* ES[2][6] = {};        // E splits Exy Exz Eyx Eyz Ezx Ezy, ping-pong with E
+    void* BS[6] = {};           // B splits Bxy Bxz Byx Byz Bzx Bzy, in place
+    void* pml_tab = nullptr;    // per-axis decay/coefficient tables (compute type)
 };
 
 namespace fdtd::rt {
@@ -117,6 +126,26 @@ fdtd::Coefs<C> make_coefs(const fdtd_grid* g) {
     return c;
 }
 
+// Layer widths, split buffers and coefficient tables of a PML grid; `in`
+// and `out` select the E-split buffers a sweep reads and writes.
+template <typename S, typename C>
+fdtd::Pml<S, C> make_pml(const fdtd_grid* g, int in, int out) {
+    fdtd::Pml<S, C> P{};
+    P.pi = g->pml_p[0];
+    P.pj = g->pml_p[1];
+    P.pk = g->pml_p[2];
+    for (int q = 0; q < 6; ++q) {
+        P.es_in[q] = static_cast<const S*>(g->ES[in][q]);
+        P.es_out[q] = static_cast<S*>(g->ES[out][q]);
+        P.bs[q] = static_cast<S*>(g->BS[q]);
+    }
+    const C* t = static_cast<const C*>(g->pml_tab);
+    P.tx = t;
+    P.ty = t ? t + 5 * g->Ni : nullptr;
+    P.tz = t ? t + 5 * (g->Ni + g->Nj) : nullptr;
+    return P;
+}
+
 template <typename S>
 fdtd::Fields<S> fields_of(fdtd_grid* g, int buf) {
     fdtd::Fields<S> f;
@@ -138,7 +167,9 @@ int default_kchunk(int Nj, int Nk, int TY);
 std::vector<fdtd::PointSource> host_sources(const fdtd_grid* g);
 // Uploads the grid's source list to its device copy when it changed.
 int flush_sources(fdtd_grid* g);
-int plain_B(fdtd_grid* g);
+// B half step on the current buffers; on a PML grid `layer` adds the
+// layer's full-step split update (the first B update of a step).
+int plain_B(fdtd_grid* g, bool layer = true);
 int zero_fill(void* p, size_t bytes, hipStream_t s);
 
 }  // namespace fdtd::rt

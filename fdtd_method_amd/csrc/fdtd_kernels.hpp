@@ -58,7 +58,7 @@ struct PointSource {
     double amp, period, t0, sigma, phase;
 };
 
-constexpr double kPi = 3.14159265358;  // include/Constants.h:14 (truncated)
+constexpr double kPi = 3.14159265358;  // include/Constants.h:11 (truncated)
 
 __device__ __forceinline__ double source_value(const PointSource& s, double t) {
     if (s.kind == 0) return s.amp;
@@ -164,16 +164,15 @@ __device__ __forceinline__ void add_point_sources(C (&t)[3][V], const Coefs<C>& 
     }
 }
 
-// Dense J (type JT, may be null) plus point sources with values `vals`.
+// One cell of the E update: dense J (type JT, may be null) plus point
+// sources with values `vals`.
 template <typename S, typename C, typename JT>
-__global__ void __launch_bounds__(256)
-plain_update_E(Fields<S> f, const JT* __restrict__ jx, const JT* __restrict__ jy,
-               const JT* __restrict__ jz, const PointSource* __restrict__ src,
-               const double* __restrict__ vals, int nsrc, Coefs<C> c, int Ni, int Nj, int Nk) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= Ni || j >= Nj) return;
+__device__ __forceinline__ void plain_E_cell(const Fields<S>& f, const JT* __restrict__ jx,
+                                             const JT* __restrict__ jy, const JT* __restrict__ jz,
+                                             const PointSource* __restrict__ src,
+                                             const double* __restrict__ vals, int nsrc,
+                                             const Coefs<C>& c, int Ni, int Nj, int Nk, int i,
+                                             int j, int k) {
     const int plane = Ni * Nj;
     const int idx = i + j * Ni + k * plane;
     const int i_pred = wrap(i - 1, Ni) + j * Ni + k * plane;
@@ -196,6 +195,149 @@ plain_update_E(Fields<S> f, const JT* __restrict__ jx, const JT* __restrict__ jy
     f.e[0][idx] = static_cast<S>(ex);
     f.e[1][idx] = static_cast<S>(ey);
     f.e[2][idx] = static_cast<S>(ez);
+}
+
+template <typename S, typename C, typename JT>
+__global__ void __launch_bounds__(256)
+plain_update_E(Fields<S> f, const JT* __restrict__ jx, const JT* __restrict__ jy,
+               const JT* __restrict__ jz, const PointSource* __restrict__ src,
+               const double* __restrict__ vals, int nsrc, Coefs<C> c, int Ni, int Nj, int Nk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= Ni || j >= Nj) return;
+    plain_E_cell<S, C, JT>(f, jx, jy, jz, src, vals, nsrc, c, Ni, Nj, Nk, i, j, k);
+}
+
+// ---------------------------------------------------------------------------
+// PML absorbing layer (src/FDTD/FDTD_PML.cpp; Kokkos:
+// include/FDTD_kokkos/kokkos_functors.h:189-415).
+//
+// Cells outside the main box [p_i, Ni-p_i) x [p_j, Nj-p_j) x [p_k, Nk-p_k)
+// carry split fields (FDTD_PML.h:12-13).  Each split decays by
+// a = exp(-sigma*dt*C) and takes one curl term with the coefficient
+// (1 - a)/(sigma*d), or C*dt/d where sigma = 0 (FDTD_PML.cpp:63-65,98-111);
+// a total is the sum of its two splits.  sigma_x depends on i only, sigma_y
+// on j and sigma_z on k (the bands span the grid, FDTD_PML.cpp:326-338), so
+// the coefficients are per-axis tables.  The layer takes one full B step
+// where the main box takes the two half steps of a step, and its E update
+// has no J term (FDTD_PML.cpp:343-365).
+
+enum { kExy = 0, kExz, kEyx, kEyz, kEzx, kEzy };  // E split order, FDTD_PML.h:12
+enum { kBxy = 0, kBxz, kByx, kByz, kBzx, kBzy };  // B split order, FDTD_PML.h:13
+
+template <typename S, typename C> struct Pml {
+    int pi, pj, pk, pad;
+    const S* es_in[6];  // E splits read by this sweep
+    S* es_out[6];       // E splits written (ping-pong partner of es_in; equal in place)
+    S* bs[6];           // B splits, updated in place
+    // Per-axis tables of 5 rows of N values: the decay a, then the
+    // coefficients of the four splits that differentiate along that axis
+    // (storage-scale ratios folded in):
+    //   tx: a, Eyx<-Bz, Ezx<-By, Byx<-Ez, Bzx<-Ey
+    //   ty: a, Exy<-Bz, Ezy<-Bx, Bxy<-Ez, Bzy<-Ex
+    //   tz: a, Exz<-By, Eyz<-Bx, Bxz<-Ey, Byz<-Ex
+    const C* tx;
+    const C* ty;
+    const C* tz;
+};
+
+__device__ __forceinline__ bool in_main(int i, int n, int p) { return i >= p && i < n - p; }
+
+// Rounds a split to its storage type, as stored: a total is the sum of the
+// stored splits (FDTD_PML.cpp:128-130,197-199).
+template <typename S, typename C> __device__ __forceinline__ C stored(C x) {
+    return static_cast<C>(static_cast<S>(x));
+}
+
+// Layer cell of the E update, FDTD_PML.cpp:113-130.
+template <typename S, typename C>
+__device__ __forceinline__ void pml_E_cell(const Fields<S>& f, const Pml<S, C>& P, int Ni, int Nj,
+                                           int Nk, int i, int j, int k) {
+    const int plane = Ni * Nj;
+    const int idx = i + j * Ni + k * plane;
+    const int i_pred = wrap(i - 1, Ni) + j * Ni + k * plane;
+    const int j_pred = i + wrap(j - 1, Nj) * Ni + k * plane;
+    const int k_pred = i + j * Ni + wrap(k - 1, Nk) * plane;
+    const C bx = f.b[0][idx], by = f.b[1][idx], bz = f.b[2][idx];
+    const C ax = P.tx[i], ay = P.ty[j], az = P.tz[k];
+    const C eyx = stored<S, C>(C(P.es_in[kEyx][idx]) * ax - P.tx[Ni + i] * (bz - C(f.b[2][i_pred])));
+    const C ezx = stored<S, C>(C(P.es_in[kEzx][idx]) * ax + P.tx[2 * Ni + i] * (by - C(f.b[1][i_pred])));
+    const C exy = stored<S, C>(C(P.es_in[kExy][idx]) * ay + P.ty[Nj + j] * (bz - C(f.b[2][j_pred])));
+    const C ezy = stored<S, C>(C(P.es_in[kEzy][idx]) * ay - P.ty[2 * Nj + j] * (bx - C(f.b[0][j_pred])));
+    const C exz = stored<S, C>(C(P.es_in[kExz][idx]) * az - P.tz[Nk + k] * (by - C(f.b[1][k_pred])));
+    const C eyz = stored<S, C>(C(P.es_in[kEyz][idx]) * az + P.tz[2 * Nk + k] * (bx - C(f.b[0][k_pred])));
+    P.es_out[kEyx][idx] = static_cast<S>(eyx);
+    P.es_out[kEzx][idx] = static_cast<S>(ezx);
+    P.es_out[kExy][idx] = static_cast<S>(exy);
+    P.es_out[kEzy][idx] = static_cast<S>(ezy);
+    P.es_out[kExz][idx] = static_cast<S>(exz);
+    P.es_out[kEyz][idx] = static_cast<S>(eyz);
+    f.e[0][idx] = static_cast<S>(exz + exy);
+    f.e[1][idx] = static_cast<S>(eyx + eyz);
+    f.e[2][idx] = static_cast<S>(ezy + ezx);
+}
+
+// Layer cell of the (full-step) B update, FDTD_PML.cpp:182-199.
+template <typename S, typename C>
+__device__ __forceinline__ void pml_B_cell(const Fields<S>& f, const Pml<S, C>& P, int Ni, int Nj,
+                                           int Nk, int i, int j, int k) {
+    const int plane = Ni * Nj;
+    const int idx = i + j * Ni + k * plane;
+    const int i_next = wrap(i + 1, Ni) + j * Ni + k * plane;
+    const int j_next = i + wrap(j + 1, Nj) * Ni + k * plane;
+    const int k_next = i + j * Ni + wrap(k + 1, Nk) * plane;
+    const C ex = f.e[0][idx], ey = f.e[1][idx], ez = f.e[2][idx];
+    const C ax = P.tx[i], ay = P.ty[j], az = P.tz[k];
+    const C byx = stored<S, C>(C(P.bs[kByx][idx]) * ax + P.tx[3 * Ni + i] * (C(f.e[2][i_next]) - ez));
+    const C bzx = stored<S, C>(C(P.bs[kBzx][idx]) * ax - P.tx[4 * Ni + i] * (C(f.e[1][i_next]) - ey));
+    const C bxy = stored<S, C>(C(P.bs[kBxy][idx]) * ay - P.ty[3 * Nj + j] * (C(f.e[2][j_next]) - ez));
+    const C bzy = stored<S, C>(C(P.bs[kBzy][idx]) * ay + P.ty[4 * Nj + j] * (C(f.e[0][j_next]) - ex));
+    const C bxz = stored<S, C>(C(P.bs[kBxz][idx]) * az + P.tz[3 * Nk + k] * (C(f.e[1][k_next]) - ey));
+    const C byz = stored<S, C>(C(P.bs[kByz][idx]) * az - P.tz[4 * Nk + k] * (C(f.e[0][k_next]) - ex));
+    P.bs[kByx][idx] = static_cast<S>(byx);
+    P.bs[kBzx][idx] = static_cast<S>(bzx);
+    P.bs[kBxy][idx] = static_cast<S>(bxy);
+    P.bs[kBzy][idx] = static_cast<S>(bzy);
+    P.bs[kBxz][idx] = static_cast<S>(bxz);
+    P.bs[kByz][idx] = static_cast<S>(byz);
+    f.b[0][idx] = static_cast<S>(bxy + bxz);
+    f.b[1][idx] = static_cast<S>(byz + byx);
+    f.b[2][idx] = static_cast<S>(bzx + bzy);
+}
+
+// B half step on the main box and, with LAYER, the full-step split update
+// on the layer (one thread per cell, in place): LAYER is the first B
+// update of a PML step (FDTD_PML.cpp:344-352), !LAYER the last (:364).
+template <typename S, typename C, bool LAYER>
+__global__ void __launch_bounds__(256)
+pml_plain_update_B(Fields<S> f, Pml<S, C> P, Coefs<C> c, int Ni, int Nj, int Nk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= Ni || j >= Nj) return;
+    if (in_main(i, Ni, P.pi) && in_main(j, Nj, P.pj) && in_main(k, Nk, P.pk))
+        plain_B_cell<S, C>(f, c, Ni, Nj, Nk, i, j, k);
+    else if (LAYER)
+        pml_B_cell<S, C>(f, P, Ni, Nj, Nk, i, j, k);
+}
+
+// E on the main box (with J) and the split update on the layer,
+// FDTD_PML.cpp:354-362.
+template <typename S, typename C, typename JT>
+__global__ void __launch_bounds__(256)
+pml_plain_update_E(Fields<S> f, Pml<S, C> P, const JT* __restrict__ jx,
+                   const JT* __restrict__ jy, const JT* __restrict__ jz,
+                   const PointSource* __restrict__ src, const double* __restrict__ vals, int nsrc,
+                   Coefs<C> c, int Ni, int Nj, int Nk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= Ni || j >= Nj) return;
+    if (in_main(i, Ni, P.pi) && in_main(j, Nj, P.pj) && in_main(k, Nk, P.pk))
+        plain_E_cell<S, C, JT>(f, jx, jy, jz, src, vals, nsrc, c, Ni, Nj, Nk, i, j, k);
+    else
+        pml_E_cell<S, C>(f, P, Ni, Nj, Nk, i, j, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -254,9 +396,10 @@ struct SweepArgs {
     int pad;
     Coefs<C> c;
     SweepGeom g;
+    Pml<S, C> pml;  // used by PML sweeps only
 };
 
-template <typename S, typename C, typename JT, int V, bool STEADY>
+template <typename S, typename C, typename JT, int V, bool STEADY, bool PML = false>
 __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int tile_y, int chunk,
                                            C* __restrict__ lds) {
     const int Ni = a.g.Ni, Nj = a.g.Nj, Nk = a.g.Nk, TY = a.g.TY;
@@ -378,6 +521,52 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                     e1[1][v] = C(0);
                 }
             }
+            if constexpr (PML) {
+                // Layer cells replace e1 by the sums of their updated E
+                // splits; lanes whose V cells are all in the main box skip.
+                const Pml<S, C>& P = a.pml;
+                const bool rowm = in_main(kw, Nk, P.pk) && in_main(j, Nj, P.pj);
+                if (!(rowm && ib >= P.pi && ib + V <= Ni - P.pi)) {
+                    C sxy[V], sxz[V], syx[V], syz[V], szx[V], szy[V];
+                    load_pack<S, C, V>(P.es_in[kExy] + off, sxy);
+                    load_pack<S, C, V>(P.es_in[kExz] + off, sxz);
+                    load_pack<S, C, V>(P.es_in[kEzx] + off, szx);
+                    load_pack<S, C, V>(P.es_in[kEzy] + off, szy);
+                    if (own) {
+                        load_pack<S, C, V>(P.es_in[kEyx] + off, syx);
+                        load_pack<S, C, V>(P.es_in[kEyz] + off, syz);
+                    }
+                    const C ay = P.ty[j], cy_xy = P.ty[Nj + j], cy_zy = P.ty[2 * Nj + j];
+                    const C az = P.tz[kw], cz_xz = P.tz[Nk + kw], cz_yz = P.tz[2 * Nk + kw];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        const int i = ib + v;
+                        if (rowm && in_main(i, Ni, P.pi)) continue;
+                        const C ax = P.tx[i], cx_yx = P.tx[Ni + i], cx_zx = P.tx[2 * Ni + i];
+                        const C by_im = (v > 0) ? bn[1][v - 1] : sBy[r * Ni + im0];
+                        sxy[v] = stored<S, C>(sxy[v] * ay + cy_xy * (bn[2][v] - sBz[r * Ni + ib + v]));
+                        szy[v] = stored<S, C>(szy[v] * ay - cy_zy * (bn[0][v] - sBx[r * Ni + ib + v]));
+                        sxz[v] = stored<S, C>(sxz[v] * az - cz_xz * (bn[1][v] - bP[1][v]));
+                        szx[v] = stored<S, C>(szx[v] * ax + cx_zx * (bn[1][v] - by_im));
+                        e1[0][v] = sxz[v] + sxy[v];
+                        e1[2][v] = szy[v] + szx[v];
+                        if (own) {
+                            const C bz_im = (v > 0) ? bn[2][v - 1] : sBz[(r + 1) * Ni + im0];
+                            syx[v] = stored<S, C>(syx[v] * ax - cx_yx * (bn[2][v] - bz_im));
+                            syz[v] = stored<S, C>(syz[v] * az + cz_yz * (bn[0][v] - bP[0][v]));
+                            e1[1][v] = syx[v] + syz[v];
+                        }
+                    }
+                    if (own && kk < k1) {
+                        store_pack<S, C, V>(P.es_out[kExy] + off, sxy);
+                        store_pack<S, C, V>(P.es_out[kExz] + off, sxz);
+                        store_pack<S, C, V>(P.es_out[kEyx] + off, syx);
+                        store_pack<S, C, V>(P.es_out[kEyz] + off, syz);
+                        store_pack<S, C, V>(P.es_out[kEzx] + off, szx);
+                        store_pack<S, C, V>(P.es_out[kEzy] + off, szy);
+                    }
+                }
+            }
             if (own && kk < k1) {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) store_pack<S, C, V>(a.out.e[d] + off, e1[d]);
@@ -401,6 +590,60 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                         nb[0][v] = nb[0][v] + dbx;
                         nb[1][v] = nb[1][v] + dby;
                         nb[2][v] = nb[2][v] + dbz;
+                    }
+                }
+                if constexpr (PML) {
+                    // Layer cells of plane kk-1: the next step's full-step B
+                    // split update (steady sweeps), or no B update at all
+                    // (the final sweep of a run).
+                    const Pml<S, C>& P = a.pml;
+                    const int kb = kk - 1;
+                    const bool rowm = in_main(kb, Nk, P.pk) && in_main(j, Nj, P.pj);
+                    if (!(rowm && ib >= P.pi && ib + V <= Ni - P.pi)) {
+                        if (STEADY) {
+                            C sxy[V], sxz[V], syx[V], syz[V], szx[V], szy[V];
+                            load_pack<S, C, V>(P.bs[kBxy] + offb, sxy);
+                            load_pack<S, C, V>(P.bs[kBxz] + offb, sxz);
+                            load_pack<S, C, V>(P.bs[kByx] + offb, syx);
+                            load_pack<S, C, V>(P.bs[kByz] + offb, syz);
+                            load_pack<S, C, V>(P.bs[kBzx] + offb, szx);
+                            load_pack<S, C, V>(P.bs[kBzy] + offb, szy);
+                            const C ay = P.ty[j], cy_xy = P.ty[3 * Nj + j], cy_zy = P.ty[4 * Nj + j];
+                            const C az = P.tz[kb], cz_xz = P.tz[3 * Nk + kb], cz_yz = P.tz[4 * Nk + kb];
+#pragma unroll
+                            for (int v = 0; v < V; ++v) {
+                                const int i = ib + v;
+                                if (rowm && in_main(i, Ni, P.pi)) continue;
+                                const C ax = P.tx[i], cx_yx = P.tx[3 * Ni + i], cx_zx = P.tx[4 * Ni + i];
+                                const C ey_ip = (v < V - 1) ? eP[1][v + 1] : sEy[r * Ni + ipl];
+                                const C ez_ip = (v < V - 1) ? eP[2][v + 1] : sEz[r * Ni + ipl];
+                                const C ex_jp = sEx[r * Ni + ib + v];
+                                const C ez_jp = sEz[(r + 1) * Ni + ib + v];
+                                syx[v] = stored<S, C>(syx[v] * ax + cx_yx * (ez_ip - eP[2][v]));
+                                szx[v] = stored<S, C>(szx[v] * ax - cx_zx * (ey_ip - eP[1][v]));
+                                sxy[v] = stored<S, C>(sxy[v] * ay - cy_xy * (ez_jp - eP[2][v]));
+                                szy[v] = stored<S, C>(szy[v] * ay + cy_zy * (ex_jp - eP[0][v]));
+                                sxz[v] = stored<S, C>(sxz[v] * az + cz_xz * (e1[1][v] - eP[1][v]));
+                                syz[v] = stored<S, C>(syz[v] * az - cz_yz * (e1[0][v] - eP[0][v]));
+                                nb[0][v] = sxy[v] + sxz[v];
+                                nb[1][v] = syz[v] + syx[v];
+                                nb[2][v] = szx[v] + szy[v];
+                            }
+                            store_pack<S, C, V>(P.bs[kBxy] + offb, sxy);
+                            store_pack<S, C, V>(P.bs[kBxz] + offb, sxz);
+                            store_pack<S, C, V>(P.bs[kByx] + offb, syx);
+                            store_pack<S, C, V>(P.bs[kByz] + offb, syz);
+                            store_pack<S, C, V>(P.bs[kBzx] + offb, szx);
+                            store_pack<S, C, V>(P.bs[kBzy] + offb, szy);
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < V; ++v) {
+                                if (rowm && in_main(ib + v, Ni, P.pi)) continue;
+                                nb[0][v] = bP[0][v];
+                                nb[1][v] = bP[1][v];
+                                nb[2][v] = bP[2][v];
+                            }
+                        }
                     }
                 }
 #pragma unroll
@@ -445,6 +688,27 @@ fused_sweep(SweepArgs<S, C, JT> a) {
     const int tile_y = t % a.g.ntile_y;
     const int chunk = t / a.g.ntile_y;
     fused_tile<S, C, JT, V, STEADY>(a, tile_y, chunk, reinterpret_cast<C*>(lds_raw));
+}
+
+// The fused sweep of a grid with a PML layer (same tiles and LDS staging;
+// layer cells also read and write their split fields).  Halo rows and
+// chunk-boundary planes recompute layer E from the unmodified es_in, so the
+// E splits are ping-ponged like E and B; B splits are only touched by
+// their owning tile and stay in place.
+__host__ __device__ constexpr int fused_pml_max_threads(int V, int csize) {
+    return V * csize >= 16 ? 512 : 1024;
+}
+
+template <typename S, typename C, typename JT, int V, bool STEADY>
+__global__ void __launch_bounds__(fused_pml_max_threads(V, sizeof(C)))
+fused_pml_sweep(SweepArgs<S, C, JT> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int b = blockIdx.x;
+    const int t = (b & 7) * a.g.per_xcd + (b >> 3);
+    if (t >= a.g.ntiles) return;
+    const int tile_y = t % a.g.ntile_y;
+    const int chunk = t / a.g.ntile_y;
+    fused_tile<S, C, JT, V, STEADY, true>(a, tile_y, chunk, reinterpret_cast<C*>(lds_raw));
 }
 
 // ---------------------------------------------------------------------------

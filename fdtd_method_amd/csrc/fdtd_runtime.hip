@@ -72,13 +72,16 @@ int v_candidates(int precision, int* out) {
 }
 
 int max_threads(const fdtd_grid* g, int V) {
-    return fdtd::fused_max_threads(V, (int)compute_size(g->precision));
+    const int cs = (int)compute_size(g->precision);
+    return g->pml ? fdtd::fused_pml_max_threads(V, cs) : fdtd::fused_max_threads(V, cs);
 }
 
 bool shape_valid(const fdtd_grid* g, int V, int TY, int kchunk) {
     if (V <= 0 || TY <= 0 || kchunk <= 0) return false;
     if (V != 1 && V != 2 && V != 4 && V != 8) return false;
     if (g->precision == FDTD_FP64 && V > 4) return false;
+    // the PML sweep holds the splits of its V cells too: wider lanes spill
+    if (g->pml && V * (int)compute_size(g->precision) > 16) return false;
     if (g->Ni % V != 0) return false;
     const int lanes = g->Ni / V;
     if ((TY + 1) * lanes > max_threads(g, V)) return false;
@@ -122,6 +125,7 @@ bool default_tuning(const fdtd_grid* g, Tuning* t) {
         for (int q = 0; q < nv; ++q) {
             const int V = vc[q];
             if (g->Ni % V) continue;
+            if (g->pml && V * cs > 16) continue;
             const int lanes = g->Ni / V;
             const int cap = cs == 8 ? 768 : 640;
             const int mt = pass == 0 ? std::min(cap, max_threads(g, V)) : max_threads(g, V);
@@ -240,12 +244,22 @@ bool blocked_valid(const fdtd_grid* g, const BlockTuning& t) {
 }
 
 template <typename S>
-int launch_plain_B(fdtd_grid* g) {
+int launch_plain_B(fdtd_grid* g, bool layer) {
     using C = typename fdtd::ComputeOf<S>::type;
     const dim3 block(64, 4, 1);
     const dim3 grid((g->Ni + 63) / 64, (g->Nj + 3) / 4, g->Nk);
-    hipLaunchKernelGGL((fdtd::plain_update_B<S, C>), grid, block, 0, g->stream,
-                       fields_of<S>(g, g->cur), make_coefs<C>(g), g->Ni, g->Nj, g->Nk);
+    if (g->pml) {
+        const auto P = make_pml<S, C>(g, g->cur, g->cur);
+        if (layer)
+            hipLaunchKernelGGL((fdtd::pml_plain_update_B<S, C, true>), grid, block, 0, g->stream,
+                               fields_of<S>(g, g->cur), P, make_coefs<C>(g), g->Ni, g->Nj, g->Nk);
+        else
+            hipLaunchKernelGGL((fdtd::pml_plain_update_B<S, C, false>), grid, block, 0, g->stream,
+                               fields_of<S>(g, g->cur), P, make_coefs<C>(g), g->Ni, g->Nj, g->Nk);
+    } else {
+        hipLaunchKernelGGL((fdtd::plain_update_B<S, C>), grid, block, 0, g->stream,
+                           fields_of<S>(g, g->cur), make_coefs<C>(g), g->Ni, g->Nj, g->Nk);
+    }
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
@@ -260,18 +274,24 @@ int launch_plain_E(fdtd_grid* g, long long step) {
     if (g->j_dense_active)
         for (int d = 0; d < 3; ++d) j[d] = static_cast<const JT*>(g->J[d]);
     if (int rc = eval_sources(g, step, 1)) return rc;
-    hipLaunchKernelGGL((fdtd::plain_update_E<S, C, JT>), grid, block, 0, g->stream,
-                       fields_of<S>(g, g->cur), j[0], j[1], j[2], g->d_src, g->d_vals, g->nsrc,
-                       make_coefs<C>(g), g->Ni, g->Nj, g->Nk);
+    if (g->pml)
+        hipLaunchKernelGGL((fdtd::pml_plain_update_E<S, C, JT>), grid, block, 0, g->stream,
+                           fields_of<S>(g, g->cur), make_pml<S, C>(g, g->cur, g->cur), j[0], j[1],
+                           j[2], g->d_src, g->d_vals, g->nsrc, make_coefs<C>(g), g->Ni, g->Nj,
+                           g->Nk);
+    else
+        hipLaunchKernelGGL((fdtd::plain_update_E<S, C, JT>), grid, block, 0, g->stream,
+                           fields_of<S>(g, g->cur), j[0], j[1], j[2], g->d_src, g->d_vals, g->nsrc,
+                           make_coefs<C>(g), g->Ni, g->Nj, g->Nk);
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
 
-int plain_B(fdtd_grid* g) {
+int plain_B(fdtd_grid* g, bool layer) {
     switch (g->precision) {
-        case FDTD_FP64: return launch_plain_B<double>(g);
-        case FDTD_FP32: return launch_plain_B<float>(g);
-        default: return launch_plain_B<_Float16>(g);
+        case FDTD_FP64: return launch_plain_B<double>(g, layer);
+        case FDTD_FP32: return launch_plain_B<float>(g, layer);
+        default: return launch_plain_B<_Float16>(g, layer);
     }
 }
 
@@ -283,13 +303,95 @@ int plain_E(fdtd_grid* g, long long step) {
     }
 }
 
+// ---- PML layer ----------------------------------------------------------------
+
+// Layer width of an axis of n cells, src/FDTD/FDTD_PML.cpp:254-256.
+int pml_width(int n, double pml_percent) {
+    return static_cast<int>(static_cast<double>(n) * pml_percent);
+}
+
+// Conductivity along one axis of n cells with layer width p and spacing d,
+// src/FDTD/FDTD_PML.cpp:3-61 as applied by :316-338: SGm * (dist/p)^4 with
+// dist = p - i in the lower band and i + 1 + p - n in the upper band (set
+// second), SGm = -(4 + 1)/2 * ln(1e-12) / (p*d); zero between the bands.
+std::vector<double> pml_sigma(int n, int p, double d) {
+    std::vector<double> s(n, 0.0);
+    if (p <= 0) return s;
+    const double sgm = -(kPmlOrder + 1.0) / 2.0 * std::log(kPmlR) / (static_cast<double>(p) * d);
+    for (int i = 0; i < p; ++i)
+        s[i] = sgm * std::pow(static_cast<double>(p - i) / static_cast<double>(p), kPmlOrder);
+    for (int i = n - p; i < n; ++i)
+        s[i] = sgm * std::pow(static_cast<double>(i + 1 + p - n) / static_cast<double>(p), kPmlOrder);
+    return s;
+}
+
+// Decay exp(-sigma*dt*C) and split coefficient (1 - decay)/(sigma*d), or
+// C*dt/d where sigma == 0 (src/FDTD/FDTD_PML.cpp:63-65,98-111).
+void pml_coef(double sigma, double dt, double d, double* decay, double* coef) {
+    *decay = std::exp(-sigma * dt * kC);
+    *coef = sigma != 0.0 ? (1.0 - std::exp(-sigma * dt * kC)) / (sigma * d) : kC * dt / d;
+}
+
+// Per-axis tables (see fdtd::Pml) in the compute type, with the storage
+// scale ratio of each split's target and source folded in.
+int build_pml_tables(fdtd_grid* g) {
+    const int n[3] = {g->Ni, g->Nj, g->Nk};
+    const double d[3] = {g->params.dx, g->params.dy, g->params.dz};
+    const double* s = g->scales;  // Ex Ey Ez Bx By Bz
+    // x: Eyx<-Bz, Ezx<-By, Byx<-Ez, Bzx<-Ey; y: Exy<-Bz, Ezy<-Bx, Bxy<-Ez,
+    // Bzy<-Ex; z: Exz<-By, Eyz<-Bx, Bxz<-Ey, Byz<-Ex
+    const double ratio[3][4] = {{s[1] / s[5], s[2] / s[4], s[4] / s[2], s[5] / s[1]},
+                                {s[0] / s[5], s[2] / s[3], s[3] / s[2], s[5] / s[0]},
+                                {s[0] / s[4], s[1] / s[3], s[3] / s[1], s[4] / s[0]}};
+    std::vector<double> tab;
+    tab.reserve(5 * (n[0] + n[1] + n[2]));
+    for (int a = 0; a < 3; ++a) {
+        const std::vector<double> sigma = pml_sigma(n[a], g->pml_p[a], d[a]);
+        std::vector<double> dec(n[a]), coef(n[a]);
+        for (int i = 0; i < n[a]; ++i) pml_coef(sigma[i], g->dt, d[a], &dec[i], &coef[i]);
+        tab.insert(tab.end(), dec.begin(), dec.end());
+        for (int r = 0; r < 4; ++r)
+            for (int i = 0; i < n[a]; ++i) tab.push_back(coef[i] * ratio[a][r]);
+    }
+    // sweeps already queued may still read the old tables
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (compute_size(g->precision) == 8) {
+        HIP_TRY(hipMemcpy(g->pml_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> f(tab.begin(), tab.end());
+        HIP_TRY(hipMemcpy(g->pml_tab, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    return FDTD_OK;
+}
+
+// Storage of a PML grid's layer: split fields (zeroed) and tables.
+int alloc_pml(fdtd_grid* g) {
+    const size_t bytes = g->N * g->esize + 16;
+    for (int q = 0; q < 6; ++q) {
+        for (int b = 0; b < 2; ++b) {
+            HIP_TRY(hipMalloc(&g->ES[b][q], bytes));
+            if (int rc = zero_fill(g->ES[b][q], bytes, g->stream)) return rc;
+        }
+        HIP_TRY(hipMalloc(&g->BS[q], bytes));
+        if (int rc = zero_fill(g->BS[q], bytes, g->stream)) return rc;
+    }
+    HIP_TRY(hipMalloc(&g->pml_tab, 5 * (size_t)(g->Ni + g->Nj + g->Nk) * compute_size(g->precision)));
+    return build_pml_tables(g);
+}
+
+// Storage component (0..5) whose scale split q (0..5 E, 6..11 B) shares.
+int split_component(int q) { return q < 6 ? q / 2 : 3 + (q - 6) / 2; }
+
+void* split_ptr(fdtd_grid* g, int q) { return q < 6 ? g->ES[g->cur][q] : g->BS[q - 6]; }
+
 // ---- fused sweep --------------------------------------------------------------
 
 template <typename S, int V, bool STEADY>
 int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out) {
     using C = typename fdtd::ComputeOf<S>::type;
     using JT = C;
-    auto kern = fdtd::fused_sweep<S, C, JT, V, STEADY>;
+    auto kern = g->pml ? fdtd::fused_pml_sweep<S, C, JT, V, STEADY>
+                       : fdtd::fused_sweep<S, C, JT, V, STEADY>;
     fdtd::SweepArgs<S, C, JT> a;
     a.in = fields_of<S>(g, in);
     a.out = fields_of<S>(g, out);
@@ -301,6 +403,7 @@ int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out) {
     a.pad = 0;
     a.c = make_coefs<C>(g);
     a.g = make_geom(g, t);
+    a.pml = make_pml<S, C>(g, in, out);
     const size_t lds = fdtd::fused_lds_bytes(t.TY, g->Ni, sizeof(C));
     if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
     const int threads = (t.TY + 1) * (g->Ni / V);
@@ -338,6 +441,7 @@ int launch_fused(fdtd_grid* g, const Tuning& t, bool steady, int slot, int in, i
 }
 
 int rescale_to_target(fdtd_grid* g, bool current_only);
+int apply_scales(fdtd_grid* g, const double s[6], bool current_only);
 
 // ---- temporally blocked sweeps ----------------------------------------------------
 
@@ -434,7 +538,7 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
 // With FP16 autoscaling, the state between two sweeps, (E_n, B_{n+1/2}),
 // is rescaled by exact powers of two every autoscale_interval sweeps.
 int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
-    if (g->btune.L >= 1 && nsteps >= g->btune.L) return run_blocked(g, nsteps, mid);
+    if (g->btune.L >= 1 && nsteps >= g->btune.L && !g->pml) return run_blocked(g, nsteps, mid);
     int rc = plain_B(g);
     if (rc) return rc;
     if (mid) HIP_TRY(hipEventRecord(mid, g->stream));
@@ -454,9 +558,9 @@ int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
 
 int run_3sweep(fdtd_grid* g, int nsteps) {
     for (int s = 0; s < nsteps; ++s) {
-        int rc = plain_B(g);
+        int rc = plain_B(g, true);
         if (!rc) rc = plain_E(g, g->step);
-        if (!rc) rc = plain_B(g);
+        if (!rc) rc = plain_B(g, false);
         if (rc) return rc;
         g->step += 1;
     }
@@ -558,7 +662,14 @@ extern "C" {
 
 const char* fdtd_last_error(void) { return g_last_error.c_str(); }
 
-int fdtd_create(const fdtd_parameters* p, double dt, int precision, fdtd_grid** out) {
+}  // extern "C"
+
+namespace fdtd::rt {
+
+// Common constructor of fdtd_create and fdtd_create_pml (pml_percent < 0:
+// no layer).
+int create_grid(const fdtd_parameters* p, double dt, double pml_percent, int precision,
+                fdtd_grid** out) {
     if (!p || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     // src/FDTD/FDTD.cpp:5-7
@@ -578,6 +689,14 @@ int fdtd_create(const fdtd_parameters* p, double dt, int precision, fdtd_grid** 
     g->dt = dt;
     g->precision = precision;
     g->esize = storage_size(precision);
+    if (pml_percent >= 0) {
+        g->pml_percent = pml_percent;
+        g->pml_p[0] = pml_width(p->Ni, pml_percent);
+        g->pml_p[1] = pml_width(p->Nj, pml_percent);
+        g->pml_p[2] = pml_width(p->Nk, pml_percent);
+        // all widths zero: the class is the plain FDTD one
+        g->pml = g->pml_p[0] > 0 || g->pml_p[1] > 0 || g->pml_p[2] > 0;
+    }
     auto cleanup = [&](int rc) {
         fdtd_destroy(g);
         return rc;
@@ -596,6 +715,8 @@ int fdtd_create(const fdtd_parameters* p, double dt, int precision, fdtd_grid** 
         if (!rc) rc = zero_fill(g->B[0][d], bytes, g->stream);
         if (rc) return cleanup(rc);
     }
+    if (g->pml)
+        if (int rc = alloc_pml(g)) return cleanup(rc);
     if (hipMalloc(&g->d_part, sizeof(double) * 3 * kReduceBlocks) != hipSuccess ||
         hipMalloc(&g->d_out, sizeof(double) * 8) != hipSuccess)
         return cleanup(fail(FDTD_ERR_HIP, "out of device memory"));
@@ -608,6 +729,24 @@ int fdtd_create(const fdtd_parameters* p, double dt, int precision, fdtd_grid** 
     return FDTD_OK;
 }
 
+}  // namespace fdtd::rt
+
+extern "C" {
+
+int fdtd_create(const fdtd_parameters* p, double dt, int precision, fdtd_grid** out) {
+    return create_grid(p, dt, -1.0, precision, out);
+}
+
+int fdtd_create_pml(const fdtd_parameters* p, double dt, double pml_percent, int precision,
+                    fdtd_grid** out) {
+    if (out) *out = nullptr;
+    // 2*width <= N keeps the six layer slabs of FDTD_PML.cpp:346-362
+    // disjoint (the reference updates overlapping cells twice).
+    if (!(pml_percent >= 0.0 && pml_percent <= 0.5))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "pml_percent %g outside [0, 0.5]", pml_percent);
+    return create_grid(p, dt, pml_percent, precision, out);
+}
+
 int fdtd_destroy(fdtd_grid* g) {
     if (!g) return FDTD_OK;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
@@ -616,6 +755,12 @@ int fdtd_destroy(fdtd_grid* g) {
             if (g->E[b][d]) (void)hipFree(g->E[b][d]);
             if (g->B[b][d]) (void)hipFree(g->B[b][d]);
         }
+    for (int q = 0; q < 6; ++q) {
+        for (int b = 0; b < 2; ++b)
+            if (g->ES[b][q]) (void)hipFree(g->ES[b][q]);
+        if (g->BS[q]) (void)hipFree(g->BS[q]);
+    }
+    if (g->pml_tab) (void)hipFree(g->pml_tab);
     for (int d = 0; d < 3; ++d)
         if (g->J[d]) (void)hipFree(g->J[d]);
     if (g->staging) (void)hipFree(g->staging);
@@ -627,6 +772,51 @@ int fdtd_destroy(fdtd_grid* g) {
         if (e) (void)hipEventDestroy(e);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
+    return FDTD_OK;
+}
+
+int fdtd_get_pml(const fdtd_grid* g, int widths[3], double* pml_percent) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (widths)
+        for (int a = 0; a < 3; ++a) widths[a] = g->pml_p[a];
+    if (pml_percent) *pml_percent = g->pml_percent;
+    return FDTD_OK;
+}
+
+int fdtd_pml_download_split(fdtd_grid* g, int q, double* host, size_t count) {
+    if (!g || !host) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (q < 0 || q > 11) return fail(FDTD_ERR_INVALID_ARGUMENT, "split index %d outside 0..11", q);
+    if (count != g->N)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "count %zu != grid cells %zu", count, g->N);
+    if (!g->pml) {
+        std::memset(host, 0, g->N * sizeof(double));
+        return FDTD_OK;
+    }
+    const void* src = split_ptr(g, q);
+    if (g->precision == FDTD_FP64) {
+        HIP_TRY(hipMemcpyAsync(host, src, g->N * 8, hipMemcpyDeviceToHost, g->stream));
+    } else {
+        if (int rc = ensure_staging(g)) return rc;
+        to_f64(g->precision, src, g->staging, g->N, 1.0 / g->scales[split_component(q)], g->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host, g->staging, g->N * 8, hipMemcpyDeviceToHost, g->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return FDTD_OK;
+}
+
+int fdtd_pml_profile(int n, double pml_percent, double d, double dt, double* sigma,
+                     double* decay, double* coef) {
+    if (n <= 0 || !(d > 0) || !(dt > 0) || !(pml_percent >= 0.0 && pml_percent <= 0.5))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    const std::vector<double> s = pml_sigma(n, pml_width(n, pml_percent), d);
+    for (int i = 0; i < n; ++i) {
+        double a, c;
+        pml_coef(s[i], dt, d, &a, &c);
+        if (sigma) sigma[i] = s[i];
+        if (decay) decay[i] = a;
+        if (coef) coef[i] = c;
+    }
     return FDTD_OK;
 }
 
@@ -965,17 +1155,7 @@ int fdtd_set_scales(fdtd_grid* g, const double scales[6]) {
                 return fail(FDTD_ERR_UNSUPPORTED, "storage scales apply to FP16 grids only");
         return FDTD_OK;
     }
-    for (int c = 0; c < 6; ++c) {
-        const double f = scales[c] / g->scales[c];
-        if (f != 1.0)
-            for (int b = 0; b < 2; ++b) {
-                void* p = c < 3 ? g->E[b][c] : g->B[b][c - 3];
-                hipLaunchKernelGGL(fdtd::scale_array<_Float16>, dim3(grid_blocks(g->N)), dim3(256), 0,
-                                   g->stream, static_cast<_Float16*>(p), g->N, (float)f);
-            }
-        g->scales[c] = scales[c];
-    }
-    HIP_TRY(hipGetLastError());
+    if (int rc = apply_scales(g, scales, false)) return rc;
     HIP_TRY(hipStreamSynchronize(g->stream));
     return FDTD_OK;
 }
@@ -1003,19 +1183,38 @@ void target_scales(const fdtd_grid* g, const double mx[6], int target_log2, doub
     }
 }
 
+// Rescales the FP16 state to the scales s (exact powers of two): E and B
+// (both buffers, or only the current ones), the PML split fields with their
+// totals' scales, and the PML tables' scale ratios.
 int apply_scales(fdtd_grid* g, const double s[6], bool current_only) {
+    auto scale = [&](void* p, double f) {
+        hipLaunchKernelGGL(fdtd::scale_array<_Float16>, dim3(grid_blocks(g->N)), dim3(256), 0,
+                           g->stream, static_cast<_Float16*>(p), g->N, (float)f);
+    };
+    bool changed = false;
     for (int c = 0; c < 6; ++c) {
         const double f = s[c] / g->scales[c];
-        if (f != 1.0)
-            for (int b = 0; b < 2; ++b) {
-                if (current_only && b != g->cur) continue;
-                void* p = c < 3 ? g->E[b][c] : g->B[b][c - 3];
-                hipLaunchKernelGGL(fdtd::scale_array<_Float16>, dim3(grid_blocks(g->N)), dim3(256), 0,
-                                   g->stream, static_cast<_Float16*>(p), g->N, (float)f);
+        if (f == 1.0) continue;
+        changed = true;
+        for (int b = 0; b < 2; ++b) {
+            if (current_only && b != g->cur) continue;
+            scale(c < 3 ? g->E[b][c] : g->B[b][c - 3], f);
+        }
+        if (g->pml)
+            for (int q = 0; q < 12; ++q) {
+                if (split_component(q) != c) continue;
+                if (q < 6) {
+                    for (int b = 0; b < 2; ++b)
+                        if (!current_only || b == g->cur) scale(g->ES[b][q], f);
+                } else {
+                    scale(g->BS[q - 6], f);
+                }
             }
         g->scales[c] = s[c];
     }
     HIP_TRY(hipGetLastError());
+    if (changed && g->pml)
+        if (int rc = build_pml_tables(g)) return rc;
     return FDTD_OK;
 }
 
@@ -1060,6 +1259,12 @@ int fdtd_zero_fields(fdtd_grid* g) {
             if (int rc = zero_fill(g->E[b][d], bytes, g->stream)) return rc;
             if (int rc = zero_fill(g->B[b][d], bytes, g->stream)) return rc;
         }
+    if (g->pml)
+        for (int q = 0; q < 6; ++q) {
+            for (int b = 0; b < 2; ++b)
+                if (int rc = zero_fill(g->ES[b][q], bytes, g->stream)) return rc;
+            if (int rc = zero_fill(g->BS[q], bytes, g->stream)) return rc;
+        }
     g->j_dense_active = false;
     if (!g->sparse_j.empty()) g->src_dirty = true;
     g->sparse_j.clear();
@@ -1088,6 +1293,7 @@ int fdtd_set_time_blocking(fdtd_grid* g, int L, int V, int lanes, int TYg, int k
         g->btune.L = 0;
         return FDTD_OK;
     }
+    if (g->pml) return fail(FDTD_ERR_UNSUPPORTED, "temporal blocking takes periodic grids (no PML layer)");
     BlockTuning t;
     t.L = L;
     t.V = V > 0 ? V : ((g->Ni % 2 == 0) ? 2 : 1);
@@ -1118,6 +1324,7 @@ int fdtd_set_time_blocking(fdtd_grid* g, int L, int V, int lanes, int TYg, int k
 
 int fdtd_autotune_blocking(fdtd_grid* g, int reps, int out[5], float* best_ms) {
     if (!g || reps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (g->pml) return fail(FDTD_ERR_UNSUPPORTED, "temporal blocking takes periodic grids (no PML layer)");
     if (int rc = flush_sources(g)) return rc;
     // Candidate passes are timed on scratch copies of the state (restored
     // afterwards); the full-row kernel's time per step is the reference.
@@ -1224,14 +1431,17 @@ int fdtd_time_update_fields(fdtd_grid* g, int nsteps, int path, float* ms_total,
 int fdtd_autotune(fdtd_grid* g, int reps, int out[3], float* best_ms) {
     if (!g || reps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
     if (int rc = flush_sources(g)) return rc;
-    // Time sweeps between the two buffers of the grid; the fields are saved
-    // to and restored from scratch copies so tuning leaves the state intact.
+    // Time sweeps between the two buffers of the grid; the fields (and PML
+    // splits) are saved to and restored from scratch copies so tuning leaves
+    // the state intact.
     std::vector<void*> saved;
     const size_t bytes = g->N * g->esize;
-    for (int d = 0; d < 6; ++d) {
+    const int nstate = g->pml ? 18 : 6;
+    auto state_ptr = [&](int d) { return d < 6 ? field_ptr(g, d) : split_ptr(g, d - 6); };
+    for (int d = 0; d < nstate; ++d) {
         void* p = nullptr;
         HIP_TRY(hipMalloc(&p, bytes));
-        HIP_TRY(hipMemcpyAsync(p, field_ptr(g, d), bytes, hipMemcpyDeviceToDevice, g->stream));
+        HIP_TRY(hipMemcpyAsync(p, state_ptr(d), bytes, hipMemcpyDeviceToDevice, g->stream));
         saved.push_back(p);
     }
     int vc[4];
@@ -1271,8 +1481,8 @@ int fdtd_autotune(fdtd_grid* g, int reps, int out[3], float* best_ms) {
             }
         }
     }
-    for (int d = 0; d < 6; ++d) {
-        HIP_TRY(hipMemcpyAsync(field_ptr(g, d), saved[d], bytes, hipMemcpyDeviceToDevice, g->stream));
+    for (int d = 0; d < nstate; ++d) {
+        HIP_TRY(hipMemcpyAsync(state_ptr(d), saved[d], bytes, hipMemcpyDeviceToDevice, g->stream));
     }
     HIP_TRY(hipStreamSynchronize(g->stream));
     for (void* p : saved) HIP_TRY(hipFree(p));

@@ -5,11 +5,15 @@
 //
 //   sample                 N = 32, 100 iterations
 //   sample N iters         as the reference
-//   sample N iters [fp64|fp32|fp16] [--dump FILE]
+//   sample N iters [fp64|fp32|fp16] [--pml [PCT]] [--dump FILE] [--dump-pml FILE]
 //
 // Prints "Execution time: X s" and the 10x10 Ex slice around the centre of
 // plane Nk/2 in the reference's format (setw(12), fixed, setprecision(5)).
-// --dump writes Ex (N^3 doubles, index i + j*N + k*N*N) to FILE.
+// --pml also runs the same wave on the PML class with pml_percent PCT
+// (default 0.2), as the reference driver's __PML_TEST__ build does
+// (sample.cpp:93-123,136-147): "Execution time (PML): X s" after the first
+// time, and "PML: " with its slice after the first slice.
+// --dump / --dump-pml write Ex (N^3 doubles, index i + j*N + k*N*N) to FILE.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -18,31 +22,40 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <memory>
 #include <string>
 
 #include "FDTD_mi355x.hpp"
 
 using namespace FDTD_mi355x;
 
-static int spherical_wave(int n, int it, Precision prec, const char* dump) {
-    // perf-tests/sample/sample.cpp:15-31
-    CurrentParameters cur_param{8, 4, 0.2};
+static Parameters wave_parameters(int n) {
+    const double d = FDTD_const::C;  // perf-tests/sample/sample.cpp:33-49
+    const double boundary = static_cast<double>(n) / 2.0 * d;
+    return Parameters{n, n, n, -boundary, boundary, -boundary, boundary, -boundary, boundary,
+                      d, d, d};
+}
+
+static CurrentParameters wave_current() {
+    CurrentParameters cur_param{8, 4, 0.2};  // sample.cpp:15-19,24
+    cur_param.iterations = static_cast<int>(static_cast<double>(cur_param.period) / cur_param.dt);
+    return cur_param;
+}
+
+// The drivers' time loop (perf-tests/sample/sample.cpp:55-87) on `method`:
+// the cos^2 box current on Jx = Jy = Jz before each of the first
+// min(iterations, it) steps, then zeroed_currents() and the remaining steps.
+// Returns the elapsed seconds.
+static double run_wave(FDTD& method, const Parameters& params, int it) {
+    const CurrentParameters cur_param = wave_current();
     const double T = cur_param.period;
     const double Tx = cur_param.period_x, Ty = cur_param.period_y, Tz = cur_param.period_z;
-    cur_param.iterations = static_cast<int>(static_cast<double>(cur_param.period) / cur_param.dt);
     auto current = [T, Tx, Ty, Tz](double x, double y, double z, double t) {
         return std::sin(2.0 * FDTD_const::PI * t / T) *
                std::pow(std::cos(2.0 * FDTD_const::PI * x / Tx), 2.0) *
                std::pow(std::cos(2.0 * FDTD_const::PI * y / Ty), 2.0) *
                std::pow(std::cos(2.0 * FDTD_const::PI * z / Tz), 2.0);
     };
-
-    const double d = FDTD_const::C;
-    const double boundary = static_cast<double>(n) / 2.0 * d;
-    Parameters params{n, n, n, -boundary, boundary, -boundary, boundary, -boundary, boundary,
-                      d, d, d};
-    FDTD method(params, cur_param.dt, prec);
-
     const int cur_time = std::min(cur_param.iterations, it);
     const int start_i = static_cast<int>(std::floor((-Tx / 4.0 - params.ax) / params.dx));
     const int start_j = static_cast<int>(std::floor((-Ty / 4.0 - params.ay) / params.dy));
@@ -72,8 +85,11 @@ static int spherical_wave(int n, int it, Precision prec, const char* dump) {
     method.synchronize();
     auto end = std::chrono::high_resolution_clock::now();
     std::chrono::duration<double> elapsed = end - start;
-    std::cout << "Execution time: " << elapsed.count() << " s" << std::endl;
+    return elapsed.count();
+}
 
+// 10x10 Ex slice of plane Nk/2 (sample.cpp:125-134).
+static void print_slice(FDTD& method, const Parameters& params) {
     Field& Ex = method.get_field(Component::EX);
     const int k = params.Nk / 2;
     for (int j = params.Nj / 2 - 5; j < params.Nj / 2 + 5; j++) {
@@ -84,27 +100,59 @@ static int spherical_wave(int n, int it, Precision prec, const char* dump) {
         std::cout << std::endl;
     }
     std::cout << std::endl;
+}
 
-    if (dump) {
-        std::ofstream out(dump, std::ios::binary);
-        const double* p = Ex.data();
-        out.write(reinterpret_cast<const char*>(p), sizeof(double) * method.cells());
-        if (!out) {
-            std::cerr << "ERROR: cannot write " << dump << std::endl;
-            return 1;
-        }
+static bool dump_ex(FDTD& method, const char* path) {
+    if (!path) return true;
+    std::ofstream out(path, std::ios::binary);
+    const double* p = method.get_field(Component::EX).data();
+    out.write(reinterpret_cast<const char*>(p), sizeof(double) * method.cells());
+    if (!out) {
+        std::cerr << "ERROR: cannot write " << path << std::endl;
+        return false;
     }
+    return true;
+}
+
+static int spherical_wave(int n, int it, Precision prec, double pml, const char* dump,
+                          const char* dump_pml) {
+    const Parameters params = wave_parameters(n);
+    const double dt = wave_current().dt;
+    FDTD method(params, dt, prec);
+    const double elapsed = run_wave(method, params, it);
+    std::cout << "Execution time: " << elapsed << " s" << std::endl;
+
+    std::unique_ptr<FDTD_PML> pml_method;
+    if (pml >= 0) {  // sample.cpp:93-123
+        pml_method.reset(new FDTD_PML(params, dt, pml, prec));
+        const double elapsed_pml = run_wave(*pml_method, params, it);
+        std::cout << "Execution time (PML): " << elapsed_pml << " s" << std::endl;
+    }
+    print_slice(method, params);
+    if (pml_method) {  // sample.cpp:136-147
+        std::cout << "PML: \n" << std::endl;
+        print_slice(*pml_method, params);
+    }
+    if (!dump_ex(method, dump)) return 1;
+    if (pml_method && !dump_ex(*pml_method, dump_pml)) return 1;
     return 0;
 }
 
 int main(int argc, char* argv[]) {
     int N = 32, iters = 100;
     Precision prec = Precision::FP64;
+    double pml = -1.0;
     const char* dump = nullptr;
+    const char* dump_pml = nullptr;
     int pos = 0;
     for (int a = 1; a < argc; ++a) {
         if (std::strcmp(argv[a], "--dump") == 0 && a + 1 < argc) {
             dump = argv[++a];
+        } else if (std::strcmp(argv[a], "--dump-pml") == 0 && a + 1 < argc) {
+            dump_pml = argv[++a];
+        } else if (std::strcmp(argv[a], "--pml") == 0) {
+            pml = 0.2;  // sample.cpp:94
+            if (a + 1 < argc && std::strchr(argv[a + 1], '.')) pml = std::atof(argv[++a]);
         } else if (std::strcmp(argv[a], "fp64") == 0) {
             prec = Precision::FP64;
         } else if (std::strcmp(argv[a], "fp32") == 0) {
@@ -126,7 +174,7 @@ int main(int argc, char* argv[]) {
         return 1;
     }
     try {
-        return spherical_wave(N, iters, prec, dump);
+        return spherical_wave(N, iters, prec, pml, dump, dump_pml);
     } catch (const std::exception& e) {
         std::cerr << e.what() << std::endl;
         return 1;

@@ -11,6 +11,7 @@ reference                              here
 ``Field& get_field(Component)``        ``get_field(component) -> Field``
 ``virtual void update_fields()``       ``update_fields(nsteps=1)``
 ``void zeroed_currents()``             ``zeroed_currents()``
+``FDTD_PML(Parameters, FP dt, FP pml)`` ``FDTD_PML(parameters, dt, pml_percent, precision=FP64)``
 =====================================  ==========================================
 
 Errors match the reference's: ``ValueError("ERROR: invalid parameters")``
@@ -104,12 +105,19 @@ class FDTD:
     """Yee solver on one MI355X grid (reference API, see module docstring)."""
 
     def __init__(self, parameters: Parameters, dt: float, precision: Precision = Precision.FP64):
+        self._open(parameters, dt, precision, None)
+
+    def _open(self, parameters, dt, precision, pml_percent) -> None:
         L = _lib.load()
         p = _lib.Parameters(parameters.Ni, parameters.Nj, parameters.Nk, parameters.ax,
                             parameters.bx, parameters.ay, parameters.by, parameters.az,
                             parameters.bz, parameters.dx, parameters.dy, parameters.dz)
         h = ctypes.c_void_p()
-        _lib.check(L.fdtd_create(ctypes.byref(p), float(dt), int(precision), ctypes.byref(h)))
+        if pml_percent is None:
+            _lib.check(L.fdtd_create(ctypes.byref(p), float(dt), int(precision), ctypes.byref(h)))
+        else:
+            _lib.check(L.fdtd_create_pml(ctypes.byref(p), float(dt), float(pml_percent),
+                                         int(precision), ctypes.byref(h)))
         self._h = h
         self._L = L
         self.parameters = parameters
@@ -316,6 +324,45 @@ class FDTD:
                                                    ctypes.byref(a), ctypes.byref(b)))
         self._invalidate()
         return a.value, b.value
+
+
+class FDTD_PML(FDTD):
+    """The reference's PML subclass, ``FDTD_PML(Parameters, FP dt, FP pml_percent)``
+    (include/FDTD/FDTD_PML.h:10-44, src/FDTD/FDTD_PML.cpp:205-365): the outer
+    ``int(N*pml_percent)`` cells of every axis absorb outgoing waves; J acts
+    on the main box only.  Same methods as FDTD, plus the layer's widths and
+    split fields."""
+
+    SPLITS = ("Exy", "Exz", "Eyx", "Eyz", "Ezx", "Ezy",
+              "Bxy", "Bxz", "Byx", "Byz", "Bzx", "Bzy")  # FDTD_PML.h:12-13
+
+    def __init__(self, parameters: Parameters, dt: float, pml_percent: float,
+                 precision: Precision = Precision.FP64):
+        self._open(parameters, dt, precision, pml_percent)
+
+    @property
+    def pml_widths(self):
+        w = (ctypes.c_int * 3)()
+        pct = ctypes.c_double()
+        _lib.check(self._L.fdtd_get_pml(self._h, w, ctypes.byref(pct)))
+        return tuple(w)
+
+    def split(self, name) -> np.ndarray:
+        """FP64 copy of one split field, by name ("Exy", ...) or index 0..11."""
+        q = self.SPLITS.index(name) if isinstance(name, str) else int(name)
+        self._flush()
+        out = np.empty(self.cells, dtype=np.float64)
+        _lib.check(self._L.fdtd_pml_download_split(self._h, q, _dptr(out), out.size))
+        return out
+
+
+def pml_profile(n: int, pml_percent: float, d: float, dt: float):
+    """(sigma, decay, coef) of the layer along one axis (host-only, no GPU)."""
+    L = _lib.load()
+    out = [np.empty(n, dtype=np.float64) for _ in range(3)]
+    _lib.check(L.fdtd_pml_profile(int(n), float(pml_percent), float(d), float(dt),
+                                  *[_dptr(a) for a in out]))
+    return tuple(out)
 
 
 def calibration_copy(nbytes: int, iters: int = 10) -> float:

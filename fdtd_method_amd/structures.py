@@ -8,7 +8,7 @@ from __future__ import annotations
 import enum
 from dataclasses import dataclass
 
-# include/Constants.h:9-14 (Gaussian units; PI is the reference's truncated value)
+# include/Constants.h:6-11 (Gaussian units; PI is the reference's truncated value)
 C = 3e10
 R = 1e-12
 EPS0 = 1.0

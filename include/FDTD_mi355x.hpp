@@ -21,6 +21,7 @@
 // libfdtd_mi355x.so; no HIP header is required by callers.
 #pragma once
 
+#include <array>
 #include <cstddef>
 #include <stdexcept>
 #include <string>
@@ -154,15 +155,7 @@ class FDTD {
 public:
     // FDTD(Parameters, FP dt) -- src/FDTD/FDTD.cpp:3-61.
     FDTD(Parameters parameters, FP dt, Precision precision = Precision::FP64)
-        : parameters_(parameters), dt_(dt) {
-        fdtd_parameters p{parameters.Ni, parameters.Nj, parameters.Nk, parameters.ax,
-                          parameters.bx, parameters.ay, parameters.by, parameters.az,
-                          parameters.bz, parameters.dx, parameters.dy, parameters.dz};
-        check_status(fdtd_create(&p, dt, static_cast<int>(precision), &grid_));
-        cells_ = static_cast<std::size_t>(parameters.Ni) * parameters.Nj * parameters.Nk;
-        fields_.reserve(9);
-        for (int c = 0; c < 9; ++c) fields_.emplace_back(this, c);
-    }
+        : FDTD(parameters, dt, precision, -1.0) {}
     FDTD(const FDTD&) = delete;
     FDTD& operator=(const FDTD&) = delete;
     virtual ~FDTD() { fdtd_destroy(grid_); }
@@ -197,6 +190,23 @@ public:
 
     void synchronize() { check_status(fdtd_synchronize(grid_)); }
 
+protected:
+    // pml_percent < 0: periodic grid (fdtd_create); otherwise the PML
+    // subclass's grid (fdtd_create_pml).
+    FDTD(Parameters parameters, FP dt, Precision precision, FP pml_percent)
+        : parameters_(parameters), dt_(dt) {
+        fdtd_parameters p{parameters.Ni, parameters.Nj, parameters.Nk, parameters.ax,
+                          parameters.bx, parameters.ay, parameters.by, parameters.az,
+                          parameters.bz, parameters.dx, parameters.dy, parameters.dz};
+        if (pml_percent < 0)
+            check_status(fdtd_create(&p, dt, static_cast<int>(precision), &grid_));
+        else
+            check_status(fdtd_create_pml(&p, dt, pml_percent, static_cast<int>(precision), &grid_));
+        cells_ = static_cast<std::size_t>(parameters.Ni) * parameters.Nj * parameters.Nk;
+        fields_.reserve(9);
+        for (int c = 0; c < 9; ++c) fields_.emplace_back(this, c);
+    }
+
 private:
     friend class Field;
 
@@ -209,6 +219,31 @@ private:
     fdtd_grid* grid_ = nullptr;
     std::size_t cells_ = 0;
     std::vector<Field> fields_;
+};
+
+// FDTD_PML(Parameters, FP dt, FP pml_percent) -- include/FDTD/FDTD_PML.h:10-44,
+// src/FDTD/FDTD_PML.cpp:205-365: the outer (int)(N*pml_percent) cells of
+// every axis form an absorbing layer; J acts on the main box only.
+// pml_percent must lie in [0, 0.5] (std::runtime_error otherwise).
+class FDTD_PML : public FDTD {
+public:
+    FDTD_PML(Parameters parameters, FP dt, FP pml_percent,
+             Precision precision = Precision::FP64)
+        : FDTD(parameters, dt, precision, pml_percent) {}
+
+    using FDTD::update_fields;
+    // void update_fields() override -- src/FDTD/FDTD_PML.cpp:343-365.
+    void update_fields() override { FDTD::update_fields(1); }
+
+    // Layer widths {p_i, p_j, p_k} (FDTD_PML.cpp:254-256).
+    std::array<int, 3> pml_widths() const {
+        std::array<int, 3> w{};
+        check_status(fdtd_get_pml(grid(), w.data(), nullptr));
+        return w;
+    }
+
+private:
+    const fdtd_grid* grid() const { return const_cast<FDTD_PML*>(this)->handle(); }
 };
 
 inline std::size_t Field::size() const { return owner_->cells(); }

@@ -70,7 +70,7 @@ typedef struct fdtd_parameters {
  *   kind 0: amp
  *   kind 1: amp*sin(2*pi*(t-t0)/period + phase)*exp(-(t-t0)^2/(2*sigma^2))
  *           (no envelope when sigma <= 0); pi = 3.14159265358 as in
- *           include/Constants.h:14. */
+ *           include/Constants.h:11. */
 typedef struct fdtd_point_source {
     int i, j, k, comp;
     int kind, reserved;
@@ -89,6 +89,32 @@ const char* fdtd_last_error(void);
  * start at zero. */
 int fdtd_create(const fdtd_parameters* params, double dt, int precision, fdtd_grid** out);
 int fdtd_destroy(fdtd_grid* grid);
+
+/* FDTD_PML(Parameters, FP dt, FP pml_percent) -- src/FDTD/FDTD_PML.cpp:205-341,
+ * src/FDTD_kokkos/FDTD_PML_kokkos.cpp:22-157: a grid whose outer
+ * (int)(N*pml_percent) cells per side and axis form an absorbing layer
+ * (split fields, graded conductivity sigma = SGm*(dist/width)^4).  Every
+ * other call behaves as for fdtd_create's grids, with the class's step
+ * (FDTD_PML.cpp:343-365): J acts on the main box only.  pml_percent must lie
+ * in [0, 0.5] (the reference's layer slabs overlap beyond 0.5); when every
+ * width is 0 the grid is a plain periodic one, as in the reference.  Split
+ * fields start at zero.  Temporal blocking and batches take periodic grids
+ * only (FDTD_ERR_UNSUPPORTED). */
+int fdtd_create_pml(const fdtd_parameters* params, double dt, double pml_percent, int precision,
+                    fdtd_grid** out);
+/* Layer widths {p_i, p_j, p_k} (FDTD_PML.cpp:254-256) and pml_percent
+ * (widths 0 and pml_percent 0 for fdtd_create's grids). */
+int fdtd_get_pml(const fdtd_grid* grid, int widths[3], double* pml_percent);
+/* FP64 copy of split field q of a PML grid, the reference's private members
+ * Exy Exz Eyx Eyz Ezx Ezy (q = 0..5) and Bxy Bxz Byx Byz Bzx Bzy (6..11),
+ * include/FDTD/FDTD_PML.h:12-13; zeros for a grid without a layer. */
+int fdtd_pml_download_split(fdtd_grid* grid, int q, double* host, size_t count);
+/* Host-only (no GPU needed): the layer profile along one axis of n cells
+ * with spacing d -- conductivity sigma (FDTD_PML.cpp:3-61,316-338), decay
+ * exp(-sigma*dt*C) (:63-65) and split coefficient (1-decay)/(sigma*d), or
+ * C*dt/d where sigma = 0 (:98-111).  Any output may be NULL. */
+int fdtd_pml_profile(int n, double pml_percent, double d, double dt, double* sigma,
+                     double* decay, double* coef);
 
 /* Write side of get_field(Component) (src/FDTD/FDTD.cpp:138-151): copy a
  * full FP64 host array into component `component` (0..8). count must be N. */

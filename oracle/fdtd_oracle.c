@@ -34,8 +34,8 @@
 #include <stdint.h>
 #include <string.h>
 
-#define ORACLE_C 3e10            /* include/Constants.h:9  */
-#define ORACLE_PI 3.14159265358  /* include/Constants.h:14 */
+#define ORACLE_C 3e10            /* include/Constants.h:6  */
+#define ORACLE_PI 3.14159265358  /* include/Constants.h:11 */
 
 /* include/FDTD/FDTD.h:24-31: -1 -> N-1, N -> 0, otherwise unchanged.
  * Written with the same integer arithmetic as the reference. */
@@ -58,15 +58,17 @@ void oracle_coefficients(double dx, double dy, double dz, double dt, double out[
     out[6] = -4.0 * ORACLE_PI * dt;
 }
 
-/* One half-step B update, kokkos_functors.h:128-151 (B += c*dt/(2d) * (-curl E)). */
-void oracle_update_B(int Ni, int Nj, int Nk, const double c[7],
-                     const double* Ex, const double* Ey, const double* Ez,
-                     double* Bx, double* By, double* Bz) {
+/* One half-step B update, kokkos_functors.h:128-151 (B += c*dt/(2d) * (-curl E)),
+ * over the box [i0,i1) x [j0,j1) x [k0,k1) (the reference's main box,
+ * src/FDTD/FDTD.cpp:99-113; the whole grid unless a PML is present). */
+static void update_B_box(int Ni, int Nj, int Nk, const double c[7], const int box[6],
+                         const double* Ex, const double* Ey, const double* Ez,
+                         double* Bx, double* By, double* Bz) {
     const double cBx = c[3], cBy = c[4], cBz = c[5];
 #pragma omp parallel for schedule(static)
-    for (int k = 0; k < Nk; ++k) {
-        for (int j = 0; j < Nj; ++j) {
-            for (int i = 0; i < Ni; ++i) {
+    for (int k = box[4]; k < box[5]; ++k) {
+        for (int j = box[2]; j < box[3]; ++j) {
+            for (int i = box[0]; i < box[1]; ++i) {
                 const int idx = i + j * Ni + k * Ni * Nj;
                 const int ip = oracle_wrap(i + 1, Ni);
                 const int jp = oracle_wrap(j + 1, Nj);
@@ -82,17 +84,18 @@ void oracle_update_B(int Ni, int Nj, int Nk, const double c[7],
     }
 }
 
-/* One full-step E update, kokkos_functors.h:64-90 (E += c*dt/d * curl B - 4*pi*dt*J).
- * J pointers may be NULL, meaning J == 0 (adding cur*0 == -0 is exact). */
-void oracle_update_E(int Ni, int Nj, int Nk, const double c[7],
-                     double* Ex, double* Ey, double* Ez,
-                     const double* Bx, const double* By, const double* Bz,
-                     const double* Jx, const double* Jy, const double* Jz) {
+/* One full-step E update, kokkos_functors.h:64-90 (E += c*dt/d * curl B - 4*pi*dt*J),
+ * over a box as update_B_box.  J pointers may be NULL, meaning J == 0
+ * (adding cur*0 == -0 is exact). */
+static void update_E_box(int Ni, int Nj, int Nk, const double c[7], const int box[6],
+                         double* Ex, double* Ey, double* Ez,
+                         const double* Bx, const double* By, const double* Bz,
+                         const double* Jx, const double* Jy, const double* Jz) {
     const double cEx = c[0], cEy = c[1], cEz = c[2], cur = c[6];
 #pragma omp parallel for schedule(static)
-    for (int k = 0; k < Nk; ++k) {
-        for (int j = 0; j < Nj; ++j) {
-            for (int i = 0; i < Ni; ++i) {
+    for (int k = box[4]; k < box[5]; ++k) {
+        for (int j = box[2]; j < box[3]; ++j) {
+            for (int i = box[0]; i < box[1]; ++i) {
                 const int idx = i + j * Ni + k * Ni * Nj;
                 const int im = oracle_wrap(i - 1, Ni);
                 const int jm = oracle_wrap(j - 1, Nj);
@@ -109,6 +112,21 @@ void oracle_update_E(int Ni, int Nj, int Nk, const double c[7],
             }
         }
     }
+}
+
+void oracle_update_B(int Ni, int Nj, int Nk, const double c[7],
+                     const double* Ex, const double* Ey, const double* Ez,
+                     double* Bx, double* By, double* Bz) {
+    const int box[6] = {0, Ni, 0, Nj, 0, Nk};
+    update_B_box(Ni, Nj, Nk, c, box, Ex, Ey, Ez, Bx, By, Bz);
+}
+
+void oracle_update_E(int Ni, int Nj, int Nk, const double c[7],
+                     double* Ex, double* Ey, double* Ez,
+                     const double* Bx, const double* By, const double* Bz,
+                     const double* Jx, const double* Jy, const double* Jz) {
+    const int box[6] = {0, Ni, 0, Nj, 0, Nk};
+    update_E_box(Ni, Nj, Nk, c, box, Ex, Ey, Ez, Bx, By, Bz, Jx, Jy, Jz);
 }
 
 /* nsteps calls of update_fields(), FDTD_kokkos.cpp:91-104: B/2, E, B/2 with
@@ -132,7 +150,7 @@ void oracle_update_fields(int Ni, int Nj, int Nk, double dx, double dy, double d
  *           envelope = exp(-(t - t0)^2 / (2*sigma^2)) if sigma > 0, else 1.
  * Kind 1 with (period 20dt, sigma 40dt, t0 120dt) is the Gaussian-modulated
  * sinusoid of the 512^3 point-source workload.  The truncated PI of
- * include/Constants.h:14 is used for 2*pi, as the reference's own sources do
+ * include/Constants.h:11 is used for 2*pi, as the reference's own sources do
  * (perf-tests/sample/sample.cpp:27). */
 double oracle_waveform(int kind, double amp, double period, double t0, double sigma,
                        double phase, double t) {
@@ -148,6 +166,22 @@ double oracle_waveform(int kind, double amp, double period, double t0, double si
  * other J is zero.  Sources that share a cell and component add up.
  * src layout per source: {i, j, k, comp(0..2), kind, amp, period, t0, sigma, phase}
  * given as doubles.  J arrays are caller-provided scratch (zeroed here). */
+/* J before step n (t = (n+1)*dt): zero at every source cell, then each
+ * source's waveform added to its component (sample.cpp:66-82). */
+static void set_source_currents(int Ni, int Nj, double* J[3], int nsrc, const double* src,
+                                double t) {
+    for (int q = 0; q < nsrc; ++q) {
+        const double* p = src + 10 * q;
+        const size_t idx = (size_t)p[0] + (size_t)p[1] * Ni + (size_t)p[2] * Ni * Nj;
+        J[(int)p[3]][idx] = 0.0;
+    }
+    for (int q = 0; q < nsrc; ++q) {
+        const double* p = src + 10 * q;
+        const size_t idx = (size_t)p[0] + (size_t)p[1] * Ni + (size_t)p[2] * Ni * Nj;
+        J[(int)p[3]][idx] += oracle_waveform((int)p[4], p[5], p[6], p[7], p[8], p[9], t);
+    }
+}
+
 void oracle_run_point_sources(int Ni, int Nj, int Nk, double dx, double dy, double dz,
                               double dt, long long step0, int nsteps,
                               double* E[3], double* B[3], double* J[3],
@@ -157,20 +191,192 @@ void oracle_run_point_sources(int Ni, int Nj, int Nk, double dx, double dy, doub
     oracle_coefficients(dx, dy, dz, dt, c);
     for (int d = 0; d < 3; ++d) memset(J[d], 0, n * sizeof(double));
     for (int s = 0; s < nsteps; ++s) {
-        const double t = (double)(step0 + s + 1) * dt;
-        for (int q = 0; q < nsrc; ++q) {
-            const double* p = src + 10 * q;
-            const size_t idx = (size_t)p[0] + (size_t)p[1] * Ni + (size_t)p[2] * Ni * Nj;
-            J[(int)p[3]][idx] = 0.0;
-        }
-        for (int q = 0; q < nsrc; ++q) {
-            const double* p = src + 10 * q;
-            const size_t idx = (size_t)p[0] + (size_t)p[1] * Ni + (size_t)p[2] * Ni * Nj;
-            J[(int)p[3]][idx] += oracle_waveform((int)p[4], p[5], p[6], p[7], p[8], p[9], t);
-        }
+        set_source_currents(Ni, Nj, J, nsrc, src, (double)(step0 + s + 1) * dt);
         oracle_update_B(Ni, Nj, Nk, c, E[0], E[1], E[2], B[0], B[1], B[2]);
         oracle_update_E(Ni, Nj, Nk, c, E[0], E[1], E[2], B[0], B[1], B[2], J[0], J[1], J[2]);
         oracle_update_B(Ni, Nj, Nk, c, E[0], E[1], E[2], B[0], B[1], B[2]);
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * PML absorbing layer: src/FDTD/FDTD_PML.cpp (OpenMP) and
+ * src/FDTD_kokkos/FDTD_PML_kokkos.cpp + include/FDTD_kokkos/kokkos_functors.h:154-415.
+ *
+ *   layer width   p = (int)((double)N * pml_percent) per axis    FDTD_PML.cpp:254-256
+ *   main box      [p, N - p) per axis, ordinary updates           :260-269
+ *   conductivity  sigma = SGm * (dist / p)^4,                     :3-61, :293-338
+ *                 SGm = -(4 + 1)/2 * ln(1e-12) / (p * d),         include/Constants.h:7,10
+ *                 dist = p - i (lower band), i + 1 + p - N (upper band);
+ *                 sigma_x depends on i only (its bands span all j, k), likewise
+ *                 sigma_y on j and sigma_z on k; E and B use the same sigma.
+ *   decay         exp(-sigma * dt * C)                            :63-65
+ *   coefficient   (1 - decay) / (sigma * d), or C*dt/d if sigma == 0   :98-111, :167-180
+ *   split fields  E: Exy Exz Eyx Eyz Ezx Ezy, B: Bxy Bxz Byx Byz Bzx Bzy
+ *                 (FDTD_PML.h:12-13), updated as :113-130 and :182-199;
+ *                 totals are the sums of their two splits
+ *   one step      B/2 on the main box; full B split update on the layer;
+ *                 E on the main box (with J); E split update on the layer;
+ *                 B/2 on the main box                              :343-365
+ *   layer slabs   the six boxes of :346-351 in that order; together they
+ *                 cover every cell outside the main box once when 2p <= N.
+ *
+ * Departures, both bugs of the reference that are not reproduced: split
+ * fields and sigma start at zero (the Kokkos constructor leaves them
+ * uninitialized, FDTD_PML_kokkos.cpp:26-69), and the per-cell coefficients
+ * are locals (FDTD_PML.cpp:69 shares them between OpenMP threads).
+ */
+#define ORACLE_R 1e-12   /* include/Constants.h:7  */
+#define ORACLE_NPOW 4.0  /* include/Constants.h:10 */
+
+/* Layer widths out[3] = {p_i, p_j, p_k}. */
+void oracle_pml_sizes(int Ni, int Nj, int Nk, double pml_percent, int out[3]) {
+    out[0] = (int)((double)Ni * pml_percent);
+    out[1] = (int)((double)Nj * pml_percent);
+    out[2] = (int)((double)Nk * pml_percent);
+}
+
+/* Conductivity profile of one axis of n cells with layer width p and
+ * spacing d: lower band set first, then the upper band (FDTD_PML.cpp:326-338). */
+void oracle_pml_sigma(int n, int p, double d, double* sigma) {
+    for (int i = 0; i < n; ++i) sigma[i] = 0.0;
+    if (p <= 0) return;
+    const double sgm = -(ORACLE_NPOW + 1.0) / 2.0 * log(ORACLE_R) / ((double)p * d);
+    for (int i = 0; i < p; ++i)
+        sigma[i] = sgm * pow((double)(p - i) / (double)p, ORACLE_NPOW);
+    for (int i = n - p; i < n; ++i)
+        sigma[i] = sgm * pow((double)(i + 1 + p - n) / (double)p, ORACLE_NPOW);
+}
+
+static inline double pml_decay(double sigma, double dt) {
+    return exp(-sigma * dt * ORACLE_C);
+}
+
+static inline double pml_coef(double sigma, double dt, double d, double c_full) {
+    return sigma != 0.0 ? (1.0 - pml_decay(sigma, dt)) / (sigma * d) : c_full;
+}
+
+typedef struct {
+    int Ni, Nj, Nk;
+    double d[3], dt, cfull[3];
+    const double* sigma[3];
+} pml_ctx;
+
+/* Full-step B split update on a box, FDTD_PML.cpp:136-203. */
+static void pml_update_B_box(const pml_ctx* x, const int box[6], const double* E[3],
+                             double* Bs[6], double* B[3]) {
+    const int Ni = x->Ni, Nj = x->Nj, Nk = x->Nk;
+#pragma omp parallel for schedule(static)
+    for (int k = box[4]; k < box[5]; ++k) {
+        for (int j = box[2]; j < box[3]; ++j) {
+            for (int i = box[0]; i < box[1]; ++i) {
+                const int idx = i + j * Ni + k * Ni * Nj;
+                const int i_next = oracle_wrap(i + 1, Ni) + j * Ni + k * Ni * Nj;
+                const int j_next = i + oracle_wrap(j + 1, Nj) * Ni + k * Ni * Nj;
+                const int k_next = i + j * Ni + oracle_wrap(k + 1, Nk) * Ni * Nj;
+                const double sx = x->sigma[0][i], sy = x->sigma[1][j], sz = x->sigma[2][k];
+                const double cx = pml_coef(sx, x->dt, x->d[0], x->cfull[0]);
+                const double cy = pml_coef(sy, x->dt, x->d[1], x->cfull[1]);
+                const double cz = pml_coef(sz, x->dt, x->d[2], x->cfull[2]);
+                const double ax = pml_decay(sx, x->dt), ay = pml_decay(sy, x->dt),
+                             az = pml_decay(sz, x->dt);
+                double* Bxy = Bs[0]; double* Bxz = Bs[1]; double* Byx = Bs[2];
+                double* Byz = Bs[3]; double* Bzx = Bs[4]; double* Bzy = Bs[5];
+                Byx[idx] = Byx[idx] * ax + cx * (E[2][i_next] - E[2][idx]);
+                Bzx[idx] = Bzx[idx] * ax - cx * (E[1][i_next] - E[1][idx]);
+                Bxy[idx] = Bxy[idx] * ay - cy * (E[2][j_next] - E[2][idx]);
+                Bzy[idx] = Bzy[idx] * ay + cy * (E[0][j_next] - E[0][idx]);
+                Bxz[idx] = Bxz[idx] * az + cz * (E[1][k_next] - E[1][idx]);
+                Byz[idx] = Byz[idx] * az - cz * (E[0][k_next] - E[0][idx]);
+                B[0][idx] = Bxy[idx] + Bxz[idx];
+                B[1][idx] = Byz[idx] + Byx[idx];
+                B[2][idx] = Bzx[idx] + Bzy[idx];
+            }
+        }
+    }
+}
+
+/* Full-step E split update on a box, FDTD_PML.cpp:67-134 (no J term). */
+static void pml_update_E_box(const pml_ctx* x, const int box[6], const double* B[3],
+                             double* Es[6], double* E[3]) {
+    const int Ni = x->Ni, Nj = x->Nj, Nk = x->Nk;
+#pragma omp parallel for schedule(static)
+    for (int k = box[4]; k < box[5]; ++k) {
+        for (int j = box[2]; j < box[3]; ++j) {
+            for (int i = box[0]; i < box[1]; ++i) {
+                const int idx = i + j * Ni + k * Ni * Nj;
+                const int i_pred = oracle_wrap(i - 1, Ni) + j * Ni + k * Ni * Nj;
+                const int j_pred = i + oracle_wrap(j - 1, Nj) * Ni + k * Ni * Nj;
+                const int k_pred = i + j * Ni + oracle_wrap(k - 1, Nk) * Ni * Nj;
+                const double sx = x->sigma[0][i], sy = x->sigma[1][j], sz = x->sigma[2][k];
+                const double cx = pml_coef(sx, x->dt, x->d[0], x->cfull[0]);
+                const double cy = pml_coef(sy, x->dt, x->d[1], x->cfull[1]);
+                const double cz = pml_coef(sz, x->dt, x->d[2], x->cfull[2]);
+                const double ax = pml_decay(sx, x->dt), ay = pml_decay(sy, x->dt),
+                             az = pml_decay(sz, x->dt);
+                double* Exy = Es[0]; double* Exz = Es[1]; double* Eyx = Es[2];
+                double* Eyz = Es[3]; double* Ezx = Es[4]; double* Ezy = Es[5];
+                Eyx[idx] = Eyx[idx] * ax - cx * (B[2][idx] - B[2][i_pred]);
+                Ezx[idx] = Ezx[idx] * ax + cx * (B[1][idx] - B[1][i_pred]);
+                Exy[idx] = Exy[idx] * ay + cy * (B[2][idx] - B[2][j_pred]);
+                Ezy[idx] = Ezy[idx] * ay - cy * (B[0][idx] - B[0][j_pred]);
+                Exz[idx] = Exz[idx] * az - cz * (B[1][idx] - B[1][k_pred]);
+                Eyz[idx] = Eyz[idx] * az + cz * (B[0][idx] - B[0][k_pred]);
+                E[0][idx] = Exz[idx] + Exy[idx];
+                E[1][idx] = Eyx[idx] + Eyz[idx];
+                E[2][idx] = Ezy[idx] + Ezx[idx];
+            }
+        }
+    }
+}
+
+/* The six layer slabs of FDTD_PML.cpp:346-351 as {i0,i1,j0,j1,k0,k1}. */
+static void pml_slabs(int Ni, int Nj, int Nk, const int p[3], int slab[6][6]) {
+    const int pi = p[0], pj = p[1], pk = p[2];
+    const int s[6][6] = {
+        {0, Ni - pi, 0, Nj, 0, pk},              /* part_from_start, solid, lower k */
+        {pi, Ni - pi, 0, pj, pk, Nk - pk},       /* main, lower j, main              */
+        {0, pi, 0, Nj, pk, Nk},                  /* lower i, solid, part_from_end    */
+        {pi, Ni, 0, Nj, Nk - pk, Nk},            /* part_from_end, solid, upper k    */
+        {pi, Ni - pi, Nj - pj, Nj, pk, Nk - pk}, /* main, upper j, main              */
+        {Ni - pi, Ni, 0, Nj, 0, Nk - pk},        /* upper i, solid, part_from_start  */
+    };
+    memcpy(slab, s, sizeof(s));
+}
+
+/* nsteps of FDTD_PML::update_fields() (FDTD_PML.cpp:343-365).  Es/Bs are
+ * the twelve split fields (zero for a new grid).  With nsrc > 0 the J
+ * arrays are scratch, set before every step from the point sources as in
+ * oracle_run_point_sources; otherwise J is constant (NULL entries = 0). */
+void oracle_pml_run(int Ni, int Nj, int Nk, double dx, double dy, double dz, double dt,
+                    double pml_percent, long long step0, int nsteps, double* E[3],
+                    double* B[3], double* Es[6], double* Bs[6], double* J[3], int nsrc,
+                    const double* src, double* sigma_scratch) {
+    double c[7];
+    int p[3], slab[6][6];
+    oracle_coefficients(dx, dy, dz, dt, c);
+    oracle_pml_sizes(Ni, Nj, Nk, pml_percent, p);
+    pml_slabs(Ni, Nj, Nk, p, slab);
+    double* sx = sigma_scratch;
+    double* sy = sx + Ni;
+    double* sz = sy + Nj;
+    oracle_pml_sigma(Ni, p[0], dx, sx);
+    oracle_pml_sigma(Nj, p[1], dy, sy);
+    oracle_pml_sigma(Nk, p[2], dz, sz);
+    const pml_ctx x = {Ni, Nj, Nk, {dx, dy, dz}, dt, {c[0], c[1], c[2]}, {sx, sy, sz}};
+    const int main_box[6] = {p[0], Ni - p[0], p[1], Nj - p[1], p[2], Nk - p[2]};
+    const size_t n = (size_t)Ni * Nj * Nk;
+    if (nsrc > 0)
+        for (int d = 0; d < 3; ++d) memset(J[d], 0, n * sizeof(double));
+    const double* cE[3] = {E[0], E[1], E[2]};
+    const double* cB[3] = {B[0], B[1], B[2]};
+    for (int s = 0; s < nsteps; ++s) {
+        if (nsrc > 0) set_source_currents(Ni, Nj, J, nsrc, src, (double)(step0 + s + 1) * dt);
+        update_B_box(Ni, Nj, Nk, c, main_box, E[0], E[1], E[2], B[0], B[1], B[2]);
+        for (int q = 0; q < 6; ++q) pml_update_B_box(&x, slab[q], cE, Bs, B);
+        update_E_box(Ni, Nj, Nk, c, main_box, E[0], E[1], E[2], B[0], B[1], B[2],
+                     J ? J[0] : NULL, J ? J[1] : NULL, J ? J[2] : NULL);
+        for (int q = 0; q < 6; ++q) pml_update_E_box(&x, slab[q], cB, Es, E);
+        update_B_box(Ni, Nj, Nk, c, main_box, E[0], E[1], E[2], B[0], B[1], B[2]);
     }
 }
 

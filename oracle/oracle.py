@@ -27,8 +27,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libfdtd_ref.so")
 
-C = 3e10              # include/Constants.h:9
-PI = 3.14159265358    # include/Constants.h:14 (truncated)
+C = 3e10              # include/Constants.h:6
+PI = 3.14159265358    # include/Constants.h:11 (truncated)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
@@ -59,6 +59,12 @@ def lib():
             + [ctypes.POINTER(_dp)] * 3 + [ctypes.c_int, _dp])
         L.oracle_sum_squares.argtypes = [_dp, ctypes.c_size_t]
         L.oracle_sum_squares.restype = ctypes.c_double
+        L.oracle_pml_sizes.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double,
+                                                            ctypes.POINTER(ctypes.c_int)]
+        L.oracle_pml_sigma.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp]
+        L.oracle_pml_run.argtypes = (
+            [ctypes.c_int] * 3 + [ctypes.c_double] * 5 + [ctypes.c_longlong, ctypes.c_int]
+            + [ctypes.POINTER(_dp)] * 5 + [ctypes.c_int, _dp, _dp])
         _lib = L
     return _lib
 
@@ -83,6 +89,10 @@ def ref_lib():
                                     ctypes.c_double]
         L.ref_update_fields.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ref_zeroed_currents.argtypes = [ctypes.c_void_p]
+        L.ref_pml_create.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 11
+        L.ref_pml_create.restype = ctypes.c_void_p
+        L.ref_pml_destroy.argtypes = [ctypes.c_void_p]
+        L.ref_pml_update_fields.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _ref = L
     return _ref
 
@@ -145,6 +155,58 @@ class OracleGrid:
         return [lib().oracle_sum_squares(_ptr(a), a.size) for a in self.E + self.B]
 
 
+def pml_sizes(Ni, Nj, Nk, pml_percent):
+    """Layer widths (p_i, p_j, p_k), src/FDTD/FDTD_PML.cpp:254-256."""
+    out = (ctypes.c_int * 3)()
+    lib().oracle_pml_sizes(Ni, Nj, Nk, pml_percent, out)
+    return tuple(out)
+
+
+def pml_sigma(n, p, d):
+    """Conductivity profile of one axis, src/FDTD/FDTD_PML.cpp:3-61,316-338."""
+    out = np.zeros(n)
+    lib().oracle_pml_sigma(n, p, d, _ptr(out))
+    return out
+
+
+class OraclePmlGrid(OracleGrid):
+    """A CPU grid with the reference's PML layer (FDTD_PML, src/FDTD/FDTD_PML.cpp).
+
+    ES = [Exy, Exz, Eyx, Eyz, Ezx, Ezy] and BS = [Bxy, Bxz, Byx, Byz, Bzx, Bzy]
+    are the split fields (include/FDTD/FDTD_PML.h:12-13)."""
+
+    def __init__(self, Ni, Nj, Nk, dx, dy, dz, dt, pml_percent):
+        super().__init__(Ni, Nj, Nk, dx, dy, dz, dt)
+        self.pml_percent = pml_percent
+        n = Ni * Nj * Nk
+        self.ES = [np.zeros(n) for _ in range(6)]
+        self.BS = [np.zeros(n) for _ in range(6)]
+        self._sigma = np.zeros(Ni + Nj + Nk)
+
+    def _run(self, nsteps, sources):
+        E = (_dp * 3)(*[_ptr(a) for a in self.E])
+        B = (_dp * 3)(*[_ptr(a) for a in self.B])
+        ES = (_dp * 6)(*[_ptr(a) for a in self.ES])
+        BS = (_dp * 6)(*[_ptr(a) for a in self.BS])
+        J = (_dp * 3)(*[_ptr(a) for a in self.J])
+        src = source_array(sources) if sources else np.zeros(10)
+        lib().oracle_pml_run(self.Ni, self.Nj, self.Nk, self.dx, self.dy, self.dz, self.dt,
+                             self.pml_percent, self.step, nsteps, E, B, ES, BS, J,
+                             len(sources) if sources else 0, _ptr(src), _ptr(self._sigma))
+        self.step += nsteps
+
+    def update_fields(self, nsteps=1):
+        self._run(nsteps, None)
+
+    def run_point_sources(self, sources, nsteps):
+        self._run(nsteps, sources)
+
+    def update_B(self):
+        raise NotImplementedError("the PML grid steps only as a whole")
+
+    update_E = update_B
+
+
 def source_array(sources):
     """Pack sources {i,j,k,comp,kind,amp,period,t0,sigma,phase} into the C layout."""
     rows = []
@@ -196,6 +258,28 @@ class RefGrid:
 
     def zeroed_currents(self):
         ref_lib().ref_zeroed_currents(self.h)
+
+
+class RefPmlGrid(RefGrid):
+    """The reference's own FDTD_openmp::FDTD_PML (src/FDTD/FDTD_PML.cpp), run
+    on one OpenMP thread (its layer updates share per-cell coefficients
+    between threads)."""
+
+    def __init__(self, Ni, Nj, Nk, dx, dy, dz, dt, pml_percent, ax=0.0, bx=1.0, ay=0.0,
+                 by=1.0, az=0.0, bz=1.0):
+        self.n = Ni * Nj * Nk
+        self.h = ref_lib().ref_pml_create(Ni, Nj, Nk, ax, bx, ay, by, az, bz, dx, dy, dz, dt,
+                                          pml_percent)
+        if not self.h:
+            raise ValueError("ERROR: invalid parameters")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref_lib().ref_pml_destroy(self.h)
+            self.h = None
+
+    def update_fields(self, nsteps=1):
+        ref_lib().ref_pml_update_fields(self.h, nsteps)
 
 
 # ---------------------------------------------------------------------------
@@ -277,14 +361,18 @@ def sample_box(n):
                 stop=stop, boundary=boundary)
 
 
-def spherical_wave(n, it):
+def spherical_wave(n, it, pml_percent=None):
     """Run the harness on the oracle; returns [Ex..Bz] after `it` steps.
 
     J is written to Jx = Jy = Jz on the box before each of the first
     min(40, it) steps, then zeroed -- the pattern for which the reference's
-    OpenMP and Kokkos E updates coincide."""
+    OpenMP and Kokkos E updates coincide.  With pml_percent the grid is the
+    driver's FDTD_PML (sample.cpp:93-123)."""
     p = sample_box(n)
-    g = OracleGrid(n, n, n, p["d"], p["d"], p["d"], p["dt"])
+    if pml_percent is None:
+        g = OracleGrid(n, n, n, p["d"], p["d"], p["d"], p["dt"])
+    else:
+        g = OraclePmlGrid(n, n, n, p["d"], p["d"], p["d"], p["dt"], pml_percent)
     cur_time = min(p["iterations"], it)
     for t in range(cur_time):
         for k in range(p["start"], p["stop"]):

@@ -10,12 +10,23 @@
 //                                         (component ordinals of include/Enums.h:5)
 //   virtual void update_fields()       -> ref_update_fields (called nsteps times)
 //   void zeroed_currents()             -> ref_zeroed_currents
+// and the PML subclass, include/FDTD/FDTD_PML.h:10-44 (src/FDTD/FDTD_PML.cpp):
+//   FDTD_PML(Parameters, FP dt, FP pml_percent) -> ref_pml_create
+//   update_fields() override                    -> ref_pml_update_fields
+// FDTD_PML::update_E_PML/update_B_PML keep their per-cell coefficients in
+// variables shared by the OpenMP team (src/FDTD/FDTD_PML.cpp:69,138), so
+// ref_pml_update_fields runs the class on one thread to get the values the
+// code means.
+#include <omp.h>
+
 #include <cstring>
 #include <exception>
 
 #include "FDTD.h"
+#include "FDTD_PML.h"
 
 using FDTD_openmp::FDTD;
+using FDTD_openmp::FDTD_PML;
 
 extern "C" {
 
@@ -69,5 +80,29 @@ void ref_update_fields(void* h, int nsteps) {
 }
 
 void ref_zeroed_currents(void* h) { static_cast<FDTD*>(h)->zeroed_currents(); }
+
+// A PML grid; the other ref_* calls take it as an FDTD (get_field and
+// zeroed_currents are inherited), except destruction and stepping.
+void* ref_pml_create(int Ni, int Nj, int Nk, double ax, double bx, double ay, double by,
+                     double az, double bz, double dx, double dy, double dz, double dt,
+                     double pml_percent) {
+    FDTD_struct::Parameters p{Ni, Nj, Nk, ax, bx, ay, by, az, bz, dx, dy, dz};
+    try {
+        FDTD_PML* g = new FDTD_PML(p, dt, pml_percent);
+        return static_cast<FDTD*>(g);
+    } catch (const std::exception&) {
+        return nullptr;
+    }
+}
+
+void ref_pml_destroy(void* h) { delete static_cast<FDTD_PML*>(static_cast<FDTD*>(h)); }
+
+void ref_pml_update_fields(void* h, int nsteps) {
+    const int threads = omp_get_max_threads();
+    omp_set_num_threads(1);
+    FDTD* g = static_cast<FDTD*>(h);
+    for (int s = 0; s < nsteps; ++s) g->update_fields();
+    omp_set_num_threads(threads);
+}
 
 }  // extern "C"

@@ -209,6 +209,49 @@ void test_multi_step_equivalence() {
     EXPECT_TRUE(same);
 }
 
+void test_pml_class() {
+    // FDTD_PML (src/FDTD/FDTD_PML.cpp:205-365): layer widths, J ignored in
+    // the layer, a pulse absorbed, pml_percent validated.
+    Parameters p{24, 20, 16, 0, 1, 0, 1, 0, 1, 1.0, 1.0, 1.0};
+    const FP dt = 0.5 / (FDTD_const::C * std::sqrt(3.0));
+    FDTD_PML m(p, dt, 0.25, g_precision);
+    const auto w = m.pml_widths();
+    EXPECT_TRUE(w[0] == 6 && w[1] == 5 && w[2] == 4);
+    m.get_field(Component::JZ)[0] = 1.0;  // a layer cell: no effect
+    m.update_fields(3);
+    bool zero = true;
+    for (int c = 0; c < 6; ++c)
+        for (int i = 0; i < (int)m.cells(); ++i)
+            zero = zero && (double)m.get_field(static_cast<Component>(c))[i] == 0.0;
+    EXPECT_TRUE(zero);
+    m.zeroed_currents();
+    const int centre = 12 + 10 * 24 + 8 * 24 * 20;
+    for (int t = 0; t < 20; ++t) {
+        m.get_field(Component::JZ)[centre] = std::sin(2.0 * FDTD_const::PI * (t + 1) / 20.0);
+        m.update_fields();
+    }
+    m.zeroed_currents();
+    auto energy = [&]() {
+        double e = 0;
+        for (int c = 0; c < 6; ++c)
+            for (int i = 0; i < (int)m.cells(); ++i) {
+                const double v = m.get_field(static_cast<Component>(c))[i];
+                e += v * v;
+            }
+        return e;
+    };
+    const double e0 = energy();
+    m.update_fields(200);
+    EXPECT_TRUE(e0 > 0 && energy() < 1e-2 * e0);
+    bool thrown = false;
+    try {
+        FDTD_PML bad(p, dt, 0.7, g_precision);
+    } catch (const std::runtime_error&) {
+        thrown = true;
+    }
+    EXPECT_TRUE(thrown);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -226,7 +269,8 @@ int main(int argc, char** argv) {
     test_errors();
     test_currents();
     test_multi_step_equivalence();
-    ran += 3;
+    test_pml_class();
+    ran += 4;
     std::printf("%d tests, %d failures\n", ran, g_failures);
     return g_failures == 0 ? 0 : 1;
 }

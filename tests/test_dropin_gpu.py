@@ -47,7 +47,7 @@ def test_cpp_unit_tests(prec):
     res = subprocess.run([exe, prec], capture_output=True, text=True, timeout=300)
     print(res.stdout[-3000:])
     assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-2000:]
-    assert "15 tests, 0 failures" in res.stdout
+    assert "16 tests, 0 failures" in res.stdout
 
 
 def test_cpp_sample_driver_matches_reference(tmp_path, golden_dir):
@@ -65,3 +65,26 @@ def test_cpp_sample_driver_matches_reference(tmp_path, golden_dir):
     np.testing.assert_allclose(printed, rec["ex_slice"], atol=5e-6)
     bad = subprocess.run([exe, "32"], capture_output=True, text=True, timeout=60)
     assert bad.returncode == 1 and "ERROR: Incorrect number of parameters" in bad.stdout
+
+
+def test_cpp_sample_driver_pml_block_matches_reference(tmp_path, golden_dir):
+    """`sample 32 100 --pml`: the reference driver's __PML_TEST__ block
+    (perf-tests/sample/sample.cpp:93-123,136-147), FDTD_PML with 0.2."""
+    rec = json.load(open(os.path.join(golden_dir, "ref_pml_sample_32_100.json")))
+    plain = json.load(open(os.path.join(golden_dir, "ref_sample_32_100.json")))
+    exe = os.path.join(ROOT, "fdtd_method_amd", "bin", "sample")
+    dump, dump_pml = tmp_path / "ex.bin", tmp_path / "ex_pml.bin"
+    res = subprocess.run([exe, str(rec["n"]), str(rec["iterations"]), "--pml",
+                          "--dump", str(dump), "--dump-pml", str(dump_pml)],
+                         capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    lines = res.stdout.splitlines()
+    assert lines[0].startswith("Execution time: ")
+    assert lines[1].startswith("Execution time (PML): ")
+    ex = np.fromfile(dump, dtype="<f8")
+    assert hashlib.sha256(ex.tobytes()).hexdigest() == plain["sha256"][0]
+    ex = np.fromfile(dump_pml, dtype="<f8")
+    assert hashlib.sha256(ex.tobytes()).hexdigest() == rec["sha256"][0]
+    k = lines.index("PML: ")
+    printed = [[float(x) for x in ln.split()] for ln in lines[k + 2:k + 12]]
+    np.testing.assert_allclose(printed, rec["ex_slice"], atol=5e-6)

@@ -59,3 +59,42 @@ def test_cpp_programs_are_built():
     for path in (os.path.join(ROOT, "fdtd_method_amd", "bin", "sample"),
                  os.path.join(ROOT, "tests", "cpp", "test_fdtd_mi355x")):
         assert os.access(path, os.X_OK), f"{path} missing: run __graft_entry__.build()"
+
+
+def test_pml_invalid_arguments_without_gpu():
+    """fdtd_create_pml validates before touching the device: the reference's
+    invalid_argument for bad dimensions, and pml_percent outside [0, 0.5]
+    (where the reference's layer slabs would overlap)."""
+    from fdtd_method_amd import FDTD_PML
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    p = _lib.Parameters(0, 4, 4, 0, 1, 0, 1, 0, 1, 1, 1, 1)
+    assert L.fdtd_create_pml(ctypes.byref(p), 1e-12, 0.2, 0, ctypes.byref(h)) == \
+        _lib.ERR_INVALID_PARAMETERS
+    p = _lib.Parameters(8, 8, 8, 0, 1, 0, 1, 0, 1, 1, 1, 1)
+    for pct in (-0.1, 0.51, float("nan")):
+        assert L.fdtd_create_pml(ctypes.byref(p), 1e-12, pct, 0, ctypes.byref(h)) == \
+            _lib.ERR_INVALID_ARGUMENT
+    with pytest.raises(ValueError, match="ERROR: invalid parameters"):
+        FDTD_PML(Parameters(4, 4, 0), 1e-12, 0.2)
+
+
+@pytest.mark.parametrize("n,pct,d", [(512, 0.2, 1.0), (37, 0.3, 1.7), (10, 0.5, 0.4),
+                                     (9, 0.05, 1.0), (64, 0.2, 3e10)])
+def test_pml_profile_matches_the_oracle(n, pct, d):
+    """The library's layer tables start from this host-side profile
+    (fdtd_pml_profile, no GPU): bit-identical to the oracle's sigma, and the
+    decay/coefficient follow FDTD_PML.cpp:63-65,98-111."""
+    import math
+    from fdtd_method_amd import pml_profile
+    from oracle import oracle as O
+    dt = 0.5 * d / (O.C * math.sqrt(3.0))
+    sigma, decay, coef = pml_profile(n, pct, d, dt)
+    p = O.pml_sizes(n, n, n, pct)[0]
+    ref = O.pml_sigma(n, p, d)
+    assert (sigma == ref).all()
+    for i in range(n):
+        a = math.exp(-ref[i] * dt * O.C)
+        assert decay[i] == a
+        c = (1.0 - math.exp(-ref[i] * dt * O.C)) / (ref[i] * d) if ref[i] != 0.0 else O.C * dt / d
+        assert coef[i] == c
