@@ -35,6 +35,11 @@ python bench.py --workload blocked [--dims 1024,1024,256] [--steps 4000]
     FP32 with 4 seeded Jz sources of distinct periods: the full-row sweep
     against the autotuned best of tiled/temporally blocked passes
     (L steps per pass over HBM); effective roofline, error vs FP64
+python bench.py --workload pml [--n 512] [--pml 0.2] [--steps 1000]
+    the centre-source workload on the PML class (absorbing layer of
+    int(N*pml) cells per side, as the reference driver's PML block): FP32 and
+    FP64 fused, roofline over 12 words per main-box cell and 36 per layer
+    cell, error vs FP64, CPU baseline of the reference PML class (256^3 sample)
 python bench.py --workload precision [--n 768] [--steps 2000]
     768^3, superposition of 8 seeded plane waves plus the centre Jz source:
     FP64, FP32 and FP16 (per-component power-of-two scales) runs, per
@@ -125,7 +130,7 @@ def energy(g):
     return sum(s) / (8.0 * np.pi)
 
 
-def cpu_baseline(n):
+def cpu_baseline(n, pml=None):
     try:
         threads = min(16, len(os.sched_getaffinity(0)))
     except AttributeError:
@@ -133,6 +138,8 @@ def cpu_baseline(n):
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                OMP_PLACES="cores")
     cmd = [sys.executable, "-m", "oracle.cpu_bench", "--n", str(n), "--seconds", "12"]
+    if pml is not None:
+        cmd += ["--pml", str(pml)]
     log("cpu baseline:", " ".join(cmd), f"(OMP_NUM_THREADS={threads})")
     res = subprocess.run(cmd, capture_output=True, text=True, cwd=ROOT, env=env, timeout=600)
     if res.returncode != 0:
@@ -312,6 +319,74 @@ def bench_blocked(a):
     print(json.dumps(out), flush=True)
 
 
+def pml_bytes_per_step(shape, widths, word):
+    """Algorithmic HBM bytes of one PML step: main-box cells read and write
+    E and B (12 words); layer cells also read and write their 12 split
+    fields (36 words)."""
+    Ni, Nj, Nk = shape
+    main = 1
+    for n, p in zip(shape, widths):
+        main *= max(0, n - 2 * p)
+    layer = Ni * Nj * Nk - main
+    return word * (12 * main + 36 * layer), main, layer
+
+
+def bench_pml(a):
+    from fdtd_method_amd import FDTD_PML
+    n, K, pml = a.n, a.steps, a.pml
+    dt = 0.5 / (C * np.sqrt(3.0))
+    src = dict(i=n // 2, j=n // 2, k=n // 2, comp=2, kind=1, amp=1.0, period=20 * dt,
+               t0=120 * dt, sigma=40 * dt, phase=0.0)
+    cells = float(n) ** 3
+
+    def make(prec):
+        g = FDTD_PML(Parameters(n, n, n), dt, pml, prec)
+        g.set_point_sources([src])
+        shape, ms = g.autotune(reps=3)
+        log(f"pml autotune {prec.name}: V={shape[0]} TY={shape[1]} kchunk={shape[2]} "
+            f"({ms:.3f} ms/sweep)")
+        return g
+
+    res = {}
+    grids = {}
+    for prec in (Precision.FP32, Precision.FP64):
+        g = grids[prec] = make(prec)
+        r = run_path(g, K, a.warmup)
+        nbytes, main, layer = pml_bytes_per_step((n, n, n), g.pml_widths, WORD[prec])
+        gbs = nbytes / (r["launch_ms"] * 1e-3) / 1e9
+        res[prec] = dict(gcells=r["gcells"], ms_per_step=r["ms_per_step"],
+                         launch_ms=r["launch_ms"], alg_bytes_per_step=nbytes, alg_gbs=gbs,
+                         frac_of_peak=gbs / PEAK_GBS, launch_shape=g.get_tuning())
+        log(f"pml {prec.name}: {r['gcells']:.1f} Gcell/s, launch {r['launch_ms']:.3f} ms, "
+            f"{gbs:.0f} GB/s algorithmic")
+    err = errors(grids[Precision.FP32], grids[Precision.FP64])
+    widths = grids[Precision.FP32].pml_widths
+    en = energy(grids[Precision.FP64])
+    nbytes, main, layer = pml_bytes_per_step((n, n, n), widths, 4)
+    r32 = res[Precision.FP32]
+    cpu = None if a.no_cpu_baseline else cpu_baseline(min(n, 256), pml)
+    out = {
+        "metric": "Gcell-updates/s and %HBM roofline of the PML (absorbing layer) step",
+        "value": r32["gcells"], "unit": "Gcell-updates/s", "n_gpus": 1, "steps": K,
+        "warmup": a.warmup, "ms_per_step": r32["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": (f"{n}^3 grid with a PML layer (pml_percent {pml}: widths "
+                                f"{list(widths)}, {layer / cells:.1%} of the cells), dx=1, "
+                                "dt=0.5dx/(c*sqrt3), Gaussian-modulated Jz point source at the "
+                                f"centre, zero initial fields, {K} steps"),
+                   "main_box_cells": main, "layer_cells": layer},
+        "roofline": {"bound": "hbm", "achieved": r32["alg_gbs"], "peak": PEAK_GBS, "unit": "GB/s",
+                     "frac": r32["alg_gbs"] / PEAK_GBS, "traffic": None,
+                     "kernel": "fdtd::fused_pml_sweep<float,float,float,V,true>",
+                     "alg_bytes_per_launch": nbytes, "launch_ms": r32["launch_ms"]},
+        "cpu_baseline": cpu,
+        "paths": {"fp32_fused": res[Precision.FP32], "fp64_fused": res[Precision.FP64]},
+        "errors_vs_fp64": {"fp32": err},
+        "energy_fp64": en,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -323,8 +398,9 @@ def main():
     ap.add_argument("--profile", choices=["fp32", "fp64", "fp16"], default=None)
     ap.add_argument("--tuning", default=None,
                     help="V,TY,kchunk launch shape for --profile (default: autotune)")
-    ap.add_argument("--workload", choices=["point", "batch", "precision", "blocked"],
+    ap.add_argument("--workload", choices=["point", "batch", "precision", "blocked", "pml"],
                     default="point")
+    ap.add_argument("--pml", type=float, default=0.2, help="pml_percent of --workload pml")
     ap.add_argument("--dims", default="1024,1024,256", help="grid of --workload blocked")
     ap.add_argument("--grids", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
@@ -340,6 +416,8 @@ def main():
         return bench_precision(a)
     if a.workload == "blocked":
         return bench_blocked(a)
+    if a.workload == "pml":
+        return bench_pml(a)
 
     if a.profile:
         prec = {"fp32": Precision.FP32, "fp64": Precision.FP64, "fp16": Precision.FP16}[a.profile]

@@ -379,6 +379,11 @@ __host__ __device__ inline size_t fused_lds_bytes(int TY, int Ni, int csize) {
     return static_cast<size_t>(6 * TY + 4) * Ni * csize;
 }
 
+// The PML sweep also stages the five x-axis layer tables (5 rows of Ni).
+__host__ __device__ inline size_t fused_pml_lds_bytes(int TY, int Ni, int csize) {
+    return fused_lds_bytes(TY, Ni, csize) + static_cast<size_t>(5) * Ni * csize;
+}
+
 // Largest workgroup of the fused sweep for V cells of size csize per lane:
 // wide lanes hold more registers, so their workgroups are capped lower.
 __host__ __device__ constexpr int fused_max_threads(int V, int csize) {
@@ -399,6 +404,23 @@ struct SweepArgs {
     Pml<S, C> pml;  // used by PML sweeps only
 };
 
+// Lane -> pack (V cells) order within a row of a PML sweep: the packs that
+// hold lower-layer cells (i < pi), then those holding upper-layer cells
+// (i >= Ni - pi), then the rest, so that in main-box rows the layer cells
+// share as few wavefronts as possible and the other wavefronts never take
+// the layer branch.  A bijection of [0, npacks); the identity when the two
+// layer groups cover the row.  Neighbours across packs come from LDS rows,
+// so any lane order gives the same results.
+__device__ __forceinline__ int pml_pack(int lane, int npacks, int pi, int V) {
+    const int nlo = (pi + V - 1) / V;
+    const int first_hi = (npacks * V - pi) / V;
+    const int nhi = npacks - first_hi;
+    if (nlo + nhi >= npacks) return lane;
+    if (lane < nlo) return lane;
+    if (lane < nlo + nhi) return first_hi + (lane - nlo);
+    return lane - nhi;
+}
+
 template <typename S, typename C, typename JT, int V, bool STEADY, bool PML = false>
 __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int tile_y, int chunk,
                                            C* __restrict__ lds) {
@@ -410,7 +432,7 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
     const int k1 = min(k0 + a.g.kchunk, Nk);
     const int tid = threadIdx.x;
     const int r = tid / lanes;
-    const int ib = (tid - r * lanes) * V;
+    const int ib = (PML ? pml_pack(tid - r * lanes, lanes, a.pml.pi, V) : tid - r * lanes) * V;
     const bool active = r <= ty;
     const bool own = r < ty;
     const bool halo = r == ty;
@@ -430,6 +452,16 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
     C* const sEz = sEy + TY * Ni;
 
     const Coefs<C> c = a.c;
+
+    // PML sweeps: the x tables in LDS after the field rows, and this row's y
+    // table values in registers (both constant over the z march).
+    C* const sTx = sEz + (TY + 1) * Ni;
+    C tyv[5];
+    if constexpr (PML) {
+        for (int q = tid; q < 5 * Ni; q += blockDim.x) sTx[q] = a.pml.tx[q];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) tyv[q] = active ? a.pml.ty[q * Nj + j] : C(0);
+    }
 
     C eP[3][V];  // E_{n+1} at plane kk-1 (own cells)
     C bP[3][V];  // B at plane kk-1 (own cells)
@@ -486,6 +518,35 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
 
         C en2[3][V], bn2[3][V], bm2[V];
         if (active && kk < k1) load_plane((kk + 1 == Nk) ? 0 : kk + 1, en2, bn2, bm2);
+        // PML sweeps: the split fields this iteration updates (E splits of
+        // plane kw, B splits of plane kk-1), for lanes holding layer cells,
+        // are loaded here, before the E update, so that their latency
+        // overlaps it instead of stalling the workgroup at the next barrier.
+        C hs[6][V];
+        C hb[6][V];
+        if constexpr (PML) {
+            const Pml<S, C>& P = a.pml;
+            const bool rowm = in_main(kw, Nk, P.pk) && in_main(j, Nj, P.pj);
+            if (active && !(rowm && ib >= P.pi && ib + V <= Ni - P.pi)) {
+                load_pack<S, C, V>(P.es_in[kExy] + off, hs[kExy]);
+                load_pack<S, C, V>(P.es_in[kExz] + off, hs[kExz]);
+                load_pack<S, C, V>(P.es_in[kEzx] + off, hs[kEzx]);
+                load_pack<S, C, V>(P.es_in[kEzy] + off, hs[kEzy]);
+                if (own) {
+                    load_pack<S, C, V>(P.es_in[kEyx] + off, hs[kEyx]);
+                    load_pack<S, C, V>(P.es_in[kEyz] + off, hs[kEyz]);
+                }
+            }
+            if (STEADY && own && kk > k0) {
+                const int kb = kk - 1;
+                const bool rowb = in_main(kb, Nk, P.pk) && in_main(j, Nj, P.pj);
+                if (!(rowb && ib >= P.pi && ib + V <= Ni - P.pi)) {
+                    const int offb = kb * plane + row_off;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) load_pack<S, C, V>(P.bs[q] + offb, hb[q]);
+                }
+            }
+        }
 
         C e1[3][V];
         if (active) {
@@ -527,22 +588,19 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                 const Pml<S, C>& P = a.pml;
                 const bool rowm = in_main(kw, Nk, P.pk) && in_main(j, Nj, P.pj);
                 if (!(rowm && ib >= P.pi && ib + V <= Ni - P.pi)) {
-                    C sxy[V], sxz[V], syx[V], syz[V], szx[V], szy[V];
-                    load_pack<S, C, V>(P.es_in[kExy] + off, sxy);
-                    load_pack<S, C, V>(P.es_in[kExz] + off, sxz);
-                    load_pack<S, C, V>(P.es_in[kEzx] + off, szx);
-                    load_pack<S, C, V>(P.es_in[kEzy] + off, szy);
-                    if (own) {
-                        load_pack<S, C, V>(P.es_in[kEyx] + off, syx);
-                        load_pack<S, C, V>(P.es_in[kEyz] + off, syz);
-                    }
-                    const C ay = P.ty[j], cy_xy = P.ty[Nj + j], cy_zy = P.ty[2 * Nj + j];
+                    C (&sxy)[V] = hs[kExy];
+                    C (&sxz)[V] = hs[kExz];
+                    C (&syx)[V] = hs[kEyx];
+                    C (&syz)[V] = hs[kEyz];
+                    C (&szx)[V] = hs[kEzx];
+                    C (&szy)[V] = hs[kEzy];
+                    const C ay = tyv[0], cy_xy = tyv[1], cy_zy = tyv[2];
                     const C az = P.tz[kw], cz_xz = P.tz[Nk + kw], cz_yz = P.tz[2 * Nk + kw];
 #pragma unroll
                     for (int v = 0; v < V; ++v) {
                         const int i = ib + v;
                         if (rowm && in_main(i, Ni, P.pi)) continue;
-                        const C ax = P.tx[i], cx_yx = P.tx[Ni + i], cx_zx = P.tx[2 * Ni + i];
+                        const C ax = sTx[i], cx_yx = sTx[Ni + i], cx_zx = sTx[2 * Ni + i];
                         const C by_im = (v > 0) ? bn[1][v - 1] : sBy[r * Ni + im0];
                         sxy[v] = stored<S, C>(sxy[v] * ay + cy_xy * (bn[2][v] - sBz[r * Ni + ib + v]));
                         szy[v] = stored<S, C>(szy[v] * ay - cy_zy * (bn[0][v] - sBx[r * Ni + ib + v]));
@@ -601,20 +659,19 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                     const bool rowm = in_main(kb, Nk, P.pk) && in_main(j, Nj, P.pj);
                     if (!(rowm && ib >= P.pi && ib + V <= Ni - P.pi)) {
                         if (STEADY) {
-                            C sxy[V], sxz[V], syx[V], syz[V], szx[V], szy[V];
-                            load_pack<S, C, V>(P.bs[kBxy] + offb, sxy);
-                            load_pack<S, C, V>(P.bs[kBxz] + offb, sxz);
-                            load_pack<S, C, V>(P.bs[kByx] + offb, syx);
-                            load_pack<S, C, V>(P.bs[kByz] + offb, syz);
-                            load_pack<S, C, V>(P.bs[kBzx] + offb, szx);
-                            load_pack<S, C, V>(P.bs[kBzy] + offb, szy);
-                            const C ay = P.ty[j], cy_xy = P.ty[3 * Nj + j], cy_zy = P.ty[4 * Nj + j];
+                            C (&sxy)[V] = hb[kBxy];
+                            C (&sxz)[V] = hb[kBxz];
+                            C (&syx)[V] = hb[kByx];
+                            C (&syz)[V] = hb[kByz];
+                            C (&szx)[V] = hb[kBzx];
+                            C (&szy)[V] = hb[kBzy];
+                            const C ay = tyv[0], cy_xy = tyv[3], cy_zy = tyv[4];
                             const C az = P.tz[kb], cz_xz = P.tz[3 * Nk + kb], cz_yz = P.tz[4 * Nk + kb];
 #pragma unroll
                             for (int v = 0; v < V; ++v) {
                                 const int i = ib + v;
                                 if (rowm && in_main(i, Ni, P.pi)) continue;
-                                const C ax = P.tx[i], cx_yx = P.tx[3 * Ni + i], cx_zx = P.tx[4 * Ni + i];
+                                const C ax = sTx[i], cx_yx = sTx[3 * Ni + i], cx_zx = sTx[4 * Ni + i];
                                 const C ey_ip = (v < V - 1) ? eP[1][v + 1] : sEy[r * Ni + ipl];
                                 const C ez_ip = (v < V - 1) ? eP[2][v + 1] : sEz[r * Ni + ipl];
                                 const C ex_jp = sEx[r * Ni + ib + v];

@@ -71,6 +71,11 @@ int v_candidates(int precision, int* out) {
     return 4;
 }
 
+size_t sweep_lds_bytes(const fdtd_grid* g, int TY) {
+    const int cs = (int)compute_size(g->precision);
+    return g->pml ? fdtd::fused_pml_lds_bytes(TY, g->Ni, cs) : fdtd::fused_lds_bytes(TY, g->Ni, cs);
+}
+
 int max_threads(const fdtd_grid* g, int V) {
     const int cs = (int)compute_size(g->precision);
     return g->pml ? fdtd::fused_pml_max_threads(V, cs) : fdtd::fused_max_threads(V, cs);
@@ -85,7 +90,7 @@ bool shape_valid(const fdtd_grid* g, int V, int TY, int kchunk) {
     if (g->Ni % V != 0) return false;
     const int lanes = g->Ni / V;
     if ((TY + 1) * lanes > max_threads(g, V)) return false;
-    if (fdtd::fused_lds_bytes(TY, g->Ni, (int)compute_size(g->precision)) > kLdsLimit) return false;
+    if (sweep_lds_bytes(g, TY) > kLdsLimit) return false;
     return true;
 }
 
@@ -132,7 +137,7 @@ bool default_tuning(const fdtd_grid* g, Tuning* t) {
             const size_t lds_cap = pass == 0 ? kLdsLimit / 2 : kLdsLimit;
             int best = 0;
             for (int ty = std::min(g->Nj, mt / lanes - 1); ty >= 1; --ty)
-                if (fdtd::fused_lds_bytes(ty, g->Ni, cs) <= lds_cap) { best = ty; break; }
+                if (sweep_lds_bytes(g, ty) <= lds_cap) { best = ty; break; }
             if (!best) continue;
             t->V = V;
             t->TY = best;
@@ -404,7 +409,7 @@ int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out) {
     a.c = make_coefs<C>(g);
     a.g = make_geom(g, t);
     a.pml = make_pml<S, C>(g, in, out);
-    const size_t lds = fdtd::fused_lds_bytes(t.TY, g->Ni, sizeof(C));
+    const size_t lds = sweep_lds_bytes(g, t.TY);
     if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
     const int threads = (t.TY + 1) * (g->Ni / V);
     hipLaunchKernelGGL(kern, dim3(8 * a.g.per_xcd), dim3(threads), lds, g->stream, a);
@@ -1455,7 +1460,7 @@ int fdtd_autotune(fdtd_grid* g, int reps, int out[3], float* best_ms) {
         const int lanes = g->Ni / V;
         for (int TY = 1; TY <= std::min(g->Nj, 16); ++TY) {
             if ((TY + 1) * lanes > max_threads(g, V)) break;
-            if (fdtd::fused_lds_bytes(TY, g->Ni, cs) > kLdsLimit) break;
+            if (sweep_lds_bytes(g, TY) > kLdsLimit) break;
             const int kdef = default_kchunk(g->Nj, g->Nk, TY);
             const int kcs[3] = {kdef, std::max(1, kdef / 2), std::min(g->Nk, kdef * 2)};
             for (int kc : kcs) {

@@ -10,6 +10,9 @@ compiled from its sources with its CMake flags (oracle/_ref); the source is
 written to Jx = Jy = Jz before each step, as the reference drivers do
 (perf-tests/sample/sample.cpp:67-82), so the OpenMP class injects it.
 kind "port": the oracle's C restatement (same loop structure and flags).
+With --pml P the grid is the PML class (src/FDTD/FDTD_PML.cpp) with
+pml_percent P, resp. the oracle's restatement of it; the reference class
+runs on the whole OpenMP team as its driver does (sample.cpp:93-123).
 
 Run as a subprocess so that OMP_NUM_THREADS takes effect:
     OMP_NUM_THREADS=16 python -m oracle.cpu_bench --n 512
@@ -38,6 +41,7 @@ def main() -> None:
     ap.add_argument("--min-steps", type=int, default=3)
     ap.add_argument("--max-steps", type=int, default=25)
     ap.add_argument("--kind", choices=["auto", "reference", "port"], default="auto")
+    ap.add_argument("--pml", type=float, default=None)
     a = ap.parse_args()
     kind = a.kind
     if kind == "auto":
@@ -50,15 +54,23 @@ def main() -> None:
     idx = src["i"] + src["j"] * n + src["k"] * n * n
     t_setup = time.perf_counter()
     if kind == "reference":
-        g = O.RefGrid(n, n, n, d, d, d, dt)
+        if a.pml is None:
+            g = O.RefGrid(n, n, n, d, d, d, dt)
+            advance = g.update_fields
+        else:
+            g = O.RefPmlGrid(n, n, n, d, d, d, dt, a.pml)
+
+            def advance(k):
+                g.update_fields(k, one_thread=False)
 
         def step(s):
             val = O.waveform(1, 1.0, src["period"], src["t0"], src["sigma"], 0.0, (s + 1) * dt)
             for comp in (6, 7, 8):
                 g.set_value(comp, idx, val)
-            g.update_fields(1)
+            advance(1)
     else:
-        g = O.OracleGrid(n, n, n, d, d, d, dt)
+        g = (O.OracleGrid(n, n, n, d, d, d, dt) if a.pml is None
+             else O.OraclePmlGrid(n, n, n, d, d, d, dt, a.pml))
 
         def step(s):
             g.run_point_sources([src], 1)
@@ -77,7 +89,9 @@ def main() -> None:
     print(json.dumps({
         "value": cells * steps / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
         "kind": kind, "steps": steps, "seconds": el, "setup_seconds": t_setup,
-        "sample": (f"{n}^3 periodic grid, Gaussian-modulated Jz point source at the centre, "
+        "sample": (f"{n}^3 " + ("periodic grid" if a.pml is None else
+                                f"grid with a PML layer (pml_percent {a.pml})")
+                   + ", Gaussian-modulated Jz point source at the centre, "
                    f"1 warm-up + {steps} timed steps (B/2,E,B/2 each), "
                    f"OMP_NUM_THREADS={threads}"),
     }))

@@ -278,8 +278,13 @@ class RefPmlGrid(RefGrid):
             ref_lib().ref_pml_destroy(self.h)
             self.h = None
 
-    def update_fields(self, nsteps=1):
-        ref_lib().ref_pml_update_fields(self.h, nsteps)
+    def update_fields(self, nsteps=1, one_thread=True):
+        """one_thread=False runs the class on the whole OpenMP team, as the
+        reference's drivers do (for timing only: the values may differ)."""
+        if one_thread:
+            ref_lib().ref_pml_update_fields(self.h, nsteps)
+        else:
+            ref_lib().ref_update_fields(self.h, nsteps)
 
 
 # ---------------------------------------------------------------------------
