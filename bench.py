@@ -342,10 +342,27 @@ def bench_pml(a):
     def make(prec):
         g = FDTD_PML(Parameters(n, n, n), dt, pml, prec)
         g.set_point_sources([src])
+        if a.tuning:
+            g.set_tuning(*[int(x) for x in a.tuning.split(",")])
+            return g
         shape, ms = g.autotune(reps=3)
         log(f"pml autotune {prec.name}: V={shape[0]} TY={shape[1]} kchunk={shape[2]} "
             f"({ms:.3f} ms/sweep)")
         return g
+
+    if a.profile:
+        # one precision only, for rocprofv3 runs
+        prec = {"fp32": Precision.FP32, "fp64": Precision.FP64, "fp16": Precision.FP16}[a.profile]
+        g = make(prec)
+        r = run_path(g, K, a.warmup)
+        nbytes, main, layer = pml_bytes_per_step((n, n, n), g.pml_widths, WORD[prec])
+        ms_copy = calibration_copy(1 << 30, 5)
+        print(json.dumps(dict(profile=a.profile, workload="pml", n=n, pml=pml, steps=K,
+                              tuning=g.get_tuning(), widths=g.pml_widths,
+                              alg_bytes_per_step=nbytes, main_cells=main, layer_cells=layer,
+                              **r, calibration_copy=dict(bytes=1 << 30, ms=ms_copy))),
+              flush=True)
+        return
 
     res = {}
     grids = {}
