@@ -24,6 +24,7 @@ PATH_FUSED, PATH_3SWEEP = 0, 1
 EXPORTED = [
     "fdtd_last_error", "fdtd_create", "fdtd_destroy", "fdtd_create_pml", "fdtd_get_pml",
     "fdtd_pml_download_split", "fdtd_pml_profile", "fdtd_upload", "fdtd_download",
+    "fdtd_download_plane",
     "fdtd_zero_currents", "fdtd_set_current_values", "fdtd_set_point_sources",
     "fdtd_update_fields", "fdtd_update_fields_3sweep", "fdtd_get_step", "fdtd_set_step",
     "fdtd_synchronize", "fdtd_get_info", "fdtd_sum_squares", "fdtd_max_abs", "fdtd_compare",
@@ -88,6 +89,7 @@ def load():
                               _dp], _i),
         "fdtd_upload": ([_vp, _i, _dp, ctypes.c_size_t], _i),
         "fdtd_download": ([_vp, _i, _dp, ctypes.c_size_t], _i),
+        "fdtd_download_plane": ([_vp, _i, _i, _dp, ctypes.c_size_t], _i),
         "fdtd_zero_currents": ([_vp], _i),
         "fdtd_set_current_values": ([_vp, _i, ctypes.POINTER(ctypes.c_longlong),
                                      ctypes.POINTER(_i), _dp], _i),

@@ -905,6 +905,49 @@ int fdtd_download(fdtd_grid* g, int c, double* host, size_t count) {
     return FDTD_OK;
 }
 
+int fdtd_download_plane(fdtd_grid* g, int c, int k, double* host, size_t count) {
+    if (!g || !host) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (int rc = check_component(c)) return rc;
+    const size_t plane = (size_t)g->Ni * g->Nj;
+    if (k < 0 || k >= g->Nk) return fail(FDTD_ERR_INVALID_ARGUMENT, "plane %d outside 0..%d", k, g->Nk - 1);
+    if (count != plane)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "count %zu != plane cells %zu", count, plane);
+    const size_t off = (size_t)k * plane;
+    if (c >= 6) {  // J: dense values of the plane, then the sparse ones
+        const int d = c - 6;
+        std::memset(host, 0, plane * sizeof(double));
+        if (g->j_dense_active) {
+            if (g->precision == FDTD_FP64) {
+                HIP_TRY(hipMemcpyAsync(host, static_cast<const double*>(g->J[d]) + off, plane * 8,
+                                       hipMemcpyDeviceToHost, g->stream));
+            } else {
+                if (int rc = ensure_staging(g)) return rc;
+                launch_to_f64<float>(static_cast<const float*>(g->J[d]) + off, g->staging, plane, 1.0,
+                                     g->stream);
+                HIP_TRY(hipMemcpyAsync(host, g->staging, plane * 8, hipMemcpyDeviceToHost, g->stream));
+            }
+            HIP_TRY(hipStreamSynchronize(g->stream));
+        }
+        for (const auto& kv : g->sparse_j) {
+            const long long idx = kv.first / 3;
+            if (kv.first % 3 == d && (size_t)idx >= off && (size_t)idx < off + plane)
+                host[idx - off] = kv.second;
+        }
+        return FDTD_OK;
+    }
+    const char* src = static_cast<const char*>(field_ptr(g, c)) + off * g->esize;
+    if (g->precision == FDTD_FP64) {
+        HIP_TRY(hipMemcpyAsync(host, src, plane * 8, hipMemcpyDeviceToHost, g->stream));
+    } else {
+        if (int rc = ensure_staging(g)) return rc;
+        to_f64(g->precision, src, g->staging, plane, 1.0 / g->scales[c], g->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(host, g->staging, plane * 8, hipMemcpyDeviceToHost, g->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return FDTD_OK;
+}
+
 int fdtd_zero_currents(fdtd_grid* g) {
     if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
     g->j_dense_active = false;

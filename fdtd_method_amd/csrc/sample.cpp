@@ -6,6 +6,11 @@
 //   sample                 N = 32, 100 iterations
 //   sample N iters         as the reference
 //   sample N iters [fp64|fp32|fp16] [--pml [PCT]] [--dump FILE] [--dump-pml FILE]
+//   sample N iters 1       also writes the plane Nk/2 of Ex..Bz after every
+//                          step t (0-based) to OutFiles_<1..6>/<t>.csv, the
+//                          slices the reference's plotting script reads
+//                          (python_script_legend/visualization.py:10-49 runs
+//                          the driver with this third argument)
 //
 // Prints "Execution time: X s" and the 10x10 Ex slice around the centre of
 // plane Nk/2 in the reference's format (setw(12), fixed, setprecision(5)).
@@ -22,8 +27,10 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <functional>
 #include <memory>
 #include <string>
+#include <sys/stat.h>
 
 #include "FDTD_mi355x.hpp"
 
@@ -46,7 +53,8 @@ static CurrentParameters wave_current() {
 // the cos^2 box current on Jx = Jy = Jz before each of the first
 // min(iterations, it) steps, then zeroed_currents() and the remaining steps.
 // Returns the elapsed seconds.
-static double run_wave(FDTD& method, const Parameters& params, int it) {
+static double run_wave(FDTD& method, const Parameters& params, int it,
+                       const std::function<void(int)>& after_step = nullptr) {
     const CurrentParameters cur_param = wave_current();
     const double T = cur_param.period;
     const double Tx = cur_param.period_x, Ty = cur_param.period_y, Tz = cur_param.period_z;
@@ -79,9 +87,17 @@ static double run_wave(FDTD& method, const Parameters& params, int it) {
                     method.get_field(Component::JZ)[index] = value;
                 }
         method.update_fields();
+        if (after_step) after_step(t);
     }
     method.zeroed_currents();
-    if (it > cur_time) method.update_fields(it - cur_time);
+    if (after_step) {
+        for (int t = cur_time; t < it; ++t) {
+            method.update_fields();
+            after_step(t);
+        }
+    } else if (it > cur_time) {
+        method.update_fields(it - cur_time);
+    }
     method.synchronize();
     auto end = std::chrono::high_resolution_clock::now();
     std::chrono::duration<double> elapsed = end - start;
@@ -114,12 +130,25 @@ static bool dump_ex(FDTD& method, const char* path) {
     return true;
 }
 
+// The plotting script's input: plane Nk/2 of Ex..Bz into OutFiles_<1..6>/<t>.csv
+// (component numbering of visualization.py:10).
+static void write_slices(FDTD& method, const Parameters& params, int t) {
+    for (int c = 0; c < 6; ++c) {
+        const std::string dir = "OutFiles_" + std::to_string(c + 1);
+        ::mkdir(dir.c_str(), 0755);
+        write_csv_slice(method, static_cast<Component>(c), params.Nk / 2,
+                        dir + "/" + std::to_string(t) + ".csv");
+    }
+}
+
 static int spherical_wave(int n, int it, Precision prec, double pml, const char* dump,
-                          const char* dump_pml) {
+                          const char* dump_pml, bool slices) {
     const Parameters params = wave_parameters(n);
     const double dt = wave_current().dt;
     FDTD method(params, dt, prec);
-    const double elapsed = run_wave(method, params, it);
+    std::function<void(int)> after_step;
+    if (slices) after_step = [&](int t) { write_slices(method, params, t); };
+    const double elapsed = run_wave(method, params, it, after_step);
     std::cout << "Execution time: " << elapsed << " s" << std::endl;
 
     std::unique_ptr<FDTD_PML> pml_method;
@@ -144,6 +173,7 @@ int main(int argc, char* argv[]) {
     double pml = -1.0;
     const char* dump = nullptr;
     const char* dump_pml = nullptr;
+    bool slices = false;
     int pos = 0;
     for (int a = 1; a < argc; ++a) {
         if (std::strcmp(argv[a], "--dump") == 0 && a + 1 < argc) {
@@ -165,16 +195,19 @@ int main(int argc, char* argv[]) {
         } else if (pos == 1) {
             iters = std::atoi(argv[a]);
             ++pos;
+        } else if (pos == 2 && std::strcmp(argv[a], "1") == 0) {
+            slices = true;  // visualization.py:48-49 passes str(int(True))
+            ++pos;
         } else {
-            pos = 3;
+            pos = 4;
         }
     }
-    if (pos == 1 || pos > 2) {  // perf-tests/sample/sample.cpp:163-166
+    if (pos == 1 || pos > 3) {  // perf-tests/sample/sample.cpp:163-166
         std::cout << "ERROR: Incorrect number of parameters" << std::endl;
         return 1;
     }
     try {
-        return spherical_wave(N, iters, prec, pml, dump, dump_pml);
+        return spherical_wave(N, iters, prec, pml, dump, dump_pml, slices);
     } catch (const std::exception& e) {
         std::cerr << e.what() << std::endl;
         return 1;

@@ -194,6 +194,14 @@ class FDTD:
         _lib.check(self._L.fdtd_download(self._h, int(comp), _dptr(out), out.size))
         return out
 
+    def download_plane(self, comp, k: int) -> np.ndarray:
+        """FP64 copy of plane k of a component, shape (Nj, Ni)."""
+        self._flush()
+        Ni, Nj, _ = self.shape
+        out = np.empty(Ni * Nj, dtype=np.float64)
+        _lib.check(self._L.fdtd_download_plane(self._h, int(comp), int(k), _dptr(out), out.size))
+        return out.reshape(Nj, Ni)
+
     def set_current_values(self, index, comp, value) -> None:
         idx = np.ascontiguousarray(index, dtype=np.int64)
         cc = np.ascontiguousarray(comp, dtype=np.int32)

@@ -23,6 +23,7 @@
 
 #include <array>
 #include <cstddef>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -190,6 +191,15 @@ public:
 
     void synchronize() { check_status(fdtd_synchronize(grid_)); }
 
+    // FP64 copy of plane k of a component (Ni*Nj values, index i + j*Ni),
+    // after pending host writes are uploaded.
+    std::vector<double> get_plane(Component c, int k) {
+        flush();
+        std::vector<double> out(static_cast<std::size_t>(parameters_.Ni) * parameters_.Nj);
+        check_status(fdtd_download_plane(grid_, static_cast<int>(c), k, out.data(), out.size()));
+        return out;
+    }
+
 protected:
     // pml_percent < 0: periodic grid (fdtd_create); otherwise the PML
     // subclass's grid (fdtd_create_pml).
@@ -245,6 +255,31 @@ public:
 private:
     const fdtd_grid* grid() const { return const_cast<FDTD_PML*>(this)->handle(); }
 };
+
+// Writes plane k of a component as the CSV slice read by the reference's
+// plotting script (python_script_legend/visualization.py:10-45, which reads
+// it with pandas.read_csv(sep=';') and turns decimal commas into points):
+// a header line of column indices 0..Ni-1, then one line per row j with the
+// Ni values of that row, ';'-separated, in %.9g with a decimal comma.
+inline void write_csv_slice(FDTD& m, Component c, int k, const std::string& path) {
+    const std::vector<double> v = m.get_plane(c, k);
+    const int Ni = m.parameters().Ni, Nj = m.parameters().Nj;
+    std::FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) throw std::runtime_error("cannot write " + path);
+    for (int i = 0; i < Ni; ++i) std::fprintf(f, i ? ";%d" : "%d", i);
+    std::fputc('\n', f);
+    char num[64];
+    for (int j = 0; j < Nj; ++j) {
+        for (int i = 0; i < Ni; ++i) {
+            std::snprintf(num, sizeof num, "%.9g", v[static_cast<std::size_t>(j) * Ni + i]);
+            for (char* q = num; *q; ++q)
+                if (*q == '.') *q = ',';
+            std::fputs(num, f);
+            std::fputc(i + 1 < Ni ? ';' : '\n', f);
+        }
+    }
+    if (std::fclose(f) != 0) throw std::runtime_error("cannot write " + path);
+}
 
 inline std::size_t Field::size() const { return owner_->cells(); }
 

@@ -122,6 +122,11 @@ int fdtd_upload(fdtd_grid* grid, int component, const double* host, size_t count
 /* Read side of get_field(Component): FP64 copy of the current values. */
 int fdtd_download(fdtd_grid* grid, int component, double* host, size_t count);
 
+/* FP64 copy of plane k (Ni*Nj values, index i + j*Ni) of component 0..8:
+ * a slice for output without moving the whole field (the CSV slices read
+ * by python_script_legend/visualization.py:10-45 are planes). */
+int fdtd_download_plane(fdtd_grid* grid, int component, int k, double* host, size_t count);
+
 /* zeroed_currents() -- src/FDTD/FDTD.cpp:132-136: J := 0 (dense J and the
  * values set by fdtd_set_current_values; point sources are kept). */
 int fdtd_zero_currents(fdtd_grid* grid);

@@ -16,6 +16,7 @@ import pytest
 
 from conftest import ROOT
 from fdtd_method_amd import FDTD, Parameters, Precision
+from oracle import oracle as O
 from test_oracle import CONVERGENCE, run_convergence
 
 pytestmark = pytest.mark.gpu
@@ -88,3 +89,42 @@ def test_cpp_sample_driver_pml_block_matches_reference(tmp_path, golden_dir):
     k = lines.index("PML: ")
     printed = [[float(x) for x in ln.split()] for ln in lines[k + 2:k + 12]]
     np.testing.assert_allclose(printed, rec["ex_slice"], atol=5e-6)
+
+
+def test_cpp_sample_driver_writes_visualization_slices(tmp_path):
+    """`sample N iters 1` (the call python_script_legend/visualization.py:48-49
+    makes) writes OutFiles_<1..6>/<t>.csv; read back exactly as that script
+    reads them (pandas, sep=';', decimal commas), each slice is plane Nk/2 of
+    the oracle's field after t+1 steps."""
+    import pandas as pd
+    exe = os.path.join(ROOT, "fdtd_method_amd", "bin", "sample")
+    n, it = 16, 6
+    res = subprocess.run([exe, str(n), str(it), "1"], capture_output=True, text=True,
+                         timeout=300, cwd=tmp_path)
+    assert res.returncode == 0, res.stdout + res.stderr
+    for t in range(it):
+        ref = O.spherical_wave(n, t + 1)
+        for c in range(6):
+            data = pd.read_csv(tmp_path / f"OutFiles_{c + 1}" / f"{t}.csv", sep=";")
+            data = data.apply(lambda x: x.str.replace(",", ".").astype(float)
+                              if x.dtype == "object" else x)
+            assert data.shape == (n, n)
+            plane = ref[c].reshape(n, n, n)[n // 2]
+            np.testing.assert_allclose(data.to_numpy(dtype=float), plane, rtol=1e-8,
+                                       atol=1e-300)
+    bad = subprocess.run([exe, "16", "6", "2"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "ERROR: Incorrect number of parameters" in bad.stdout
+
+
+def test_download_plane_matches_full_download():
+    from fdtd_method_amd import FDTD, Parameters, Precision
+    rng = np.random.default_rng(3)
+    for prec in (Precision.FP64, Precision.FP32):
+        g = FDTD(Parameters(12, 10, 7), 1e-12, prec)
+        for c in range(6):
+            g.upload(c, rng.standard_normal(12 * 10 * 7))
+        g.set_current_values([5 + 3 * 12 + 4 * 120], [8], [2.5])
+        for c in range(9):
+            full = g.download(c).reshape(7, 10, 12)
+            for k in (0, 4, 6):
+                assert np.array_equal(g.download_plane(c, k), full[k])
