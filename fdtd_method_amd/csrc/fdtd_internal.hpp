@@ -85,6 +85,7 @@ struct fdtd_grid {
     int autoscale_log2 = 12;
     double* d_part = nullptr;
     double* d_out = nullptr;
+    unsigned* d_max = nullptr;  // per-component max |stored| tracked by a sweep (float bits)
     hipEvent_t ev[3] = {};
     // PML layer (include/FDTD/FDTD_PML.h): set when some layer width is > 0.
     bool pml = false;

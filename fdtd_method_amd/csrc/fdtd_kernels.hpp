@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace fdtd {
 
 // ---------------------------------------------------------------------------
@@ -79,6 +81,30 @@ template <typename S, int V> struct alignas(sizeof(S) * V) Pack { S v[V]; };
 template <typename S, typename C, int V>
 __device__ __forceinline__ void load_pack(const S* __restrict__ p, C (&out)[V]) {
     const Pack<S, V> pk = *reinterpret_cast<const Pack<S, V>*>(p);
+#pragma unroll
+    for (int v = 0; v < V; ++v) out[v] = static_cast<C>(pk.v[v]);
+}
+
+// A pack held as opaque machine words until it is widened where it is
+// used: the compiler then cannot split FP16 halves (or convert) right after
+// the load, which would make a prefetch wait for its own loads.
+template <typename S, int V> struct RawPack {
+    using W = typename std::conditional<(sizeof(S) * V >= 4), uint32_t, uint16_t>::type;
+    static constexpr int kWords = static_cast<int>(sizeof(S) * V / sizeof(W));
+    struct alignas(sizeof(S) * V) Words { W w[kWords]; };
+    Words words;
+};
+
+template <typename S, int V>
+__device__ __forceinline__ RawPack<S, V> load_raw(const S* __restrict__ p) {
+    RawPack<S, V> r;
+    r.words = *reinterpret_cast<const typename RawPack<S, V>::Words*>(p);
+    return r;
+}
+
+template <typename S, typename C, int V>
+__device__ __forceinline__ void widen(const RawPack<S, V>& raw, C (&out)[V]) {
+    const Pack<S, V> pk = __builtin_bit_cast(Pack<S, V>, raw.words);
 #pragma unroll
     for (int v = 0; v < V; ++v) out[v] = static_cast<C>(pk.v[v]);
 }
@@ -401,6 +427,12 @@ struct SweepArgs {
     int pad;
     Coefs<C> c;
     SweepGeom g;
+    // FP16 storage only: stored values are multiplied by oscale (per
+    // component Ex..Bz; exact powers of two, 1 outside rescaling sweeps), and
+    // when maxbits is set each wavefront folds max |stored value| per
+    // component into it (float bits, atomic max).
+    C oscale[6];
+    unsigned* maxbits;
     Pml<S, C> pml;  // used by PML sweeps only
 };
 
@@ -421,7 +453,8 @@ __device__ __forceinline__ int pml_pack(int lane, int npacks, int pi, int V) {
     return lane - nhi;
 }
 
-template <typename S, typename C, typename JT, int V, bool STEADY, bool PML = false>
+template <typename S, typename C, typename JT, int V, bool STEADY, bool PML = false,
+          bool AUTO = false>
 __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int tile_y, int chunk,
                                            C* __restrict__ lds) {
     const int Ni = a.g.Ni, Nj = a.g.Nj, Nk = a.g.Nk, TY = a.g.TY;
@@ -463,6 +496,23 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
         for (int q = 0; q < 5; ++q) tyv[q] = active ? a.pml.ty[q * Nj + j] : C(0);
     }
 
+    // FP16 storage: rescaling factors and the running max of stored values.
+    constexpr bool kAuto = AUTO && sizeof(S) < sizeof(C);
+    C mx[6] = {C(0), C(0), C(0), C(0), C(0), C(0)};
+    auto put = [&](S* p, const C (&x)[V], int comp) {
+        if constexpr (kAuto) {
+            C o[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                o[v] = x[v] * a.oscale[comp];
+                mx[comp] = fmax(mx[comp], fabs(o[v]));
+            }
+            store_pack<S, C, V>(p, o);
+        } else {
+            store_pack<S, C, V>(p, x);
+        }
+    };
+
     C eP[3][V];  // E_{n+1} at plane kk-1 (own cells)
     C bP[3][V];  // B at plane kk-1 (own cells)
 #pragma unroll
@@ -470,24 +520,25 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
 #pragma unroll
         for (int v = 0; v < V; ++v) { eP[d][v] = C(0); bP[d][v] = C(0); }
 
-    // Plane loads, software-pipelined one plane ahead.  Halo-row threads do
-    // not need Ey; its slot carries Bz of row j0-1 instead, and bm carries
-    // Bx of row j0-1.
+    // Plane loads, software-pipelined one plane ahead and kept in the
+    // storage type until the rotation at the end of the iteration.
+    // Halo-row threads do not need Ey; its slot carries Bz of row j0-1
+    // instead, and bm carries Bx of row j0-1.
+    using PK = RawPack<S, V>;
     C en[3][V], bn[3][V], bm[V];
-    auto load_plane = [&](int kw, C (&e)[3][V], C (&b)[3][V], C (&m)[V]) {
+    // The Ey slot is one load for every thread with a selected address (a
+    // branch whose two sides load into the same registers would make the
+    // halo row wait for the whole prefetch).
+    auto load_plane = [&](int kw, PK (&e)[3], PK (&b)[3], PK& m) {
         const int off = kw * plane + row_off;
-        load_pack<S, C, V>(a.in.b[0] + off, b[0]);
-        load_pack<S, C, V>(a.in.b[1] + off, b[1]);
-        load_pack<S, C, V>(a.in.b[2] + off, b[2]);
-        load_pack<S, C, V>(a.in.e[0] + off, e[0]);
-        load_pack<S, C, V>(a.in.e[2] + off, e[2]);
-        if (own) {
-            load_pack<S, C, V>(a.in.e[1] + off, e[1]);
-        } else {
-            const int offm = kw * plane + rowm_off;
-            load_pack<S, C, V>(a.in.b[2] + offm, e[1]);
-            load_pack<S, C, V>(a.in.b[0] + offm, m);
-        }
+        const int offm = kw * plane + rowm_off;
+        b[0] = load_raw<S, V>(a.in.b[0] + off);
+        b[1] = load_raw<S, V>(a.in.b[1] + off);
+        b[2] = load_raw<S, V>(a.in.b[2] + off);
+        e[0] = load_raw<S, V>(a.in.e[0] + off);
+        e[2] = load_raw<S, V>(a.in.e[2] + off);
+        e[1] = load_raw<S, V>(own ? a.in.e[1] + off : a.in.b[2] + offm);
+        if (!own) m = load_raw<S, V>(a.in.b[0] + offm);
     };
 
     if (active) {
@@ -495,7 +546,14 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
         const int off = kp * plane + row_off;
         load_pack<S, C, V>(a.in.b[0] + off, bP[0]);
         load_pack<S, C, V>(a.in.b[1] + off, bP[1]);
-        load_plane(k0, en, bn, bm);
+        PK e0[3], b0[3], m0;
+        load_plane(k0, e0, b0, m0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            widen<S, C, V>(e0[d], en[d]);
+            widen<S, C, V>(b0[d], bn[d]);
+        }
+        widen<S, C, V>(m0, bm);
     }
 
     for (int kk = k0; kk <= k1; ++kk) {
@@ -516,7 +574,7 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
         }
         __syncthreads();
 
-        C en2[3][V], bn2[3][V], bm2[V];
+        PK en2[3], bn2[3], bm2;
         if (active && kk < k1) load_plane((kk + 1 == Nk) ? 0 : kk + 1, en2, bn2, bm2);
         // PML sweeps: the split fields this iteration updates (E splits of
         // plane kw, B splits of plane kk-1), for lanes holding layer cells,
@@ -616,18 +674,18 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                         }
                     }
                     if (own && kk < k1) {
-                        store_pack<S, C, V>(P.es_out[kExy] + off, sxy);
-                        store_pack<S, C, V>(P.es_out[kExz] + off, sxz);
-                        store_pack<S, C, V>(P.es_out[kEyx] + off, syx);
-                        store_pack<S, C, V>(P.es_out[kEyz] + off, syz);
-                        store_pack<S, C, V>(P.es_out[kEzx] + off, szx);
-                        store_pack<S, C, V>(P.es_out[kEzy] + off, szy);
+                        put(P.es_out[kExy] + off, sxy, 0);
+                        put(P.es_out[kExz] + off, sxz, 0);
+                        put(P.es_out[kEyx] + off, syx, 1);
+                        put(P.es_out[kEyz] + off, syz, 1);
+                        put(P.es_out[kEzx] + off, szx, 2);
+                        put(P.es_out[kEzy] + off, szy, 2);
                     }
                 }
             }
             if (own && kk < k1) {
 #pragma unroll
-                for (int d = 0; d < 3; ++d) store_pack<S, C, V>(a.out.e[d] + off, e1[d]);
+                for (int d = 0; d < 3; ++d) put(a.out.e[d] + off, e1[d], d);
             }
             if (own && kk > k0) {
                 const int offb = (kk - 1) * plane + row_off;
@@ -686,12 +744,12 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                                 nb[1][v] = syz[v] + syx[v];
                                 nb[2][v] = szx[v] + szy[v];
                             }
-                            store_pack<S, C, V>(P.bs[kBxy] + offb, sxy);
-                            store_pack<S, C, V>(P.bs[kBxz] + offb, sxz);
-                            store_pack<S, C, V>(P.bs[kByx] + offb, syx);
-                            store_pack<S, C, V>(P.bs[kByz] + offb, syz);
-                            store_pack<S, C, V>(P.bs[kBzx] + offb, szx);
-                            store_pack<S, C, V>(P.bs[kBzy] + offb, szy);
+                            put(P.bs[kBxy] + offb, sxy, 3);
+                            put(P.bs[kBxz] + offb, sxz, 3);
+                            put(P.bs[kByx] + offb, syx, 4);
+                            put(P.bs[kByz] + offb, syz, 4);
+                            put(P.bs[kBzx] + offb, szx, 5);
+                            put(P.bs[kBzy] + offb, szy, 5);
                         } else {
 #pragma unroll
                             for (int v = 0; v < V; ++v) {
@@ -704,7 +762,7 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                     }
                 }
 #pragma unroll
-                for (int d = 0; d < 3; ++d) store_pack<S, C, V>(a.out.b[d] + offb, nb[d]);
+                for (int d = 0; d < 3; ++d) put(a.out.b[d] + offb, nb[d], 3 + d);
             }
         }
         __syncthreads();
@@ -718,16 +776,26 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
             }
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
+        for (int d = 0; d < 3; ++d) {
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 eP[d][v] = e1[d][v];
                 bP[d][v] = bn[d][v];
-                en[d][v] = en2[d][v];
-                bn[d][v] = bn2[d][v];
             }
+            widen<S, C, V>(en2[d], en[d]);
+            widen<S, C, V>(bn2[d], bn[d]);
+        }
+        widen<S, C, V>(bm2, bm);
+    }
+    if constexpr (kAuto) {
+        if (a.maxbits != nullptr) {
 #pragma unroll
-        for (int v = 0; v < V; ++v) bm[v] = bm2[v];
+            for (int c6 = 0; c6 < 6; ++c6) {
+                float m = mx[c6];
+                for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+                if ((threadIdx.x & 63) == 0) atomicMax(a.maxbits + c6, __float_as_uint(m));
+            }
+        }
     }
 }
 
@@ -735,7 +803,9 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
 // the hardware's round-robin dispatch) owns the contiguous tile range
 // [xcd*per_xcd, (xcd+1)*per_xcd): tiles sharing halo rows and z-chunk
 // boundary planes then share one L2.
-template <typename S, typename C, typename JT, int V, bool STEADY>
+// AUTO (FP16 storage only): the variant that applies SweepArgs::oscale to
+// what it stores and tracks its max; run only around rescaling points.
+template <typename S, typename C, typename JT, int V, bool STEADY, bool AUTO = false>
 __global__ void __launch_bounds__(fused_max_threads(V, sizeof(C)))
 fused_sweep(SweepArgs<S, C, JT> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -744,7 +814,7 @@ fused_sweep(SweepArgs<S, C, JT> a) {
     if (t >= a.g.ntiles) return;
     const int tile_y = t % a.g.ntile_y;
     const int chunk = t / a.g.ntile_y;
-    fused_tile<S, C, JT, V, STEADY>(a, tile_y, chunk, reinterpret_cast<C*>(lds_raw));
+    fused_tile<S, C, JT, V, STEADY, false, AUTO>(a, tile_y, chunk, reinterpret_cast<C*>(lds_raw));
 }
 
 // The fused sweep of a grid with a PML layer (same tiles and LDS staging;
@@ -756,7 +826,7 @@ __host__ __device__ constexpr int fused_pml_max_threads(int V, int csize) {
     return V * csize >= 16 ? 512 : 1024;
 }
 
-template <typename S, typename C, typename JT, int V, bool STEADY>
+template <typename S, typename C, typename JT, int V, bool STEADY, bool AUTO = false>
 __global__ void __launch_bounds__(fused_pml_max_threads(V, sizeof(C)))
 fused_pml_sweep(SweepArgs<S, C, JT> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -765,7 +835,7 @@ fused_pml_sweep(SweepArgs<S, C, JT> a) {
     if (t >= a.g.ntiles) return;
     const int tile_y = t % a.g.ntile_y;
     const int chunk = t / a.g.ntile_y;
-    fused_tile<S, C, JT, V, STEADY, true>(a, tile_y, chunk, reinterpret_cast<C*>(lds_raw));
+    fused_tile<S, C, JT, V, STEADY, true, AUTO>(a, tile_y, chunk, reinterpret_cast<C*>(lds_raw));
 }
 
 // ---------------------------------------------------------------------------
@@ -1119,6 +1189,9 @@ __device__ __forceinline__ SweepArgs<S, C, C> load_batch_grid(const BatchGrid<S,
         a.out.b[d] = uni(out->b[d]);
         a.j[d] = nullptr;
     }
+    a.maxbits = nullptr;
+#pragma unroll
+    for (int c6 = 0; c6 < 6; ++c6) a.oscale[c6] = C(1);
     a.src = uni(p->src);
     a.vals = vals_row ? vals_row + uni(p->src_off) : nullptr;
     a.nsrc = uni(p->nsrc);

@@ -391,12 +391,13 @@ void* split_ptr(fdtd_grid* g, int q) { return q < 6 ? g->ES[g->cur][q] : g->BS[q
 
 // ---- fused sweep --------------------------------------------------------------
 
-template <typename S, int V, bool STEADY>
-int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out) {
+template <typename S, int V, bool STEADY, bool AUTO>
+int launch_fused_a(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
+                   const double* oscale, unsigned* maxbits) {
     using C = typename fdtd::ComputeOf<S>::type;
     using JT = C;
-    auto kern = g->pml ? fdtd::fused_pml_sweep<S, C, JT, V, STEADY>
-                       : fdtd::fused_sweep<S, C, JT, V, STEADY>;
+    auto kern = g->pml ? fdtd::fused_pml_sweep<S, C, JT, V, STEADY, AUTO>
+                       : fdtd::fused_sweep<S, C, JT, V, STEADY, AUTO>;
     fdtd::SweepArgs<S, C, JT> a;
     a.in = fields_of<S>(g, in);
     a.out = fields_of<S>(g, out);
@@ -409,6 +410,8 @@ int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out) {
     a.c = make_coefs<C>(g);
     a.g = make_geom(g, t);
     a.pml = make_pml<S, C>(g, in, out);
+    for (int c = 0; c < 6; ++c) a.oscale[c] = C(oscale ? oscale[c] : 1.0);
+    a.maxbits = maxbits;
     const size_t lds = sweep_lds_bytes(g, t.TY);
     if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
     const int threads = (t.TY + 1) * (g->Ni / V);
@@ -417,36 +420,53 @@ int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out) {
     return FDTD_OK;
 }
 
+// FP16 sweeps that rescale or track the max take the AUTO variant.
+template <typename S, int V, bool STEADY>
+int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
+                   const double* oscale, unsigned* maxbits) {
+    if constexpr (sizeof(S) == 2)
+        if (oscale || maxbits)
+            return launch_fused_a<S, V, STEADY, true>(g, t, slot, in, out, oscale, maxbits);
+    return launch_fused_a<S, V, STEADY, false>(g, t, slot, in, out, oscale, maxbits);
+}
+
 template <typename S, bool STEADY>
-int launch_fused_s(fdtd_grid* g, const Tuning& t, int slot, int in, int out) {
+int launch_fused_s(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
+                   const double* oscale, unsigned* maxbits) {
     switch (t.V) {
-        case 1: return launch_fused_v<S, 1, STEADY>(g, t, slot, in, out);
-        case 2: return launch_fused_v<S, 2, STEADY>(g, t, slot, in, out);
-        case 4: return launch_fused_v<S, 4, STEADY>(g, t, slot, in, out);
+        case 1: return launch_fused_v<S, 1, STEADY>(g, t, slot, in, out, oscale, maxbits);
+        case 2: return launch_fused_v<S, 2, STEADY>(g, t, slot, in, out, oscale, maxbits);
+        case 4: return launch_fused_v<S, 4, STEADY>(g, t, slot, in, out, oscale, maxbits);
         case 8:
-            if constexpr (sizeof(S) <= 4) return launch_fused_v<S, 8, STEADY>(g, t, slot, in, out);
+            if constexpr (sizeof(S) <= 4)
+                return launch_fused_v<S, 8, STEADY>(g, t, slot, in, out, oscale, maxbits);
             [[fallthrough]];
         default: return fail(FDTD_ERR_UNSUPPORTED, "unsupported V=%d", t.V);
     }
 }
 
 // One fused sweep; its source values are row `slot` of the value table.
-int launch_fused(fdtd_grid* g, const Tuning& t, bool steady, int slot, int in, int out) {
+// FP16 grids: `oscale` (may be null) rescales what the sweep stores, and
+// `maxbits` (may be null) receives the per-component max of it.
+int launch_fused(fdtd_grid* g, const Tuning& t, bool steady, int slot, int in, int out,
+                 const double* oscale = nullptr, unsigned* maxbits = nullptr) {
     switch (g->precision) {
         case FDTD_FP64:
-            return steady ? launch_fused_s<double, true>(g, t, slot, in, out)
-                          : launch_fused_s<double, false>(g, t, slot, in, out);
+            return steady ? launch_fused_s<double, true>(g, t, slot, in, out, oscale, maxbits)
+                          : launch_fused_s<double, false>(g, t, slot, in, out, oscale, maxbits);
         case FDTD_FP32:
-            return steady ? launch_fused_s<float, true>(g, t, slot, in, out)
-                          : launch_fused_s<float, false>(g, t, slot, in, out);
+            return steady ? launch_fused_s<float, true>(g, t, slot, in, out, oscale, maxbits)
+                          : launch_fused_s<float, false>(g, t, slot, in, out, oscale, maxbits);
         default:
-            return steady ? launch_fused_s<_Float16, true>(g, t, slot, in, out)
-                          : launch_fused_s<_Float16, false>(g, t, slot, in, out);
+            return steady ? launch_fused_s<_Float16, true>(g, t, slot, in, out, oscale, maxbits)
+                          : launch_fused_s<_Float16, false>(g, t, slot, in, out, oscale, maxbits);
     }
 }
 
 int rescale_to_target(fdtd_grid* g, bool current_only);
 int apply_scales(fdtd_grid* g, const double s[6], bool current_only);
+void target_scales(const fdtd_grid* g, const double mx[6], int target_log2, double floor_mag,
+                   double s[6]);
 
 // ---- temporally blocked sweeps ----------------------------------------------------
 
@@ -542,20 +562,54 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
 // n steps: B half step in place, then n fused sweeps between the buffers.
 // With FP16 autoscaling, the state between two sweeps, (E_n, B_{n+1/2}),
 // is rescaled by exact powers of two every autoscale_interval sweeps.
+// FP16 in-run rescaling: the sweep before a decision point tracks the max
+// of what it stores; at the decision point the host reads it (one small
+// copy), picks new power-of-two scales, and the next sweep stores its
+// output multiplied by new/old (computing with the old coefficients), so
+// the state is rescaled without another pass over HBM.  The last sweep of
+// a run never rescales (a PML grid's final sweep leaves its B splits
+// untouched).
 int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
     if (g->btune.L >= 1 && nsteps >= g->btune.L && !g->pml) return run_blocked(g, nsteps, mid);
     int rc = plain_B(g);
     if (rc) return rc;
     if (mid) HIP_TRY(hipEventRecord(mid, g->stream));
+    const int iv = g->precision == FDTD_FP16 ? g->autoscale_interval : 0;
     for (int s = 0; s < nsteps; ++s) {
-        if (g->autoscale_interval > 0 && s > 0 && s % g->autoscale_interval == 0)
-            if ((rc = rescale_to_target(g, true))) return rc;
+        const bool decide = iv > 0 && s > 0 && s % iv == 0 && s + 1 < nsteps;
+        const bool track = iv > 0 && (s + 1) % iv == 0 && s + 2 < nsteps;
+        double f[6] = {1, 1, 1, 1, 1, 1}, snew[6];
+        if (decide) {
+            unsigned bits[6];
+            HIP_TRY(hipMemcpyAsync(bits, g->d_max, sizeof bits, hipMemcpyDeviceToHost, g->stream));
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            double mx[6];
+            for (int c = 0; c < 6; ++c) {
+                float m;
+                std::memcpy(&m, &bits[c], sizeof m);
+                mx[c] = static_cast<double>(m) / g->scales[c];
+            }
+            target_scales(g, mx, g->autoscale_log2, 0.0, snew);
+            for (int c = 0; c < 6; ++c) f[c] = snew[c] / g->scales[c];
+        }
+        if (track)
+            if ((rc = zero_fill(g->d_max, 32, g->stream))) return rc;
         if (s % kValTable == 0)
             if ((rc = eval_sources(g, g->step + s, std::min(kValTable, nsteps - s)))) return rc;
         const int in = g->cur, out = 1 - g->cur;
-        rc = launch_fused(g, g->tune, s + 1 < nsteps, s % kValTable, in, out);
+        rc = launch_fused(g, g->tune, s + 1 < nsteps, s % kValTable, in, out, decide ? f : nullptr,
+                          track ? g->d_max : nullptr);
         if (rc) return rc;
         g->cur = out;
+        if (decide) {
+            bool changed = false;
+            for (int c = 0; c < 6; ++c) {
+                changed = changed || snew[c] != g->scales[c];
+                g->scales[c] = snew[c];
+            }
+            if (changed && g->pml)
+                if ((rc = build_pml_tables(g))) return rc;
+        }
     }
     g->step += nsteps;
     return FDTD_OK;
@@ -619,30 +673,40 @@ int ensure_dense_j(fdtd_grid* g) {
     return FDTD_OK;
 }
 
-// sum x^2 and max|x| of one storage array (physical units).
-int reduce_field(fdtd_grid* g, const void* p, double inv, double* sumsq, double* maxabs) {
+void* field_ptr(fdtd_grid* g, int c);
+
+// sum x^2 and max|x| of the six E and B components (physical units): all
+// partial and final passes queued back to back, one copy and one wait.
+int reduce_fields(fdtd_grid* g, double sumsq[6], double maxabs[6]) {
     const int nb = std::min<int>(kReduceBlocks, grid_blocks(g->N));
-    switch (g->precision) {
-        case FDTD_FP64:
-            hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<double>, dim3(nb), dim3(256), 0,
-                               g->stream, static_cast<const double*>(p), g->N, inv, g->d_part);
-            break;
-        case FDTD_FP32:
-            hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<float>, dim3(nb), dim3(256), 0,
-                               g->stream, static_cast<const float*>(p), g->N, inv, g->d_part);
-            break;
-        default:
-            hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<_Float16>, dim3(nb), dim3(256), 0,
-                               g->stream, static_cast<const _Float16*>(p), g->N, inv, g->d_part);
+    for (int c = 0; c < 6; ++c) {
+        const void* p = field_ptr(g, c);
+        const double inv = 1.0 / g->scales[c];
+        double* part = g->d_part + (size_t)2 * kReduceBlocks * c;
+        switch (g->precision) {
+            case FDTD_FP64:
+                hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<double>, dim3(nb), dim3(256), 0,
+                                   g->stream, static_cast<const double*>(p), g->N, inv, part);
+                break;
+            case FDTD_FP32:
+                hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<float>, dim3(nb), dim3(256), 0,
+                                   g->stream, static_cast<const float*>(p), g->N, inv, part);
+                break;
+            default:
+                hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<_Float16>, dim3(nb), dim3(256), 0,
+                                   g->stream, static_cast<const _Float16*>(p), g->N, inv, part);
+        }
+        hipLaunchKernelGGL(fdtd::reduce_partials, dim3(1), dim3(256), 0, g->stream, part, nb, 2,
+                           2u, g->d_out + 2 * c);
     }
-    hipLaunchKernelGGL(fdtd::reduce_partials, dim3(1), dim3(256), 0, g->stream, g->d_part, nb, 2,
-                       2u, g->d_out);
     HIP_TRY(hipGetLastError());
-    double r[2];
+    double r[12];
     HIP_TRY(hipMemcpyAsync(r, g->d_out, sizeof r, hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
-    *sumsq = r[0];
-    *maxabs = r[1];
+    for (int c = 0; c < 6; ++c) {
+        if (sumsq) sumsq[c] = r[2 * c];
+        if (maxabs) maxabs[c] = r[2 * c + 1];
+    }
     return FDTD_OK;
 }
 
@@ -722,8 +786,9 @@ int create_grid(const fdtd_parameters* p, double dt, double pml_percent, int pre
     }
     if (g->pml)
         if (int rc = alloc_pml(g)) return cleanup(rc);
-    if (hipMalloc(&g->d_part, sizeof(double) * 3 * kReduceBlocks) != hipSuccess ||
-        hipMalloc(&g->d_out, sizeof(double) * 8) != hipSuccess)
+    if (hipMalloc(&g->d_part, sizeof(double) * 12 * kReduceBlocks) != hipSuccess ||
+        hipMalloc(&g->d_out, sizeof(double) * 16) != hipSuccess ||
+        hipMalloc(&g->d_max, 32) != hipSuccess)
         return cleanup(fail(FDTD_ERR_HIP, "out of device memory"));
     for (auto& e : g->ev)
         if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(FDTD_ERR_HIP, "hipEventCreate"));
@@ -773,6 +838,7 @@ int fdtd_destroy(fdtd_grid* g) {
     if (g->d_vals) (void)hipFree(g->d_vals);
     if (g->d_part) (void)hipFree(g->d_part);
     if (g->d_out) (void)hipFree(g->d_out);
+    if (g->d_max) (void)hipFree(g->d_max);
     for (auto& e : g->ev)
         if (e) (void)hipEventDestroy(e);
     if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -1071,20 +1137,12 @@ int fdtd_get_info(const fdtd_grid* g, int dims[3], int* precision, long long* ce
 
 int fdtd_sum_squares(fdtd_grid* g, double out[6]) {
     if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
-    for (int c = 0; c < 6; ++c) {
-        double m;
-        if (int rc = reduce_field(g, field_ptr(g, c), 1.0 / g->scales[c], &out[c], &m)) return rc;
-    }
-    return FDTD_OK;
+    return reduce_fields(g, out, nullptr);
 }
 
 int fdtd_max_abs(fdtd_grid* g, double out[6]) {
     if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
-    for (int c = 0; c < 6; ++c) {
-        double s;
-        if (int rc = reduce_field(g, field_ptr(g, c), 1.0 / g->scales[c], &s, &out[c])) return rc;
-    }
-    return FDTD_OK;
+    return reduce_fields(g, nullptr, out);
 }
 
 int fdtd_compare(fdtd_grid* t, fdtd_grid* r, double out[18]) {
