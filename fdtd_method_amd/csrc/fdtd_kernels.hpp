@@ -109,12 +109,14 @@ __device__ __forceinline__ void widen(const RawPack<S, V>& raw, C (&out)[V]) {
     for (int v = 0; v < V; ++v) out[v] = static_cast<C>(pk.v[v]);
 }
 
+// Sweep outputs are not read again within the sweep (and the state is far
+// larger than the L2s), so they are stored non-temporally: the L2 keeps the
+// input rows that neighbouring tiles re-read.  The per-element stores are
+// merged into one V-wide store.
 template <typename S, typename C, int V>
 __device__ __forceinline__ void store_pack(S* __restrict__ p, const C (&in)[V]) {
-    Pack<S, V> pk;
 #pragma unroll
-    for (int v = 0; v < V; ++v) pk.v[v] = static_cast<S>(in[v]);
-    *reinterpret_cast<Pack<S, V>*>(p) = pk;
+    for (int v = 0; v < V; ++v) __builtin_nontemporal_store(static_cast<S>(in[v]), p + v);
 }
 
 // Field pointers of one grid (current buffers, or in/out buffer pairs).
@@ -1070,7 +1072,8 @@ __device__ __forceinline__ void blocked_tile(const BlockedArgs<S, C>& a, int tx,
                     S* dst = a.out.e[d] + off;
 #pragma unroll
                     for (int v = 0; v < V; ++v)
-                        if (col_out[v]) dst[v] = static_cast<S>(enew[L - 1][d][v]);
+                        if (col_out[v])
+                            __builtin_nontemporal_store(static_cast<S>(enew[L - 1][d][v]), dst + v);
                 }
             }
             if (pe - 1 >= k0 && pe - 1 < k1) {
@@ -1080,7 +1083,7 @@ __device__ __forceinline__ void blocked_tile(const BlockedArgs<S, C>& a, int tx,
                     S* dst = a.out.b[d] + off;
 #pragma unroll
                     for (int v = 0; v < V; ++v)
-                        if (col_out[v]) dst[v] = static_cast<S>(bnew[d][v]);
+                        if (col_out[v]) __builtin_nontemporal_store(static_cast<S>(bnew[d][v]), dst + v);
                 }
             }
         }
