@@ -30,7 +30,7 @@ EXPORTED = [
     "fdtd_synchronize", "fdtd_get_info", "fdtd_sum_squares", "fdtd_max_abs", "fdtd_compare",
     "fdtd_checksum", "fdtd_add_plane_wave",
     "fdtd_set_scales", "fdtd_get_scales", "fdtd_autoscale", "fdtd_set_autoscale_interval",
-    "fdtd_zero_fields", "fdtd_set_time_blocking", "fdtd_get_time_blocking",
+    "fdtd_zero_fields", "fdtd_set_step_carry", "fdtd_set_time_blocking", "fdtd_get_time_blocking",
     "fdtd_autotune_blocking", "fdtd_set_tuning",
     "fdtd_get_tuning", "fdtd_autotune", "fdtd_time_update_fields", "fdtd_calibration_copy",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_update_fields",
@@ -112,6 +112,7 @@ def load():
         "fdtd_autoscale": ([_vp, _i, ctypes.c_double], _i),
         "fdtd_set_autoscale_interval": ([_vp, _i, _i], _i),
         "fdtd_zero_fields": ([_vp], _i),
+        "fdtd_set_step_carry": ([_vp, _i], _i),
         "fdtd_set_time_blocking": ([_vp, _i, _i, _i, _i, _i], _i),
         "fdtd_get_time_blocking": ([_vp, ctypes.POINTER(_i)], _i),
         "fdtd_autotune_blocking": ([_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)], _i),

@@ -343,6 +343,7 @@ void finish_steps(fdtd_batch* b, int nsteps) {
     for (fdtd_grid* g : b->grids) {
         g->cur = (g->cur + nsteps) & 1;
         g->step += nsteps;
+        g->ba_cur = -1;  // a batch run leaves no carried half step
     }
 }
 

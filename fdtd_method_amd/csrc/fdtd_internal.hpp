@@ -73,9 +73,16 @@ struct fdtd_grid {
     long long step = 0;
     std::map<long long, double> sparse_j;  // key = index*3 + comp
     std::vector<fdtd::PointSource> waveforms;
-    fdtd::PointSource* d_src = nullptr;
+    fdtd::PointSource* d_src = nullptr;  // the list the next launches read (one of d_src_buf)
     double* d_vals = nullptr;  // source values, kValTable steps x nsrc
     int d_src_cap = 0;
+    // Source lists are uploaded asynchronously from pinned staging, double
+    // buffered so a new list never overwrites one that queued work reads.
+    fdtd::PointSource* d_src_buf[2] = {};
+    fdtd::PointSource* h_src[2] = {};
+    hipEvent_t src_ev[2] = {};
+    bool src_ev_used[2] = {false, false};
+    int src_slot = 0;
     int nsrc = 0;
     bool src_dirty = true;
     fdtd::rt::Tuning tune;
@@ -86,6 +93,12 @@ struct fdtd_grid {
     double* d_part = nullptr;
     double* d_out = nullptr;
     unsigned* d_max = nullptr;  // per-component max |stored| tracked by a sweep (float bits)
+    // Carried half step (periodic grids): BA[ba_cur] holds B_{n+1/2} =
+    // (B_n + d) + d of the current state, written by the last run's final
+    // sweep, so the next run starts without the B/2 prologue.  -1: none.
+    void* BA[2][3] = {};
+    int ba_cur = -1;
+    bool carry = true;
     hipEvent_t ev[3] = {};
     // PML layer (include/FDTD/FDTD_PML.h): set when some layer width is > 0.
     bool pml = false;

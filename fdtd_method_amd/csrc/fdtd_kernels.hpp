@@ -435,6 +435,9 @@ struct SweepArgs {
     // component into it (float bits, atomic max).
     C oscale[6];
     unsigned* maxbits;
+    // Final sweeps of periodic grids: when set, also store the next run's
+    // first B half step, (B_n + d) + d, so that run needs no B/2 prologue.
+    S* ahead[3];
     Pml<S, C> pml;  // used by PML sweeps only
 };
 
@@ -692,6 +695,7 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
             if (own && kk > k0) {
                 const int offb = (kk - 1) * plane + row_off;
                 C nb[3][V];
+                C na[3][V];  // the carried half step (final sweeps with a.ahead)
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     const C ey_ip = (v < V - 1) ? eP[1][v + 1] : sEy[r * Ni + ipl];
@@ -708,7 +712,15 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                         nb[0][v] = nb[0][v] + dbx;
                         nb[1][v] = nb[1][v] + dby;
                         nb[2][v] = nb[2][v] + dbz;
+                    } else if (!PML && a.ahead[0] != nullptr) {
+                        na[0][v] = nb[0][v] + dbx;
+                        na[1][v] = nb[1][v] + dby;
+                        na[2][v] = nb[2][v] + dbz;
                     }
+                }
+                if (!STEADY && !PML && a.ahead[0] != nullptr) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) put(a.ahead[d] + offb, na[d], 3 + d);
                 }
                 if constexpr (PML) {
                     // Layer cells of plane kk-1: the next step's full-step B
@@ -1195,6 +1207,8 @@ __device__ __forceinline__ SweepArgs<S, C, C> load_batch_grid(const BatchGrid<S,
     a.maxbits = nullptr;
 #pragma unroll
     for (int c6 = 0; c6 < 6; ++c6) a.oscale[c6] = C(1);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) a.ahead[d] = nullptr;
     a.src = uni(p->src);
     a.vals = vals_row ? vals_row + uni(p->src_off) : nullptr;
     a.nsrc = uni(p->nsrc);

@@ -181,24 +181,45 @@ std::vector<fdtd::PointSource> host_sources(const fdtd_grid* g) {
     return all;
 }
 
-// Push the combined source list to the device when it changed.
+// Push the combined source list to the device when it changed: copied into
+// one of two pinned staging buffers and uploaded on the grid's stream with
+// no host wait (the drivers change J every step).  The other device buffer
+// may still be read by queued sweeps; a staging buffer is reused only after
+// its previous upload has completed.
 int flush_sources(fdtd_grid* g) {
     if (!g->src_dirty) return FDTD_OK;
     const std::vector<fdtd::PointSource> all = host_sources(g);
     if ((int)all.size() > g->d_src_cap) {
-        if (g->d_src) HIP_TRY(hipFree(g->d_src));
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        for (int b = 0; b < 2; ++b) {
+            if (g->d_src_buf[b]) HIP_TRY(hipFree(g->d_src_buf[b]));
+            if (g->h_src[b]) HIP_TRY(hipHostFree(g->h_src[b]));
+            g->d_src_buf[b] = nullptr;
+            g->h_src[b] = nullptr;
+            g->src_ev_used[b] = false;
+        }
         if (g->d_vals) HIP_TRY(hipFree(g->d_vals));
-        g->d_src = nullptr;
         g->d_vals = nullptr;
-        const int cap = std::max<int>(16, (int)all.size());
-        HIP_TRY(hipMalloc(&g->d_src, sizeof(fdtd::PointSource) * cap));
+        g->d_src = nullptr;
+        const int cap = std::max<int>(16, 2 * (int)all.size());
+        for (int b = 0; b < 2; ++b) {
+            HIP_TRY(hipMalloc(&g->d_src_buf[b], sizeof(fdtd::PointSource) * cap));
+            HIP_TRY(hipHostMalloc(&g->h_src[b], sizeof(fdtd::PointSource) * cap, hipHostMallocDefault));
+        }
         HIP_TRY(hipMalloc(&g->d_vals, sizeof(double) * cap * kValTable));
         g->d_src_cap = cap;
     }
-    if (!all.empty())
-        HIP_TRY(hipMemcpyAsync(g->d_src, all.data(), sizeof(fdtd::PointSource) * all.size(),
-                               hipMemcpyHostToDevice, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (!all.empty()) {
+        const int slot = g->src_slot = 1 - g->src_slot;
+        if (g->src_ev_used[slot]) HIP_TRY(hipEventSynchronize(g->src_ev[slot]));
+        std::memcpy(g->h_src[slot], all.data(), sizeof(fdtd::PointSource) * all.size());
+        HIP_TRY(hipMemcpyAsync(g->d_src_buf[slot], g->h_src[slot],
+                               sizeof(fdtd::PointSource) * all.size(), hipMemcpyHostToDevice,
+                               g->stream));
+        HIP_TRY(hipEventRecord(g->src_ev[slot], g->stream));
+        g->src_ev_used[slot] = true;
+        g->d_src = g->d_src_buf[slot];
+    }
     g->nsrc = (int)all.size();
     g->src_dirty = false;
     return FDTD_OK;
@@ -391,9 +412,19 @@ void* split_ptr(fdtd_grid* g, int q) { return q < 6 ? g->ES[g->cur][q] : g->BS[q
 
 // ---- fused sweep --------------------------------------------------------------
 
+// One sweep's inputs and outputs beyond the buffer pair: FP16 rescaling
+// (oscale, maxbits), an input B other than B[in] (in_b: the carried half
+// step), and where a final sweep stores the carried half step (ahead).
+struct SweepExtra {
+    const double* oscale = nullptr;
+    unsigned* maxbits = nullptr;
+    void* const* in_b = nullptr;
+    void* const* ahead = nullptr;
+};
+
 template <typename S, int V, bool STEADY, bool AUTO>
 int launch_fused_a(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
-                   const double* oscale, unsigned* maxbits) {
+                   const SweepExtra& x) {
     using C = typename fdtd::ComputeOf<S>::type;
     using JT = C;
     auto kern = g->pml ? fdtd::fused_pml_sweep<S, C, JT, V, STEADY, AUTO>
@@ -401,8 +432,11 @@ int launch_fused_a(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
     fdtd::SweepArgs<S, C, JT> a;
     a.in = fields_of<S>(g, in);
     a.out = fields_of<S>(g, out);
-    for (int d = 0; d < 3; ++d)
+    for (int d = 0; d < 3; ++d) {
+        if (x.in_b) a.in.b[d] = static_cast<S*>(x.in_b[d]);
+        a.ahead[d] = x.ahead ? static_cast<S*>(x.ahead[d]) : nullptr;
         a.j[d] = g->j_dense_active ? static_cast<const JT*>(g->J[d]) : nullptr;
+    }
     a.src = g->d_src;
     a.vals = g->d_vals ? g->d_vals + (size_t)slot * g->nsrc : nullptr;
     a.nsrc = g->nsrc;
@@ -410,8 +444,8 @@ int launch_fused_a(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
     a.c = make_coefs<C>(g);
     a.g = make_geom(g, t);
     a.pml = make_pml<S, C>(g, in, out);
-    for (int c = 0; c < 6; ++c) a.oscale[c] = C(oscale ? oscale[c] : 1.0);
-    a.maxbits = maxbits;
+    for (int c = 0; c < 6; ++c) a.oscale[c] = C(x.oscale ? x.oscale[c] : 1.0);
+    a.maxbits = x.maxbits;
     const size_t lds = sweep_lds_bytes(g, t.TY);
     if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
     const int threads = (t.TY + 1) * (g->Ni / V);
@@ -423,44 +457,61 @@ int launch_fused_a(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
 // FP16 sweeps that rescale or track the max take the AUTO variant.
 template <typename S, int V, bool STEADY>
 int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
-                   const double* oscale, unsigned* maxbits) {
+                   const SweepExtra& x) {
     if constexpr (sizeof(S) == 2)
-        if (oscale || maxbits)
-            return launch_fused_a<S, V, STEADY, true>(g, t, slot, in, out, oscale, maxbits);
-    return launch_fused_a<S, V, STEADY, false>(g, t, slot, in, out, oscale, maxbits);
+        if (x.oscale || x.maxbits)
+            return launch_fused_a<S, V, STEADY, true>(g, t, slot, in, out, x);
+    return launch_fused_a<S, V, STEADY, false>(g, t, slot, in, out, x);
 }
 
 template <typename S, bool STEADY>
 int launch_fused_s(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
-                   const double* oscale, unsigned* maxbits) {
+                   const SweepExtra& x) {
     switch (t.V) {
-        case 1: return launch_fused_v<S, 1, STEADY>(g, t, slot, in, out, oscale, maxbits);
-        case 2: return launch_fused_v<S, 2, STEADY>(g, t, slot, in, out, oscale, maxbits);
-        case 4: return launch_fused_v<S, 4, STEADY>(g, t, slot, in, out, oscale, maxbits);
+        case 1: return launch_fused_v<S, 1, STEADY>(g, t, slot, in, out, x);
+        case 2: return launch_fused_v<S, 2, STEADY>(g, t, slot, in, out, x);
+        case 4: return launch_fused_v<S, 4, STEADY>(g, t, slot, in, out, x);
         case 8:
-            if constexpr (sizeof(S) <= 4)
-                return launch_fused_v<S, 8, STEADY>(g, t, slot, in, out, oscale, maxbits);
+            if constexpr (sizeof(S) <= 4) return launch_fused_v<S, 8, STEADY>(g, t, slot, in, out, x);
             [[fallthrough]];
         default: return fail(FDTD_ERR_UNSUPPORTED, "unsupported V=%d", t.V);
     }
 }
 
 // One fused sweep; its source values are row `slot` of the value table.
-// FP16 grids: `oscale` (may be null) rescales what the sweep stores, and
-// `maxbits` (may be null) receives the per-component max of it.
 int launch_fused(fdtd_grid* g, const Tuning& t, bool steady, int slot, int in, int out,
-                 const double* oscale = nullptr, unsigned* maxbits = nullptr) {
+                 const SweepExtra& x = SweepExtra()) {
     switch (g->precision) {
         case FDTD_FP64:
-            return steady ? launch_fused_s<double, true>(g, t, slot, in, out, oscale, maxbits)
-                          : launch_fused_s<double, false>(g, t, slot, in, out, oscale, maxbits);
+            return steady ? launch_fused_s<double, true>(g, t, slot, in, out, x)
+                          : launch_fused_s<double, false>(g, t, slot, in, out, x);
         case FDTD_FP32:
-            return steady ? launch_fused_s<float, true>(g, t, slot, in, out, oscale, maxbits)
-                          : launch_fused_s<float, false>(g, t, slot, in, out, oscale, maxbits);
+            return steady ? launch_fused_s<float, true>(g, t, slot, in, out, x)
+                          : launch_fused_s<float, false>(g, t, slot, in, out, x);
         default:
-            return steady ? launch_fused_s<_Float16, true>(g, t, slot, in, out, oscale, maxbits)
-                          : launch_fused_s<_Float16, false>(g, t, slot, in, out, oscale, maxbits);
+            return steady ? launch_fused_s<_Float16, true>(g, t, slot, in, out, x)
+                          : launch_fused_s<_Float16, false>(g, t, slot, in, out, x);
     }
+}
+
+// The carried half step's buffers (allocated on first use; false when the
+// device has no room, and the run simply keeps its B/2 prologue).
+bool ensure_carry(fdtd_grid* g) {
+    if (g->BA[0][0]) return true;
+    const size_t bytes = g->N * g->esize + 16;
+    for (int b = 0; b < 2; ++b)
+        for (int d = 0; d < 3; ++d)
+            if (hipMalloc(&g->BA[b][d], bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                for (int q = 0; q < 2; ++q)
+                    for (int e = 0; e < 3; ++e)
+                        if (g->BA[q][e]) {
+                            (void)hipFree(g->BA[q][e]);
+                            g->BA[q][e] = nullptr;
+                        }
+                return false;
+            }
+    return true;
 }
 
 int rescale_to_target(fdtd_grid* g, bool current_only);
@@ -570,9 +621,20 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
 // a run never rescales (a PML grid's final sweep leaves its B splits
 // untouched).
 int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
-    if (g->btune.L >= 1 && nsteps >= g->btune.L && !g->pml) return run_blocked(g, nsteps, mid);
-    int rc = plain_B(g);
+    if (g->btune.L >= 1 && nsteps >= g->btune.L && !g->pml) {
+        g->ba_cur = -1;
+        return run_blocked(g, nsteps, mid);
+    }
+    // A carried half step replaces the B/2 prologue (bit-identical: the
+    // final sweep of the previous run computed (B_n + d) + d from the same
+    // E_n); otherwise B/2 in place.
+    const int carried = g->ba_cur;
+    int rc = FDTD_OK;
+    if (carried < 0) rc = plain_B(g);
     if (rc) return rc;
+    const bool carry_out = g->carry && !g->pml && ensure_carry(g);
+    const int ahead_buf = carried < 0 ? 0 : 1 - carried;
+    g->ba_cur = -1;
     if (mid) HIP_TRY(hipEventRecord(mid, g->stream));
     const int iv = g->precision == FDTD_FP16 ? g->autoscale_interval : 0;
     for (int s = 0; s < nsteps; ++s) {
@@ -597,10 +659,15 @@ int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
         if (s % kValTable == 0)
             if ((rc = eval_sources(g, g->step + s, std::min(kValTable, nsteps - s)))) return rc;
         const int in = g->cur, out = 1 - g->cur;
-        rc = launch_fused(g, g->tune, s + 1 < nsteps, s % kValTable, in, out, decide ? f : nullptr,
-                          track ? g->d_max : nullptr);
+        SweepExtra x;
+        x.oscale = decide ? f : nullptr;
+        x.maxbits = track ? g->d_max : nullptr;
+        if (s == 0 && carried >= 0) x.in_b = g->BA[carried];
+        if (s + 1 == nsteps && carry_out) x.ahead = g->BA[ahead_buf];
+        rc = launch_fused(g, g->tune, s + 1 < nsteps, s % kValTable, in, out, x);
         if (rc) return rc;
         g->cur = out;
+        if (s + 1 == nsteps && carry_out) g->ba_cur = ahead_buf;
         if (decide) {
             bool changed = false;
             for (int c = 0; c < 6; ++c) {
@@ -616,6 +683,7 @@ int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
 }
 
 int run_3sweep(fdtd_grid* g, int nsteps) {
+    g->ba_cur = -1;
     for (int s = 0; s < nsteps; ++s) {
         int rc = plain_B(g, true);
         if (!rc) rc = plain_E(g, g->step);
@@ -792,6 +860,9 @@ int create_grid(const fdtd_parameters* p, double dt, double pml_percent, int pre
         return cleanup(fail(FDTD_ERR_HIP, "out of device memory"));
     for (auto& e : g->ev)
         if (hipEventCreate(&e) != hipSuccess) return cleanup(fail(FDTD_ERR_HIP, "hipEventCreate"));
+    for (auto& e : g->src_ev)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return cleanup(fail(FDTD_ERR_HIP, "hipEventCreate"));
     g->fused_ok = default_tuning(g, &g->tune);
     if (hipStreamSynchronize(g->stream) != hipSuccess)
         return cleanup(fail(FDTD_ERR_HIP, "initialisation failed"));
@@ -831,10 +902,17 @@ int fdtd_destroy(fdtd_grid* g) {
         if (g->BS[q]) (void)hipFree(g->BS[q]);
     }
     if (g->pml_tab) (void)hipFree(g->pml_tab);
+    for (int b = 0; b < 2; ++b)
+        for (int d = 0; d < 3; ++d)
+            if (g->BA[b][d]) (void)hipFree(g->BA[b][d]);
     for (int d = 0; d < 3; ++d)
         if (g->J[d]) (void)hipFree(g->J[d]);
     if (g->staging) (void)hipFree(g->staging);
-    if (g->d_src) (void)hipFree(g->d_src);
+    for (int b = 0; b < 2; ++b) {
+        if (g->d_src_buf[b]) (void)hipFree(g->d_src_buf[b]);
+        if (g->h_src[b]) (void)hipHostFree(g->h_src[b]);
+        if (g->src_ev[b]) (void)hipEventDestroy(g->src_ev[b]);
+    }
     if (g->d_vals) (void)hipFree(g->d_vals);
     if (g->d_part) (void)hipFree(g->d_part);
     if (g->d_out) (void)hipFree(g->d_out);
@@ -922,6 +1000,7 @@ int fdtd_upload(fdtd_grid* g, int c, const double* host, size_t count) {
         HIP_TRY(hipStreamSynchronize(g->stream));
         return FDTD_OK;
     }
+    g->ba_cur = -1;  // the carried half step no longer matches E, B
     void* dst = field_ptr(g, c);
     if (g->precision == FDTD_FP64) {
         HIP_TRY(hipMemcpyAsync(dst, host, g->N * 8, hipMemcpyHostToDevice, g->stream));
@@ -1221,6 +1300,7 @@ int fdtd_add_plane_wave(fdtd_grid* g, const int m[3], double amplitude, double p
     const double en = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
     if (!(en > 1e-12)) return fail(FDTD_ERR_INVALID_ARGUMENT, "polarization parallel to k");
     for (int d = 0; d < 3; ++d) e[d] = amplitude * e[d] / en;
+    g->ba_cur = -1;
     const double b[3] = {kh[1] * e[2] - kh[2] * e[1], kh[2] * e[0] - kh[0] * e[2],
                          kh[0] * e[1] - kh[1] * e[0]};
     const double* sc = g->scales;
@@ -1302,6 +1382,7 @@ int apply_scales(fdtd_grid* g, const double s[6], bool current_only) {
         const double f = s[c] / g->scales[c];
         if (f == 1.0) continue;
         changed = true;
+        g->ba_cur = -1;
         for (int b = 0; b < 2; ++b) {
             if (current_only && b != g->cur) continue;
             scale(c < 3 ? g->E[b][c] : g->B[b][c - 3], f);
@@ -1359,6 +1440,7 @@ int fdtd_set_autoscale_interval(fdtd_grid* g, int interval, int target_log2) {
 
 int fdtd_zero_fields(fdtd_grid* g) {
     if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    g->ba_cur = -1;
     const size_t bytes = g->N * g->esize + 16;
     for (int b = 0; b < 2; ++b)
         for (int d = 0; d < 3; ++d) {
@@ -1376,6 +1458,22 @@ int fdtd_zero_fields(fdtd_grid* g) {
     g->sparse_j.clear();
     g->step = 0;
     HIP_TRY(hipStreamSynchronize(g->stream));
+    return FDTD_OK;
+}
+
+int fdtd_set_step_carry(fdtd_grid* g, int enable) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    g->carry = enable != 0;
+    if (!g->carry) {
+        g->ba_cur = -1;
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        for (int b = 0; b < 2; ++b)
+            for (int d = 0; d < 3; ++d)
+                if (g->BA[b][d]) {
+                    HIP_TRY(hipFree(g->BA[b][d]));
+                    g->BA[b][d] = nullptr;
+                }
+    }
     return FDTD_OK;
 }
 

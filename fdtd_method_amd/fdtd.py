@@ -310,6 +310,11 @@ class FDTD:
         _lib.check(self._L.fdtd_get_time_blocking(self._h, out))
         return dict(L=out[0], V=out[1], lanes=out[2], TYg=out[3], kchunk=out[4])
 
+    def set_step_carry(self, enable: bool) -> None:
+        """Carry the next call's first B half step across update_fields calls
+        (on by default; bit-identical either way)."""
+        _lib.check(self._L.fdtd_set_step_carry(self._h, int(bool(enable))))
+
     def set_tuning(self, V=0, TY=0, kchunk=0) -> None:
         _lib.check(self._L.fdtd_set_tuning(self._h, int(V), int(TY), int(kchunk)))
 

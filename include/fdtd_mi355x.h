@@ -195,6 +195,14 @@ int fdtd_set_autoscale_interval(fdtd_grid* grid, int interval, int target_log2);
 /* Reset: E, B and J to zero, step counter to 0 (sources are kept). */
 int fdtd_zero_fields(fdtd_grid* grid);
 
+/* Carried half step (periodic grids, on by default): the final sweep of a
+ * fdtd_update_fields call also stores the next call's first B half step,
+ * (B_n + d) + d, in two extra B buffer sets allocated on first use, so
+ * consecutive calls -- the reference drivers' one-step loops -- run no B/2
+ * prologue.  Results are bit-identical either way; host writes to E or B
+ * drop the carried step.  enable = 0 frees the buffers. */
+int fdtd_set_step_carry(fdtd_grid* grid, int enable);
+
 /* Fused-sweep launch shape: V cells per lane along x, TY rows per tile,
  * kchunk planes per z chunk (0 = keep current).  fdtd_autotune times the
  * candidate shapes on scratch buffers and keeps the fastest. */
