@@ -78,12 +78,13 @@ def build_cpp_tests(verbose: bool = False) -> None:
 
 def build_oracle(verbose: bool = False) -> None:
     """Compile the CPU checker (oracle/liboracle.so) and, when the reference
-    checkout exists, oracle/_ref/libfdtd_ref.so."""
+    checkout exists, oracle/_ref/libfdtd_ref.so and the reference's own
+    sample driver (oracle/_ref/sample, sample_pml) for CPU timings."""
     odir = os.path.join(ROOT, "oracle")
     targets = ["oracle"]
     ref = os.environ.get("FDTD_REFERENCE", "/root/reference")
     if os.path.isdir(os.path.join(ref, "src", "FDTD")):
-        targets.append("ref")
+        targets += ["ref", "ref-sample"]
     cmd = ["make", "-B", "-C", odir, f"REFERENCE={ref}", *targets]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if verbose:
