@@ -80,6 +80,9 @@ struct fdtd_grid {
     // buffered so a new list never overwrites one that queued work reads.
     fdtd::PointSource* d_src_buf[2] = {};
     fdtd::PointSource* h_src[2] = {};
+    int* d_kstart = nullptr;  // per-plane source ranges of d_src (Nk+1), one per buffer
+    int* d_kst_buf[2] = {};
+    int* h_kst[2] = {};
     hipEvent_t src_ev[2] = {};
     bool src_ev_used[2] = {false, false};
     int src_slot = 0;

@@ -169,15 +169,26 @@ eval_sources(const PointSource* __restrict__ src, int nsrc, int nsteps,
     vals[t] = source_value(src[q], static_cast<double>(n + 1) * dt);
 }
 
-// Adds the J term of every source located in cells (ib..ib+V-1, j, k) of the
-// E components' partial sums t[d][v] = first curl term, as the reference's
-// (cur*J + curl term) -- kokkos_functors.h:81-89.
+// Read-only kernel inputs through the constant address space: uniform
+// reads become scalar loads instead of vector loads a wavefront waits on.
+template <typename T> using cptr = const __attribute__((address_space(4))) T*;
+template <typename T> __device__ __forceinline__ cptr<T> as_const(const T* p) {
+    return (cptr<T>)(p);
+}
+
+// Sources q0..q1-1 of the list (those of plane k when the list is indexed
+// by plane, else all of them): adds the J term of every source located in
+// cells (ib..ib+V-1, j, k) to the E components' partial sums t[d][v] =
+// first curl term, as the reference's (cur*J + curl term) --
+// kokkos_functors.h:81-89.
 template <typename C, int V>
 __device__ __forceinline__ void add_point_sources(C (&t)[3][V], const Coefs<C>& c,
-                                                  const PointSource* __restrict__ src,
-                                                  const double* __restrict__ vals, int nsrc,
+                                                  const PointSource* src_g,
+                                                  const double* vals_g, int q0, int q1,
                                                   int ib, int j, int k) {
-    for (int q = 0; q < nsrc; ++q) {
+    const cptr<PointSource> src = as_const(src_g);
+    const cptr<double> vals = as_const(vals_g);
+    for (int q = q0; q < q1; ++q) {
         const int sk = src[q].k, sj = src[q].j, si = src[q].i;
         if (sk != k || sj != j || si < ib || si >= ib + V) continue;
         const int comp = src[q].comp;
@@ -216,7 +227,7 @@ __device__ __forceinline__ void plain_E_cell(const Fields<S>& f, const JT* __res
         t[1][0] = c.jy * C(jy[idx]) + t[1][0];
         t[2][0] = c.jz * C(jz[idx]) + t[2][0];
     }
-    if (nsrc > 0) add_point_sources<C, 1>(t, c, src, vals, nsrc, i, j, k);
+    if (nsrc > 0) add_point_sources<C, 1>(t, c, src, vals, 0, nsrc, i, j, k);
     const C ex = C(f.e[0][idx]) + (t[0][0] - c.ex_by * (by - C(f.b[1][k_pred])));
     const C ey = C(f.e[1][idx]) + (t[1][0] - c.ey_bz * (bz - C(f.b[2][i_pred])));
     const C ez = C(f.e[2][idx]) + (t[2][0] - c.ez_bx * (bx - C(f.b[0][j_pred])));
@@ -427,6 +438,9 @@ struct SweepArgs {
     const double* vals;             // their J values for this step
     int nsrc;
     int pad;
+    // When set, the sources are sorted by plane and those of plane k are
+    // src[kstart[k]] .. src[kstart[k+1]-1] (Nk+1 entries).
+    const int* kstart;
     Coefs<C> c;
     SweepGeom g;
     // FP16 storage only: stored values are multiplied by oscale (per
@@ -633,7 +647,15 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                     for (int v = 0; v < V; ++v) t[d][v] = cj * jv[v] + t[d][v];
                 }
             }
-            if (a.nsrc > 0) add_point_sources<C, V>(t, c, a.src, a.vals, a.nsrc, ib, j, kw);
+            if (a.nsrc > 0) {
+                int q0 = 0, q1 = a.nsrc;
+                if (a.kstart != nullptr) {
+                    const cptr<int> ks = as_const(a.kstart);
+                    q0 = ks[kw];
+                    q1 = ks[kw + 1];
+                }
+                add_point_sources<C, V>(t, c, a.src, a.vals, q0, q1, ib, j, kw);
+            }
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 e1[0][v] = en[0][v] + (t[0][v] - c.ex_by * (bn[1][v] - bP[1][v]));
@@ -1016,7 +1038,7 @@ __device__ __forceinline__ void blocked_tile(const BlockedArgs<S, C>& a, int tx,
                     t[2][v] = c.ez_by * (bin[1][v] - by_im);
                 }
                 if (a.nsrc > 0)
-                    add_point_sources<C, V>(t, c, a.src, a.vals + l * a.nsrc, a.nsrc, gi0, gj,
+                    add_point_sources<C, V>(t, c, a.src, a.vals + l * a.nsrc, 0, a.nsrc, gi0, gj,
                                             wrap_mod(pe, Nk));
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
@@ -1205,6 +1227,7 @@ __device__ __forceinline__ SweepArgs<S, C, C> load_batch_grid(const BatchGrid<S,
         a.j[d] = nullptr;
     }
     a.maxbits = nullptr;
+    a.kstart = nullptr;
 #pragma unroll
     for (int c6 = 0; c6 < 6; ++c6) a.oscale[c6] = C(1);
 #pragma unroll

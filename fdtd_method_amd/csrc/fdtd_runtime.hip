@@ -186,9 +186,23 @@ std::vector<fdtd::PointSource> host_sources(const fdtd_grid* g) {
 // no host wait (the drivers change J every step).  The other device buffer
 // may still be read by queued sweeps; a staging buffer is reused only after
 // its previous upload has completed.
+// The list is sorted by plane (stably, so sources sharing a cell keep their
+// order) with a table of per-plane ranges: a sweep looks at the sources of
+// the plane it updates only.
 int flush_sources(fdtd_grid* g) {
     if (!g->src_dirty) return FDTD_OK;
-    const std::vector<fdtd::PointSource> all = host_sources(g);
+    std::vector<fdtd::PointSource> all = host_sources(g);
+    std::stable_sort(all.begin(), all.end(),
+                     [](const fdtd::PointSource& x, const fdtd::PointSource& y) { return x.k < y.k; });
+    std::vector<int> kstart(g->Nk + 1, 0);
+    for (const auto& q : all) kstart[q.k + 1] += 1;
+    for (int k = 0; k < g->Nk; ++k) kstart[k + 1] += kstart[k];
+    if (!g->d_kst_buf[0]) {
+        for (int b = 0; b < 2; ++b) {
+            HIP_TRY(hipMalloc(&g->d_kst_buf[b], sizeof(int) * (g->Nk + 1)));
+            HIP_TRY(hipHostMalloc(&g->h_kst[b], sizeof(int) * (g->Nk + 1), hipHostMallocDefault));
+        }
+    }
     if ((int)all.size() > g->d_src_cap) {
         HIP_TRY(hipStreamSynchronize(g->stream));
         for (int b = 0; b < 2; ++b) {
@@ -213,12 +227,16 @@ int flush_sources(fdtd_grid* g) {
         const int slot = g->src_slot = 1 - g->src_slot;
         if (g->src_ev_used[slot]) HIP_TRY(hipEventSynchronize(g->src_ev[slot]));
         std::memcpy(g->h_src[slot], all.data(), sizeof(fdtd::PointSource) * all.size());
+        std::memcpy(g->h_kst[slot], kstart.data(), sizeof(int) * kstart.size());
         HIP_TRY(hipMemcpyAsync(g->d_src_buf[slot], g->h_src[slot],
                                sizeof(fdtd::PointSource) * all.size(), hipMemcpyHostToDevice,
                                g->stream));
+        HIP_TRY(hipMemcpyAsync(g->d_kst_buf[slot], g->h_kst[slot], sizeof(int) * kstart.size(),
+                               hipMemcpyHostToDevice, g->stream));
         HIP_TRY(hipEventRecord(g->src_ev[slot], g->stream));
         g->src_ev_used[slot] = true;
         g->d_src = g->d_src_buf[slot];
+        g->d_kstart = g->d_kst_buf[slot];
     }
     g->nsrc = (int)all.size();
     g->src_dirty = false;
@@ -441,6 +459,7 @@ int launch_fused_a(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
     a.vals = g->d_vals ? g->d_vals + (size_t)slot * g->nsrc : nullptr;
     a.nsrc = g->nsrc;
     a.pad = 0;
+    a.kstart = g->nsrc > 0 ? g->d_kstart : nullptr;
     a.c = make_coefs<C>(g);
     a.g = make_geom(g, t);
     a.pml = make_pml<S, C>(g, in, out);
@@ -911,6 +930,8 @@ int fdtd_destroy(fdtd_grid* g) {
     for (int b = 0; b < 2; ++b) {
         if (g->d_src_buf[b]) (void)hipFree(g->d_src_buf[b]);
         if (g->h_src[b]) (void)hipHostFree(g->h_src[b]);
+        if (g->d_kst_buf[b]) (void)hipFree(g->d_kst_buf[b]);
+        if (g->h_kst[b]) (void)hipHostFree(g->h_kst[b]);
         if (g->src_ev[b]) (void)hipEventDestroy(g->src_ev[b]);
     }
     if (g->d_vals) (void)hipFree(g->d_vals);
