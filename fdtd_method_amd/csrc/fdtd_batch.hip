@@ -259,9 +259,11 @@ int prepare(fdtd_batch* b) {
         if (b->d_vals) HIP_TRY(hipFree(b->d_vals));
         b->d_src = nullptr;
         b->d_vals = nullptr;
-        b->src_cap = std::max<int>(64, (int)all.size());
-        HIP_TRY(hipMalloc(&b->d_src, sizeof(fdtd::PointSource) * b->src_cap));
-        HIP_TRY(hipMalloc(&b->d_vals, sizeof(double) * b->src_cap * kValTable));
+        b->src_cap = 0;  // until the new buffers exist
+        const int cap = std::max<int>(64, (int)all.size());
+        HIP_TRY(hipMalloc(&b->d_src, sizeof(fdtd::PointSource) * cap));
+        HIP_TRY(hipMalloc(&b->d_vals, sizeof(double) * cap * kValTable));
+        b->src_cap = cap;
         if (b->graph) {
             (void)hipGraphExecDestroy(b->graph);
             b->graph = nullptr;
@@ -279,6 +281,8 @@ int prepare(fdtd_batch* b) {
     }
     if (b->h_grids.size() != b->grid_desc_bytes) {
         if (b->d_grids) HIP_TRY(hipFree(b->d_grids));
+        b->d_grids = nullptr;
+        b->grid_desc_bytes = 0;
         HIP_TRY(hipMalloc(&b->d_grids, b->h_grids.size()));
         b->grid_desc_bytes = b->h_grids.size();
     }
@@ -533,17 +537,47 @@ int fdtd_batch_autotune(fdtd_batch* b, int reps, long long out[4], float* best_m
     if (!b || reps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
     if (int rc = check_steps(b)) return rc;
     if (int rc = prepare(b)) return rc;
-    // Time steady sweeps on scratch copies of every grid's state.
-    std::vector<void*> saved;
-    for (fdtd_grid* g : b->grids)
-        for (int c = 0; c < 6; ++c) {
-            void* p = nullptr;
-            const size_t bytes = g->N * g->esize;
-            HIP_TRY(hipMalloc(&p, bytes));
-            void* src = c < 3 ? g->E[g->cur][c] : g->B[g->cur][c - 3];
-            HIP_TRY(hipMemcpyAsync(p, src, bytes, hipMemcpyDeviceToDevice, b->stream));
-            saved.push_back(p);
+    // Steady sweeps are timed on the grids' own buffers; every grid's state
+    // is copied aside first and put back on every return path, and a failed
+    // run also restores the launch shape.
+    struct Snapshot {
+        fdtd_batch* b;
+        BatchShape shape;
+        std::vector<void*> saved;
+        bool done = false;
+        static void* field(fdtd_grid* g, int c) {
+            return c < 3 ? g->E[g->cur][c] : g->B[g->cur][c - 3];
         }
+        int save() {
+            for (fdtd_grid* g : b->grids)
+                for (int c = 0; c < 6; ++c) {
+                    void* p = nullptr;
+                    HIP_TRY(hipMalloc(&p, g->N * g->esize));
+                    saved.push_back(p);
+                    HIP_TRY(hipMemcpyAsync(p, field(g, c), g->N * g->esize,
+                                           hipMemcpyDeviceToDevice, b->stream));
+                }
+            return FDTD_OK;
+        }
+        int restore() {
+            done = true;
+            size_t k = 0;
+            for (fdtd_grid* g : b->grids)
+                for (int c = 0; c < 6 && k < saved.size(); ++c, ++k)
+                    HIP_TRY(hipMemcpyAsync(field(g, c), saved[k], g->N * g->esize,
+                                           hipMemcpyDeviceToDevice, b->stream));
+            HIP_TRY(hipStreamSynchronize(b->stream));
+            return FDTD_OK;
+        }
+        ~Snapshot() {
+            if (!done) {
+                (void)restore();
+                (void)apply_shape(b, shape);
+            }
+            for (void* p : saved) (void)hipFree(p);
+        }
+    } snap{b, b->shape};
+    if (int rc = snap.save()) return rc;
     const BatchShape orig = b->shape;
     BatchShape best = orig;
     float best_t = 1e30f;
@@ -575,15 +609,10 @@ int fdtd_batch_autotune(fdtd_batch* b, int reps, long long out[4], float* best_m
                     best = b->shape;
                 }
             }
-    size_t k = 0;
-    for (fdtd_grid* g : b->grids)
-        for (int c = 0; c < 6; ++c) {
-            void* dst = c < 3 ? g->E[g->cur][c] : g->B[g->cur][c - 3];
-            HIP_TRY(hipMemcpyAsync(dst, saved[k++], g->N * g->esize, hipMemcpyDeviceToDevice,
-                                   b->stream));
-        }
-    HIP_TRY(hipStreamSynchronize(b->stream));
-    for (void* p : saved) HIP_TRY(hipFree(p));
+    if (int rc = snap.restore()) {
+        (void)apply_shape(b, orig);
+        return rc;
+    }
     if (best_t >= 1e30f) {
         (void)apply_shape(b, orig);
         return fail(FDTD_ERR_UNSUPPORTED, "no batch launch shape fits");

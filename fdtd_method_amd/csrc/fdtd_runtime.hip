@@ -197,11 +197,10 @@ int flush_sources(fdtd_grid* g) {
     std::vector<int> kstart(g->Nk + 1, 0);
     for (const auto& q : all) kstart[q.k + 1] += 1;
     for (int k = 0; k < g->Nk; ++k) kstart[k + 1] += kstart[k];
-    if (!g->d_kst_buf[0]) {
-        for (int b = 0; b < 2; ++b) {
-            HIP_TRY(hipMalloc(&g->d_kst_buf[b], sizeof(int) * (g->Nk + 1)));
+    for (int b = 0; b < 2; ++b) {  // per-plane tables: allocated once (each one left missing)
+        if (!g->d_kst_buf[b]) HIP_TRY(hipMalloc(&g->d_kst_buf[b], sizeof(int) * (g->Nk + 1)));
+        if (!g->h_kst[b])
             HIP_TRY(hipHostMalloc(&g->h_kst[b], sizeof(int) * (g->Nk + 1), hipHostMallocDefault));
-        }
     }
     if ((int)all.size() > g->d_src_cap) {
         HIP_TRY(hipStreamSynchronize(g->stream));
@@ -215,6 +214,7 @@ int flush_sources(fdtd_grid* g) {
         if (g->d_vals) HIP_TRY(hipFree(g->d_vals));
         g->d_vals = nullptr;
         g->d_src = nullptr;
+        g->d_src_cap = 0;  // until the new buffers exist
         const int cap = std::max<int>(16, 2 * (int)all.size());
         for (int b = 0; b < 2; ++b) {
             HIP_TRY(hipMalloc(&g->d_src_buf[b], sizeof(fdtd::PointSource) * cap));
@@ -427,6 +427,50 @@ int alloc_pml(fdtd_grid* g) {
 int split_component(int q) { return q < 6 ? q / 2 : 3 + (q - 6) / 2; }
 
 void* split_ptr(fdtd_grid* g, int q) { return q < 6 ? g->ES[g->cur][q] : g->BS[q - 6]; }
+
+void* field_ptr(fdtd_grid* g, int c);
+
+// Scratch copies of a grid's current state (E, B and a PML grid's split
+// fields) taken before tuning runs that sweep the grid's own buffers; the
+// state is put back by restore() or, on any early return, by the destructor.
+class StateSnapshot {
+public:
+    explicit StateSnapshot(fdtd_grid* g) : g_(g) {}
+    StateSnapshot(const StateSnapshot&) = delete;
+    StateSnapshot& operator=(const StateSnapshot&) = delete;
+    ~StateSnapshot() {
+        if (!restored_) (void)copy_back();
+        for (void* p : saved_) (void)hipFree(p);
+    }
+    int save() {
+        const int n = g_->pml ? 18 : 6;
+        for (int d = 0; d < n; ++d) {
+            void* p = nullptr;
+            HIP_TRY(hipMalloc(&p, bytes()));
+            saved_.push_back(p);
+            HIP_TRY(hipMemcpyAsync(p, state_ptr(d), bytes(), hipMemcpyDeviceToDevice, g_->stream));
+        }
+        return FDTD_OK;
+    }
+    int restore() {
+        restored_ = true;
+        return copy_back();
+    }
+
+private:
+    size_t bytes() const { return g_->N * g_->esize; }
+    void* state_ptr(int d) const { return d < 6 ? field_ptr(g_, d) : split_ptr(g_, d - 6); }
+    int copy_back() {
+        for (size_t d = 0; d < saved_.size(); ++d)
+            HIP_TRY(hipMemcpyAsync(state_ptr((int)d), saved_[d], bytes(), hipMemcpyDeviceToDevice,
+                                   g_->stream));
+        HIP_TRY(hipStreamSynchronize(g_->stream));
+        return FDTD_OK;
+    }
+    fdtd_grid* g_;
+    std::vector<void*> saved_;
+    bool restored_ = false;
+};
 
 // ---- fused sweep --------------------------------------------------------------
 
@@ -1551,16 +1595,11 @@ int fdtd_autotune_blocking(fdtd_grid* g, int reps, int out[5], float* best_ms) {
     if (!g || reps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
     if (g->pml) return fail(FDTD_ERR_UNSUPPORTED, "temporal blocking takes periodic grids (no PML layer)");
     if (int rc = flush_sources(g)) return rc;
-    // Candidate passes are timed on scratch copies of the state (restored
-    // afterwards); the full-row kernel's time per step is the reference.
-    std::vector<void*> saved;
-    const size_t bytes = g->N * g->esize;
-    for (int d = 0; d < 6; ++d) {
-        void* p = nullptr;
-        HIP_TRY(hipMalloc(&p, bytes));
-        HIP_TRY(hipMemcpyAsync(p, field_ptr(g, d), bytes, hipMemcpyDeviceToDevice, g->stream));
-        saved.push_back(p);
-    }
+    // Candidate passes are timed on the grid's buffers and the state is
+    // restored afterwards (on every exit path); the full-row kernel's time
+    // per step is the reference.
+    StateSnapshot snap(g);
+    if (int rc = snap.save()) return rc;
     if (int rc = eval_sources(g, g->step, 4)) return rc;
     auto time_passes = [&](bool blocked, float* ms_step) -> int {
         const int per = blocked ? g->btune.L : 1;
@@ -1595,7 +1634,10 @@ int fdtd_autotune_blocking(fdtd_grid* g, int reps, int out[5], float* best_ms) {
         for (const auto& sh : shapes) {
             if (fdtd_set_time_blocking(g, L, sh[0], sh[1], sh[2], 0) != FDTD_OK) continue;
             float t;
-            if (int rc = time_passes(true, &t)) return rc;
+            if (int rc = time_passes(true, &t)) {
+                g->btune = orig;
+                return rc;
+            }
             if (t < best_t) {
                 best_t = t;
                 best = g->btune;
@@ -1603,10 +1645,7 @@ int fdtd_autotune_blocking(fdtd_grid* g, int reps, int out[5], float* best_ms) {
         }
     g_last_error.clear();
     g->btune = best_t < 1e30f ? best : orig;
-    for (int d = 0; d < 6; ++d)
-        HIP_TRY(hipMemcpyAsync(field_ptr(g, d), saved[d], bytes, hipMemcpyDeviceToDevice, g->stream));
-    HIP_TRY(hipStreamSynchronize(g->stream));
-    for (void* p : saved) HIP_TRY(hipFree(p));
+    if (int rc = snap.restore()) return rc;
     if (best_t >= 1e30f) return fail(FDTD_ERR_UNSUPPORTED, "no launch shape fits this grid");
     if (out) fdtd_get_time_blocking(g, out);
     if (best_ms) *best_ms = best_t;
@@ -1658,17 +1697,9 @@ int fdtd_autotune(fdtd_grid* g, int reps, int out[3], float* best_ms) {
     if (int rc = flush_sources(g)) return rc;
     // Time sweeps between the two buffers of the grid; the fields (and PML
     // splits) are saved to and restored from scratch copies so tuning leaves
-    // the state intact.
-    std::vector<void*> saved;
-    const size_t bytes = g->N * g->esize;
-    const int nstate = g->pml ? 18 : 6;
-    auto state_ptr = [&](int d) { return d < 6 ? field_ptr(g, d) : split_ptr(g, d - 6); };
-    for (int d = 0; d < nstate; ++d) {
-        void* p = nullptr;
-        HIP_TRY(hipMalloc(&p, bytes));
-        HIP_TRY(hipMemcpyAsync(p, state_ptr(d), bytes, hipMemcpyDeviceToDevice, g->stream));
-        saved.push_back(p);
-    }
+    // the state intact, also when a launch fails.
+    StateSnapshot snap(g);
+    if (int rc = snap.save()) return rc;
     int vc[4];
     const int nv = v_candidates(g->precision, vc);
     Tuning best = g->tune;
@@ -1706,11 +1737,7 @@ int fdtd_autotune(fdtd_grid* g, int reps, int out[3], float* best_ms) {
             }
         }
     }
-    for (int d = 0; d < nstate; ++d) {
-        HIP_TRY(hipMemcpyAsync(state_ptr(d), saved[d], bytes, hipMemcpyDeviceToDevice, g->stream));
-    }
-    HIP_TRY(hipStreamSynchronize(g->stream));
-    for (void* p : saved) HIP_TRY(hipFree(p));
+    if (int rc = snap.restore()) return rc;
     if (best_t >= 1e30f) return fail(FDTD_ERR_UNSUPPORTED, "no fused launch shape fits this grid");
     g->tune = best;
     g->fused_ok = true;
@@ -1721,34 +1748,42 @@ int fdtd_autotune(fdtd_grid* g, int reps, int out[3], float* best_ms) {
 
 int fdtd_calibration_copy(size_t bytes, int iters, float* ms) {
     if (bytes == 0 || bytes % 16 || iters <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
-    void *a = nullptr, *b = nullptr;
-    HIP_TRY(hipMalloc(&a, bytes));
-    HIP_TRY(hipMalloc(&b, bytes));
-    hipStream_t s;
-    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    // Everything created here is released on every return path.
+    struct Resources {
+        void* a = nullptr;
+        void* b = nullptr;
+        hipStream_t s = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Resources() {
+            if (s) (void)hipStreamSynchronize(s);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (s) (void)hipStreamDestroy(s);
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+        }
+    } r;
+    HIP_TRY(hipMalloc(&r.a, bytes));
+    HIP_TRY(hipMalloc(&r.b, bytes));
+    HIP_TRY(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&r.e0));
+    HIP_TRY(hipEventCreate(&r.e1));
     const size_t n16 = bytes / 16;
-    hipLaunchKernelGGL(fdtd::zero_bytes, dim3(grid_blocks(n16)), dim3(256), 0, s,
-                       static_cast<uint4*>(a), n16);
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    hipLaunchKernelGGL(fdtd::zero_bytes, dim3(grid_blocks(n16)), dim3(256), 0, r.s,
+                       static_cast<uint4*>(r.a), n16);
     const int nb = 256 * 8 * 4;
-    hipLaunchKernelGGL(fdtd::copy_calibration, dim3(nb), dim3(256), 0, s,
-                       static_cast<const uint4*>(a), static_cast<uint4*>(b), n16);
-    HIP_TRY(hipEventRecord(e0, s));
-    for (int r = 0; r < iters; ++r)
-        hipLaunchKernelGGL(fdtd::copy_calibration, dim3(nb), dim3(256), 0, s,
-                           static_cast<const uint4*>(a), static_cast<uint4*>(b), n16);
-    HIP_TRY(hipEventRecord(e1, s));
-    HIP_TRY(hipEventSynchronize(e1));
+    hipLaunchKernelGGL(fdtd::copy_calibration, dim3(nb), dim3(256), 0, r.s,
+                       static_cast<const uint4*>(r.a), static_cast<uint4*>(r.b), n16);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(r.e0, r.s));
+    for (int it = 0; it < iters; ++it)
+        hipLaunchKernelGGL(fdtd::copy_calibration, dim3(nb), dim3(256), 0, r.s,
+                           static_cast<const uint4*>(r.a), static_cast<uint4*>(r.b), n16);
+    HIP_TRY(hipEventRecord(r.e1, r.s));
+    HIP_TRY(hipEventSynchronize(r.e1));
     float t = 0;
-    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    HIP_TRY(hipEventElapsedTime(&t, r.e0, r.e1));
     if (ms) *ms = t / iters;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(s);
-    (void)hipFree(a);
-    (void)hipFree(b);
     return FDTD_OK;
 }
 
