@@ -275,21 +275,34 @@ def bench_blocked(a):
     # blocked passes: short timing of candidate configurations, then the run
     g.zero_fields()
     best = None
-    for cand in [(4, 1, 32, 32), (3, 1, 32, 32), (4, 1, 64, 16), (3, 1, 64, 16), (2, 2, 64, 16),
-                 (2, 1, 32, 32), (4, 2, 32, 16)]:
+    # wavefront-strip kernel {L, V, R, W} and ghost-zone patch kernel {L, V, lanes, TYg}
+    cands = [("strips", (L, V, R, W)) for L in (4, 3, 2)
+             for V, R, W in ((2, 2, 8), (1, 2, 16), (1, 2, 8), (1, 4, 8))]
+    cands += [("patch", c) for c in ((4, 1, 32, 32), (3, 1, 32, 32), (3, 1, 64, 16))]
+
+    def apply(kind, c):
+        if kind == "strips":
+            g.set_time_blocking_strips(*c, 0)
+        else:
+            g.set_time_blocking(*c, 0)
+
+    def describe(kind):
+        return g.get_time_blocking_strips() if kind == "strips" else g.get_time_blocking()
+
+    for kind, cand in cands:
         try:
-            g.set_time_blocking(*cand, 0)
+            apply(kind, cand)
         except Exception:
             continue
         L = cand[0]
         g.time_update_fields(2 * L)
         _, m = g.time_update_fields(12 * L)
         m /= 12 * L
-        log(f"  blocking {g.get_time_blocking()}: {m:.3f} ms/step")
+        log(f"  {kind} {describe(kind)}: {m:.3f} ms/step")
         if best is None or m < best[0]:
-            best = (m, cand)
-    g.set_time_blocking(*best[1], 0)
-    cfg = g.get_time_blocking()
+            best = (m, kind, cand)
+    apply(best[1], best[2])
+    cfg = dict(kernel=best[1], **describe(best[1]))
     g.zero_fields()
     g.synchronize()
     t0 = time.perf_counter()

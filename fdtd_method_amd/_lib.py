@@ -31,7 +31,8 @@ EXPORTED = [
     "fdtd_checksum", "fdtd_add_plane_wave",
     "fdtd_set_scales", "fdtd_get_scales", "fdtd_autoscale", "fdtd_set_autoscale_interval",
     "fdtd_zero_fields", "fdtd_set_step_carry", "fdtd_set_time_blocking", "fdtd_get_time_blocking",
-    "fdtd_autotune_blocking", "fdtd_set_tuning",
+    "fdtd_autotune_blocking", "fdtd_set_time_blocking_strips", "fdtd_get_time_blocking_strips",
+    "fdtd_set_tuning",
     "fdtd_get_tuning", "fdtd_autotune", "fdtd_time_update_fields", "fdtd_calibration_copy",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_update_fields",
     "fdtd_batch_update_fields_graph", "fdtd_batch_synchronize", "fdtd_batch_time",
@@ -116,6 +117,8 @@ def load():
         "fdtd_set_time_blocking": ([_vp, _i, _i, _i, _i, _i], _i),
         "fdtd_get_time_blocking": ([_vp, ctypes.POINTER(_i)], _i),
         "fdtd_autotune_blocking": ([_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)], _i),
+        "fdtd_set_time_blocking_strips": ([_vp, _i, _i, _i, _i, _i], _i),
+        "fdtd_get_time_blocking_strips": ([_vp, ctypes.POINTER(_i)], _i),
         "fdtd_set_tuning": ([_vp, _i, _i, _i], _i),
         "fdtd_get_tuning": ([_vp, ctypes.POINTER(_i)], _i),
         "fdtd_autotune": ([_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)], _i),

@@ -44,6 +44,7 @@ struct Tuning {
 // z chunk.
 struct BlockTuning {
     int L = 0, V = 2, lanes = 64, TYg = 16, kchunk = 64;
+    int R = 0;  // > 0: wavefront-strip kernel, R rows per lane, TYg / R wavefronts
 };
 
 inline size_t storage_size(int precision) {

@@ -119,6 +119,19 @@ __device__ __forceinline__ void store_pack(S* __restrict__ p, const C (&in)[V]) 
     for (int v = 0; v < V; ++v) __builtin_nontemporal_store(static_cast<S>(in[v]), p + v);
 }
 
+// A field value as the storage type holds it.  Blocked levels keep their
+// intermediate fields in registers; with FP16 storage a level's output is
+// rounded where the next unblocked sweep would read it back from memory
+// (the next level's E and B inputs), while within a level the B update uses
+// E unrounded, as the fused sweep does.
+template <typename S, typename C>
+__device__ __forceinline__ C as_stored(C x) {
+    if constexpr (sizeof(S) < sizeof(C))
+        return static_cast<C>(static_cast<S>(x));
+    else
+        return x;
+}
+
 // Field pointers of one grid (current buffers, or in/out buffer pairs).
 template <typename S> struct Fields {
     S* e[3];
@@ -1069,6 +1082,8 @@ __device__ __forceinline__ void blocked_tile(const BlockedArgs<S, C>& a, int tx,
                         bnew[1][v] = bnew[1][v] + dby;
                         bnew[2][v] = bnew[2][v] + dbz;
                     }
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) bnew[d][v] = as_stored<S>(bnew[d][v]);
                 }
             }
             if (l < L - 1) {
@@ -1084,8 +1099,9 @@ __device__ __forceinline__ void blocked_tile(const BlockedArgs<S, C>& a, int tx,
                 for (int d = 0; d < 3; ++d)
 #pragma unroll
                     for (int v = 0; v < V; ++v) {
-                        // level l+1 inputs: E_l(pe-1) = eP[l] (previous iteration), B_l(pe-1)
-                        ein[d][v] = eP[l][d][v];
+                        // level l+1 inputs: E_l(pe-1) = eP[l] (previous iteration) as
+                        // it is stored, B_l(pe-1)
+                        ein[d][v] = as_stored<S>(eP[l][d][v]);
                         bP[l][d][v] = bin[d][v];   // level l's B input at pe: next iteration's previous plane
                         bin[d][v] = bnew[d][v];
                     }
@@ -1159,6 +1175,369 @@ blocked_sweep(BlockedArgs<S, C> a) {
     const int ty = rem / a.g.ntx;
     const int tx = rem - ty * a.g.ntx;
     blocked_tile<S, C, V, L, FINAL>(a, tx, ty, chunk, reinterpret_cast<C*>(lds_raw));
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront-strip temporally blocked sweep: L fused steps per pass over HBM.
+//
+// A workgroup of W wavefronts covers a tile of 64*V columns by W*R rows of
+// every plane.  Lane l of wavefront w owns columns l*V .. l*V+V-1 of tile
+// rows w*R .. w*R+R-1 and keeps them in registers, so of the x and y
+// neighbours a cell update reads:
+//   - x: the neighbouring lane's cell, moved by a DPP wavefront shift
+//     (wave_shr:1 / wave_shl:1) -- no LDS;
+//   - y: the lane's own next/previous row, except across a wavefront's
+//     first and last row, whose two needed components (Bx, Bz below; Ex, Ez
+//     above) go through a small LDS row exchange.
+// The outer Gx columns (L rounded up to a multiple of V) and L rows on each
+// side are ghost cells: computed on every level, never stored (overlapped
+// tiling), since after L levels only cells at least L from the tile edge
+// hold the values of L unblocked steps.  Along z the levels form the same
+// pipeline as blocked_sweep: at iteration kk level l computes E_l at plane
+// kk-l and B_l at plane kk-l-1, with the per-cell expressions of
+// fused_tile, so results are bit-identical to L single fused sweeps.
+//
+// LDS (per level l): sB[l] holds each wavefront's last-row Bx, Bz input of
+// level l (published before level l runs); sE[l] holds each wavefront's
+// first-row Ex, Ez of E_l one plane below (published one phase after the
+// reads of the previous value).  One barrier per level and plane (two for
+// L = 1).
+
+struct StripGeom {
+    int Ni, Nj, Nk;
+    int W;            // wavefronts per tile (R rows each)
+    int Gx;           // ghost columns per side: L rounded up to a multiple of V
+    int TX, TY;       // stored columns and rows per tile: 64V - 2Gx, W*R - 2L
+    int kchunk;
+    int ntx, nty;
+    int ntiles;
+    int per_xcd;
+};
+
+__host__ __device__ inline size_t strip_lds_bytes(int L, int W, int V, int csize) {
+    return static_cast<size_t>(4) * L * W * 64 * V * csize;
+}
+
+// Workgroup size bound of a strip variant: lanes holding more state get
+// fewer threads per workgroup, hence more registers.
+__host__ __device__ constexpr int strip_max_threads(int V, int R, int L, int csize) {
+    return (12 * L + 24) * V * R * csize / 4 > 288 ? 256
+         : ((12 * L + 24) * V * R * csize / 4 > 120 ? 512 : 1024);
+}
+
+template <typename S, typename C>
+struct StripArgs {
+    Fields<S> in;
+    Fields<S> out;
+    const PointSource* src;   // sorted by plane
+    const int* kstart;        // sources of plane k: [kstart[k], kstart[k+1])
+    const double* vals;       // source values of the pass's first step; step l at vals + l*nsrc
+    int nsrc;
+    int pad;
+    S* ahead[3];              // FINAL passes: also store the next half step (B + d) + d here
+    Coefs<C> c;
+    StripGeom g;
+};
+
+// Value of the same register in lane-1 (lane_prev) or lane+1 (lane_next) of
+// the wavefront; the end lane (0, resp. 63) reads 0.
+__device__ __forceinline__ int dpp_prev(int x) {
+    return __builtin_amdgcn_mov_dpp(x, 0x138 /* wave_shr:1 */, 0xF, 0xF, true);
+}
+__device__ __forceinline__ int dpp_next(int x) {
+    return __builtin_amdgcn_mov_dpp(x, 0x130 /* wave_shl:1 */, 0xF, 0xF, true);
+}
+__device__ __forceinline__ float lane_prev(float x) {
+    return __builtin_bit_cast(float, dpp_prev(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ float lane_next(float x) {
+    return __builtin_bit_cast(float, dpp_next(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double lane_prev(double x) {
+    int2 h = __builtin_bit_cast(int2, x);
+    h.x = dpp_prev(h.x);
+    h.y = dpp_prev(h.y);
+    return __builtin_bit_cast(double, h);
+}
+__device__ __forceinline__ double lane_next(double x) {
+    int2 h = __builtin_bit_cast(int2, x);
+    h.x = dpp_next(h.x);
+    h.y = dpp_next(h.y);
+    return __builtin_bit_cast(double, h);
+}
+
+template <typename S, typename C, int V, int R, int L, bool FINAL>
+__global__ void __launch_bounds__(strip_max_threads(V, R, L, sizeof(C)))
+strip_sweep(StripArgs<S, C> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    C* __restrict__ lds = reinterpret_cast<C*>(lds_raw);
+    const StripGeom& G = a.g;
+    const int blk = blockIdx.x;
+    const int tile = (blk & 7) * G.per_xcd + (blk >> 3);
+    if (tile >= G.ntiles) return;  // whole workgroup
+    const int txy = G.ntx * G.nty;
+    const int chunk = tile / txy;
+    const int rem = tile - chunk * txy;
+    const int ty = rem / G.ntx;
+    const int tx = rem - ty * G.ntx;
+
+    constexpr int RW = 64 * V;
+    const int Ni = G.Ni, Nj = G.Nj, Nk = G.Nk, W = G.W;
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int cx0 = lane * V;                          // tile column of the lane's first cell
+    const int px0 = tx * G.TX;                         // global column of the first stored column
+    const int gi0 = wrap_mod(px0 - G.Gx + cx0, Ni);    // V | Gx, V | TX: packs never straddle the wrap
+    const int py0 = ty * G.TY;
+    int gj[R], row_off[R];
+    bool row_out[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int tr = w * R + r;
+        gj[r] = wrap_mod(py0 - L + tr, Nj);
+        row_off[r] = gj[r] * Ni + gi0;
+        row_out[r] = tr >= L && tr < L + G.TY && py0 + tr - L < Nj;
+    }
+    bool col_out[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int cc = cx0 + v;
+        col_out[v] = cc >= G.Gx && cc < G.Gx + G.TX && px0 + cc - G.Gx < Ni;
+    }
+    const int k0 = chunk * G.kchunk;
+    const int k1 = min(k0 + G.kchunk, Nk);
+    const int plane = Ni * Nj;
+    // wavefront whose last row lies below / first row above this one's
+    // (the end wavefronts read their own: those rows are ghost rows)
+    const int wm = w > 0 ? w - 1 : w, wp = w + 1 < W ? w + 1 : w;
+    auto sB = [&](int l, int c, int ww) { return lds + ((l * 2 + c) * W + ww) * RW + cx0; };
+    auto sE = [&](int l, int c, int ww) { return lds + (((L + l) * 2 + c) * W + ww) * RW + cx0; };
+    const Coefs<C> c = a.c;
+    const cptr<int> kst = as_const(a.kstart);
+
+    C bP[L][3][R][V];  // level l: B input one plane below its E plane
+    C eP[L][3][R][V];  // level l: E output of the previous iteration
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    bP[l][d][r][v] = C(0);
+                    eP[l][d][r][v] = C(0);
+                }
+
+    const int kfirst = k0 - L + 1, klast = k1 + L - 1;
+    C en[3][R][V], bn[3][R][V];
+    {
+        const int offm = wrap_mod(kfirst - 1, Nk) * plane;
+        const int off = wrap_mod(kfirst, Nk) * plane;
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                load_pack<S, C, V>(a.in.b[d] + offm + row_off[r], bP[0][d][r]);
+                load_pack<S, C, V>(a.in.b[d] + off + row_off[r], bn[d][r]);
+                load_pack<S, C, V>(a.in.e[d] + off + row_off[r], en[d][r]);
+            }
+    }
+    // E rows are read (for ghost-plane results only) before their first write
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int v = 0; v < V; ++v) sE(l, q, w)[v] = C(0);
+
+    for (int kk = kfirst; kk <= klast; ++kk) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            sB(0, 0, w)[v] = bn[0][R - 1][v];
+            sB(0, 1, w)[v] = bn[2][R - 1][v];
+        }
+        RawPack<S, V> rawE[3][R], rawB[3][R];
+        const bool more = kk < klast;
+        if (more) {
+            const int off = wrap_mod(kk + 1, Nk) * plane;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    rawB[d][r] = load_raw<S, V>(a.in.b[d] + off + row_off[r]);
+                    rawE[d][r] = load_raw<S, V>(a.in.e[d] + off + row_off[r]);
+                }
+        }
+        __syncthreads();
+
+        C bin[3][R][V];  // B input of the current level at its E plane
+        C ein[3][R][V];  // E input of the current level
+        C na[3][R][V];   // the next half step (last level of a FINAL pass)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    bin[d][r][v] = bn[d][r][v];
+                    ein[d][r][v] = en[d][r][v];
+                }
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            const int pe = kk - l;  // this level's E plane; its B plane is pe - 1
+            if (L > 1) {
+                // publish the first-row Ex, Ez of the level whose readers
+                // have all passed the last barrier
+                const int lp = l == 0 ? L - 1 : l - 1;
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    sE(lp, 0, w)[v] = eP[lp][0][0][v];
+                    sE(lp, 1, w)[v] = eP[lp][2][0][v];
+                }
+            }
+            int q0 = 0, q1 = 0;
+            if (a.nsrc > 0) {
+                const int pk = wrap_mod(pe, Nk);
+                q0 = kst[pk];
+                q1 = kst[pk + 1];
+            }
+            C enew[3][R][V];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                C t[3][V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const C bz_jm = r > 0 ? bin[2][r - 1][v] : sB(l, 1, wm)[v];
+                    const C by_im = v > 0 ? bin[1][r][v - 1] : lane_prev(bin[1][r][V - 1]);
+                    t[0][v] = c.ex_bz * (bin[2][r][v] - bz_jm);
+                    t[1][v] = c.ey_bx * (bin[0][r][v] - bP[l][0][r][v]);
+                    t[2][v] = c.ez_by * (bin[1][r][v] - by_im);
+                }
+                if (q1 > q0)
+                    add_point_sources<C, V>(t, c, a.src, a.vals + l * a.nsrc, q0, q1, gi0, gj[r],
+                                            wrap_mod(pe, Nk));
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const C bx_jm = r > 0 ? bin[0][r - 1][v] : sB(l, 0, wm)[v];
+                    const C bz_im = v > 0 ? bin[2][r][v - 1] : lane_prev(bin[2][r][V - 1]);
+                    enew[0][r][v] = ein[0][r][v] + (t[0][v] - c.ex_by * (bin[1][r][v] - bP[l][1][r][v]));
+                    enew[1][r][v] = ein[1][r][v] + (t[1][v] - c.ey_bz * (bin[2][r][v] - bz_im));
+                    enew[2][r][v] = ein[2][r][v] + (t[2][v] - c.ez_bx * (bin[0][r][v] - bx_jm));
+                }
+            }
+            // B_l at plane pe-1 from E_l(pe-1) (eP) and E_l(pe) (enew)
+            C bnew[3][R][V];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const C ey_ip = v < V - 1 ? eP[l][1][r][v + 1] : lane_next(eP[l][1][r][0]);
+                    const C ez_ip = v < V - 1 ? eP[l][2][r][v + 1] : lane_next(eP[l][2][r][0]);
+                    const C ex_jp = r < R - 1 ? eP[l][0][r + 1][v] : sE(l, 0, wp)[v];
+                    const C ez_jp = r < R - 1 ? eP[l][2][r + 1][v] : sE(l, 1, wp)[v];
+                    const C dbx = c.bx_ey * (enew[1][r][v] - eP[l][1][r][v]) - c.bx_ez * (ez_jp - eP[l][2][r][v]);
+                    const C dby = c.by_ez * (ez_ip - eP[l][2][r][v]) - c.by_ex * (enew[0][r][v] - eP[l][0][r][v]);
+                    const C dbz = c.bz_ex * (ex_jp - eP[l][0][r][v]) - c.bz_ey * (ey_ip - eP[l][1][r][v]);
+                    bnew[0][r][v] = bP[l][0][r][v] + dbx;
+                    bnew[1][r][v] = bP[l][1][r][v] + dby;
+                    bnew[2][r][v] = bP[l][2][r][v] + dbz;
+                    if (!(FINAL && l == L - 1)) {
+                        bnew[0][r][v] = bnew[0][r][v] + dbx;
+                        bnew[1][r][v] = bnew[1][r][v] + dby;
+                        bnew[2][r][v] = bnew[2][r][v] + dbz;
+                    } else {
+                        na[0][r][v] = bnew[0][r][v] + dbx;
+                        na[1][r][v] = bnew[1][r][v] + dby;
+                        na[2][r][v] = bnew[2][r][v] + dbz;
+                    }
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) bnew[d][r][v] = as_stored<S>(bnew[d][r][v]);
+                }
+            // level l+1 inputs: E_l(pe-1) (previous iteration) and B_l(pe-1)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        ein[d][r][v] = as_stored<S>(eP[l][d][r][v]);
+                        eP[l][d][r][v] = enew[d][r][v];
+                        bP[l][d][r][v] = bin[d][r][v];
+                        bin[d][r][v] = bnew[d][r][v];
+                    }
+            if (l < L - 1) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    sB(l + 1, 0, w)[v] = bin[0][R - 1][v];
+                    sB(l + 1, 1, w)[v] = bin[2][R - 1][v];
+                }
+                __syncthreads();
+            }
+        }
+        // stores of the last level: E_{L-1} at plane kk-L+1, B_{L-1} at kk-L
+        {
+            const int pe = kk - L + 1;
+            if (pe >= k0 && pe < k1) {
+                const int off = pe * plane;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (row_out[r])
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            S* dst = a.out.e[d] + off + row_off[r];
+#pragma unroll
+                            for (int v = 0; v < V; ++v)
+                                if (col_out[v])
+                                    __builtin_nontemporal_store(static_cast<S>(eP[L - 1][d][r][v]), dst + v);
+                        }
+            }
+            if (pe - 1 >= k0 && pe - 1 < k1) {
+                const int off = (pe - 1) * plane;
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    if (row_out[r])
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            S* dst = a.out.b[d] + off + row_off[r];
+#pragma unroll
+                            for (int v = 0; v < V; ++v)
+                                if (col_out[v])
+                                    __builtin_nontemporal_store(static_cast<S>(bin[d][r][v]), dst + v);
+                        }
+                if (FINAL && a.ahead[0] != nullptr) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (row_out[r])
+#pragma unroll
+                            for (int d = 0; d < 3; ++d) {
+                                S* dst = a.ahead[d] + off + row_off[r];
+#pragma unroll
+                                for (int v = 0; v < V; ++v)
+                                    if (col_out[v])
+                                        __builtin_nontemporal_store(static_cast<S>(na[d][r][v]), dst + v);
+                            }
+                }
+            }
+        }
+        if (L == 1) {
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                sE(0, 0, w)[v] = eP[0][0][0][v];
+                sE(0, 1, w)[v] = eP[0][2][0][v];
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    widen<S, C, V>(rawB[d][r], bn[d][r]);
+                    widen<S, C, V>(rawE[d][r], en[d][r]);
+                }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -287,6 +287,41 @@ bool blocked_valid(const fdtd_grid* g, const BlockTuning& t) {
     return true;
 }
 
+fdtd::StripGeom make_strip_geom(const fdtd_grid* g, const BlockTuning& t) {
+    fdtd::StripGeom G;
+    G.Ni = g->Ni;
+    G.Nj = g->Nj;
+    G.Nk = g->Nk;
+    G.W = t.TYg / t.R;
+    G.Gx = (t.L + t.V - 1) / t.V * t.V;
+    G.TX = 64 * t.V - 2 * G.Gx;
+    G.TY = t.TYg - 2 * t.L;
+    G.kchunk = t.kchunk;
+    G.ntx = (g->Ni + G.TX - 1) / G.TX;
+    G.nty = (g->Nj + G.TY - 1) / G.TY;
+    const int nchunk = (g->Nk + t.kchunk - 1) / t.kchunk;
+    G.ntiles = G.ntx * G.nty * nchunk;
+    G.per_xcd = (G.ntiles + 7) / 8;
+    return G;
+}
+
+// Strip variants built: cells per lane V*R with V*R*sizeof(C) <= 16 bytes
+// (FP32/FP16: V=1 R=2, V=1 R=4, V=2 R=2; FP64: V=1 R=2).
+constexpr bool strip_built(int csize, int V, int R) {
+    return (V == 1 || V == 2) && (R == 2 || R == 4) && V * R * csize <= 16;
+}
+
+bool strip_valid(const fdtd_grid* g, const BlockTuning& t) {
+    const int cs = (int)compute_size(g->precision);
+    if (t.L < 1 || t.L > 4 || !strip_built(cs, t.V, t.R)) return false;
+    if (g->Ni % t.V || t.TYg <= 0 || t.TYg % t.R || t.kchunk <= 0) return false;
+    const int W = t.TYg / t.R;
+    if (W < 1 || W > 16 || 64 * W > fdtd::strip_max_threads(t.V, t.R, t.L, cs)) return false;
+    const int Gx = (t.L + t.V - 1) / t.V * t.V;
+    if (64 * t.V - 2 * Gx < 1 || t.TYg - 2 * t.L < 1) return false;
+    return fdtd::strip_lds_bytes(t.L, W, t.V, cs) <= kLdsLimit;
+}
+
 template <typename S>
 int launch_plain_B(fdtd_grid* g, bool layer) {
     using C = typename fdtd::ComputeOf<S>::type;
@@ -605,6 +640,140 @@ int launch_blocked_v(fdtd_grid* g, int slot, int in, int out) {
     return FDTD_OK;
 }
 
+template <typename S, int V, int R, int L, bool FINAL>
+int launch_strip_v(fdtd_grid* g, int slot, int in, int out, void* const* in_b, void* const* ahead) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    if constexpr (!strip_built(sizeof(C), V, R)) {
+        return fail(FDTD_ERR_UNSUPPORTED, "strip variant V=%d R=%d not built", V, R);
+    } else {
+        auto kern = fdtd::strip_sweep<S, C, V, R, L, FINAL>;
+        const BlockTuning& t = g->btune;
+        fdtd::StripArgs<S, C> a;
+        a.in = fields_of<S>(g, in);
+        if (in_b)
+            for (int d = 0; d < 3; ++d) a.in.b[d] = static_cast<S*>(in_b[d]);
+        a.out = fields_of<S>(g, out);
+        for (int d = 0; d < 3; ++d) a.ahead[d] = ahead ? static_cast<S*>(ahead[d]) : nullptr;
+        a.src = g->d_src;
+        a.kstart = g->d_kstart;
+        a.vals = g->d_vals ? g->d_vals + (size_t)slot * g->nsrc : nullptr;
+        a.nsrc = g->nsrc;
+        a.pad = 0;
+        a.c = make_coefs<C>(g);
+        a.g = make_strip_geom(g, t);
+        const size_t lds = fdtd::strip_lds_bytes(L, a.g.W, V, sizeof(C));
+        if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
+        hipLaunchKernelGGL(kern, dim3(8 * a.g.per_xcd), dim3(64 * a.g.W), lds, g->stream, a);
+        HIP_TRY(hipGetLastError());
+        return FDTD_OK;
+    }
+}
+
+// Workgroups of a strip variant resident per CU (0 if it cannot launch).
+template <typename S, int V, int R, int L>
+int strip_occupancy_v(int W) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    if constexpr (!strip_built(sizeof(C), V, R)) {
+        return 0;
+    } else {
+        auto kern = fdtd::strip_sweep<S, C, V, R, L, false>;
+        const size_t lds = fdtd::strip_lds_bytes(L, W, V, sizeof(C));
+        if (allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds) != FDTD_OK) return 0;
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, 64 * W, lds) != hipSuccess) {
+            (void)hipGetLastError();
+            return 0;
+        }
+        return n;
+    }
+}
+
+template <typename S, int V, int R>
+int strip_occupancy_l(int L, int W) {
+    switch (L) {
+        case 1: return strip_occupancy_v<S, V, R, 1>(W);
+        case 2: return strip_occupancy_v<S, V, R, 2>(W);
+        case 3: return strip_occupancy_v<S, V, R, 3>(W);
+        default: return strip_occupancy_v<S, V, R, 4>(W);
+    }
+}
+
+template <typename S>
+int strip_occupancy_s(const BlockTuning& t) {
+    const int W = t.TYg / t.R;
+    if (t.V == 1 && t.R == 2) return strip_occupancy_l<S, 1, 2>(t.L, W);
+    if (t.V == 1 && t.R == 4) return strip_occupancy_l<S, 1, 4>(t.L, W);
+    if (t.V == 2 && t.R == 2) return strip_occupancy_l<S, 2, 2>(t.L, W);
+    return 0;
+}
+
+int strip_occupancy(const fdtd_grid* g, const BlockTuning& t) {
+    switch (g->precision) {
+        case FDTD_FP64: return strip_occupancy_s<double>(t);
+        case FDTD_FP32: return strip_occupancy_s<float>(t);
+        default: return strip_occupancy_s<_Float16>(t);
+    }
+}
+
+int device_cus() {
+    static int cus = 0;
+    if (cus == 0) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+        else
+            cus = 256;
+    }
+    return cus;
+}
+
+// z chunk length of a strip pass: every tile costs kchunk + 2(L-1) planes
+// (the pipeline fills and drains over L-1 planes at each end), and the
+// tiles run in rounds of (resident workgroups per CU x CUs); the chunking
+// with the fewest plane-rounds wins (ties: fewer chunks).
+int strip_default_kchunk(const fdtd_grid* g, const BlockTuning& t) {
+    BlockTuning u = t;
+    u.kchunk = g->Nk;
+    const fdtd::StripGeom G = make_strip_geom(g, u);
+    const long long nxy = (long long)G.ntx * G.nty;
+    const long long slots = (long long)std::max(1, strip_occupancy(g, t)) * device_cus();
+    const int maxchunks = std::max(1, g->Nk / 16);
+    long long best_cost = -1;
+    int best = g->Nk;
+    for (int nchunk = 1; nchunk <= maxchunks; ++nchunk) {
+        const int kc = (g->Nk + nchunk - 1) / nchunk;
+        const int nc = (g->Nk + kc - 1) / kc;
+        const long long rounds = (nxy * nc + slots - 1) / slots;
+        const long long cost = rounds * (kc + 2 * (t.L - 1));
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = kc;
+        }
+    }
+    return best;
+}
+
+template <typename S, int V, int R, bool FINAL>
+int launch_strip_l(fdtd_grid* g, int slot, int in, int out, void* const* in_b, void* const* ahead) {
+    switch (g->btune.L) {
+        case 1: return launch_strip_v<S, V, R, 1, FINAL>(g, slot, in, out, in_b, ahead);
+        case 2: return launch_strip_v<S, V, R, 2, FINAL>(g, slot, in, out, in_b, ahead);
+        case 3: return launch_strip_v<S, V, R, 3, FINAL>(g, slot, in, out, in_b, ahead);
+        case 4: return launch_strip_v<S, V, R, 4, FINAL>(g, slot, in, out, in_b, ahead);
+        default: return fail(FDTD_ERR_UNSUPPORTED, "unsupported blocking depth %d", g->btune.L);
+    }
+}
+
+template <typename S, bool FINAL>
+int launch_strip_s(fdtd_grid* g, int slot, int in, int out, void* const* in_b, void* const* ahead) {
+    const int V = g->btune.V, R = g->btune.R;
+    if (V == 1 && R == 2) return launch_strip_l<S, 1, 2, FINAL>(g, slot, in, out, in_b, ahead);
+    if (V == 1 && R == 4) return launch_strip_l<S, 1, 4, FINAL>(g, slot, in, out, in_b, ahead);
+    if (V == 2 && R == 2) return launch_strip_l<S, 2, 2, FINAL>(g, slot, in, out, in_b, ahead);
+    return fail(FDTD_ERR_UNSUPPORTED, "unsupported strip shape V=%d R=%d", V, R);
+}
+
 template <typename S, int V, bool FINAL>
 int launch_blocked_l(fdtd_grid* g, int slot, int in, int out) {
     switch (g->btune.L) {
@@ -617,7 +786,9 @@ int launch_blocked_l(fdtd_grid* g, int slot, int in, int out) {
 }
 
 template <typename S, bool FINAL>
-int launch_blocked_s(fdtd_grid* g, int slot, int in, int out) {
+int launch_blocked_s(fdtd_grid* g, int slot, int in, int out, void* const* in_b,
+                     void* const* ahead) {
+    if (g->btune.R > 0) return launch_strip_s<S, FINAL>(g, slot, in, out, in_b, ahead);
     switch (g->btune.V) {
         case 1: return launch_blocked_l<S, 1, FINAL>(g, slot, in, out);
         case 2: return launch_blocked_l<S, 2, FINAL>(g, slot, in, out);
@@ -626,27 +797,38 @@ int launch_blocked_s(fdtd_grid* g, int slot, int in, int out) {
 }
 
 // One pass of btune.L fused steps; its source values are rows slot ..
-// slot+L-1 of the value table.
-int launch_blocked(fdtd_grid* g, bool final, int slot, int in, int out) {
+// slot+L-1 of the value table.  Strip passes may read their B input from
+// in_b (a carried half step) and, when final, also store the next half step
+// to ahead; the patch kernel takes neither.
+int launch_blocked(fdtd_grid* g, bool final, int slot, int in, int out,
+                   void* const* in_b = nullptr, void* const* ahead = nullptr) {
     switch (g->precision) {
         case FDTD_FP64:
-            return final ? launch_blocked_s<double, true>(g, slot, in, out)
-                         : launch_blocked_s<double, false>(g, slot, in, out);
+            return final ? launch_blocked_s<double, true>(g, slot, in, out, in_b, ahead)
+                         : launch_blocked_s<double, false>(g, slot, in, out, in_b, ahead);
         case FDTD_FP32:
-            return final ? launch_blocked_s<float, true>(g, slot, in, out)
-                         : launch_blocked_s<float, false>(g, slot, in, out);
+            return final ? launch_blocked_s<float, true>(g, slot, in, out, in_b, ahead)
+                         : launch_blocked_s<float, false>(g, slot, in, out, in_b, ahead);
         default:
-            return final ? launch_blocked_s<_Float16, true>(g, slot, in, out)
-                         : launch_blocked_s<_Float16, false>(g, slot, in, out);
+            return final ? launch_blocked_s<_Float16, true>(g, slot, in, out, in_b, ahead)
+                         : launch_blocked_s<_Float16, false>(g, slot, in, out, in_b, ahead);
     }
 }
 
-// n steps with temporal blocking: B/2, then n / L blocked passes, then the
-// remaining n % L steps as single fused sweeps.  Source tables are filled
-// in whole passes.
+// n steps with temporal blocking: B/2 (or, for strip passes, the carried
+// half step), then n / L blocked passes, then the remaining n % L steps as
+// single fused sweeps.  Source tables are filled in whole passes.  Strip
+// runs carry the next half step out of their last pass or sweep, as the
+// unblocked path does.
 int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
-    int rc = plain_B(g);
+    const bool strips = g->btune.R > 0;
+    const int carried = strips ? g->ba_cur : -1;
+    g->ba_cur = -1;
+    int rc = FDTD_OK;
+    if (carried < 0) rc = plain_B(g);
     if (rc) return rc;
+    const bool carry_out = strips && g->carry && ensure_carry(g);
+    const int ahead_buf = carried < 0 ? 0 : 1 - carried;
     if (mid) HIP_TRY(hipEventRecord(mid, g->stream));
     const int L = g->btune.L;
     const int npass = nsteps / L, rem = nsteps % L;
@@ -658,17 +840,23 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
         if (s % chunk == 0)
             if ((rc = eval_sources(g, g->step + s, std::min(chunk, npass * L - s)))) return rc;
         const int in = g->cur, out = 1 - g->cur;
-        rc = launch_blocked(g, rem == 0 && p + 1 == npass, s % chunk, in, out);
+        const bool last = rem == 0 && p + 1 == npass;
+        void* const* in_b = p == 0 && carried >= 0 ? g->BA[carried] : nullptr;
+        void* const* ahead = last && carry_out ? g->BA[ahead_buf] : nullptr;
+        rc = launch_blocked(g, last, s % chunk, in, out, in_b, ahead);
         if (rc) return rc;
         g->cur = out;
     }
     for (int q = 0; q < rem; ++q, ++s) {
         if ((rc = eval_sources(g, g->step + s, 1))) return rc;
         const int in = g->cur, out = 1 - g->cur;
-        rc = launch_fused(g, g->tune, q + 1 < rem, 0, in, out);
+        SweepExtra x;
+        if (q + 1 == rem && carry_out) x.ahead = g->BA[ahead_buf];
+        rc = launch_fused(g, g->tune, q + 1 < rem, 0, in, out, x);
         if (rc) return rc;
         g->cur = out;
     }
+    if (carry_out) g->ba_cur = ahead_buf;
     g->step += nsteps;
     return FDTD_OK;
 }
@@ -684,10 +872,7 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
 // a run never rescales (a PML grid's final sweep leaves its B splits
 // untouched).
 int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
-    if (g->btune.L >= 1 && nsteps >= g->btune.L && !g->pml) {
-        g->ba_cur = -1;
-        return run_blocked(g, nsteps, mid);
-    }
+    if (g->btune.L >= 1 && nsteps >= g->btune.L && !g->pml) return run_blocked(g, nsteps, mid);
     // A carried half step replaces the B/2 prologue (bit-identical: the
     // final sweep of the previous run computed (B_n + d) + d from the same
     // E_n); otherwise B/2 in place.
@@ -1591,6 +1776,30 @@ int fdtd_set_time_blocking(fdtd_grid* g, int L, int V, int lanes, int TYg, int k
     return FDTD_OK;
 }
 
+int fdtd_set_time_blocking_strips(fdtd_grid* g, int L, int V, int R, int W, int kchunk) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (L <= 0) {
+        g->btune.L = 0;
+        return FDTD_OK;
+    }
+    if (g->pml) return fail(FDTD_ERR_UNSUPPORTED, "temporal blocking takes periodic grids (no PML layer)");
+    BlockTuning t;
+    t.L = L;
+    t.V = V > 0 ? V : 1;
+    t.R = R > 0 ? R : 4;
+    t.lanes = 64;
+    const int cs = (int)compute_size(g->precision);
+    const int wmax = std::min(16, fdtd::strip_max_threads(t.V, t.R, L, cs) / 64);
+    t.TYg = (W > 0 ? W : std::min(8, wmax)) * t.R;
+    t.kchunk = 1;
+    if (!strip_valid(g, t))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "strip blocking L=%d V=%d R=%d W=%d invalid for this grid",
+                    t.L, t.V, t.R, t.TYg / t.R);
+    t.kchunk = kchunk > 0 ? kchunk : strip_default_kchunk(g, t);
+    g->btune = t;
+    return FDTD_OK;
+}
+
 int fdtd_autotune_blocking(fdtd_grid* g, int reps, int out[5], float* best_ms) {
     if (!g || reps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
     if (g->pml) return fail(FDTD_ERR_UNSUPPORTED, "temporal blocking takes periodic grids (no PML layer)");
@@ -1628,20 +1837,30 @@ int fdtd_autotune_blocking(fdtd_grid* g, int reps, int out[5], float* best_ms) {
         if (int rc = time_passes(false, &t)) return rc;
         best_t = t;
     }
+    auto consider = [&]() -> int {
+        float t;
+        if (int rc = time_passes(true, &t)) {
+            g->btune = orig;
+            return rc;
+        }
+        if (t < best_t) {
+            best_t = t;
+            best = g->btune;
+        }
+        return FDTD_OK;
+    };
     const int shapes[][3] = {{1, 32, 32}, {1, 64, 16}, {2, 64, 16}, {2, 32, 16}, {1, 64, 8},
                              {2, 64, 8}, {1, 32, 16}, {2, 32, 32}};
     for (int L = 1; L <= 4; ++L)
         for (const auto& sh : shapes) {
             if (fdtd_set_time_blocking(g, L, sh[0], sh[1], sh[2], 0) != FDTD_OK) continue;
-            float t;
-            if (int rc = time_passes(true, &t)) {
-                g->btune = orig;
-                return rc;
-            }
-            if (t < best_t) {
-                best_t = t;
-                best = g->btune;
-            }
+            if (int rc = consider()) return rc;
+        }
+    const int strips[][3] = {{2, 2, 8}, {1, 2, 16}, {1, 2, 8}, {1, 4, 8}, {2, 2, 4}, {1, 4, 4}};
+    for (int L = 1; L <= 4; ++L)
+        for (const auto& sh : strips) {
+            if (fdtd_set_time_blocking_strips(g, L, sh[0], sh[1], sh[2], 0) != FDTD_OK) continue;
+            if (int rc = consider()) return rc;
         }
     g_last_error.clear();
     g->btune = best_t < 1e30f ? best : orig;
@@ -1658,6 +1877,17 @@ int fdtd_get_time_blocking(const fdtd_grid* g, int out[5]) {
     out[1] = g->btune.V;
     out[2] = g->btune.lanes;
     out[3] = g->btune.TYg;
+    out[4] = g->btune.kchunk;
+    return FDTD_OK;
+}
+
+int fdtd_get_time_blocking_strips(const fdtd_grid* g, int out[5]) {
+    if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    const bool strips = g->btune.L > 0 && g->btune.R > 0;
+    out[0] = g->btune.L;
+    out[1] = g->btune.V;
+    out[2] = strips ? g->btune.R : 0;
+    out[3] = strips ? g->btune.TYg / g->btune.R : 0;
     out[4] = g->btune.kchunk;
     return FDTD_OK;
 }

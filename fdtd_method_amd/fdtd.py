@@ -298,12 +298,29 @@ class FDTD:
         _lib.check(self._L.fdtd_set_time_blocking(self._h, int(L), int(V), int(lanes), int(TYg),
                                                   int(kchunk)))
 
+    def set_time_blocking_strips(self, L: int, V: int = 0, R: int = 0, W: int = 0,
+                                 kchunk: int = 0) -> None:
+        """L steps per pass with the wavefront-strip kernel: tiles of 64*V
+        columns by W*R rows, R rows per lane (0 = defaults); bit-identical
+        results."""
+        _lib.check(self._L.fdtd_set_time_blocking_strips(self._h, int(L), int(V), int(R), int(W),
+                                                         int(kchunk)))
+
+    def get_time_blocking_strips(self):
+        """{L, V, R, W, kchunk}; R = W = 0 unless the strip kernel is set."""
+        out = (ctypes.c_int * 5)()
+        _lib.check(self._L.fdtd_get_time_blocking_strips(self._h, out))
+        return dict(L=out[0], V=out[1], R=out[2], W=out[3], kchunk=out[4])
+
     def autotune_blocking(self, reps: int = 3):
         """Fastest of the full-row sweep and tiled/blocked passes (ms per step)."""
         out = (ctypes.c_int * 5)()
         ms = ctypes.c_float()
         _lib.check(self._L.fdtd_autotune_blocking(self._h, int(reps), out, ctypes.byref(ms)))
-        return dict(L=out[0], V=out[1], lanes=out[2], TYg=out[3], kchunk=out[4]), ms.value
+        cfg = dict(L=out[0], V=out[1], lanes=out[2], TYg=out[3], kchunk=out[4])
+        strips = self.get_time_blocking_strips()
+        cfg.update(R=strips["R"], W=strips["W"])
+        return cfg, ms.value
 
     def get_time_blocking(self):
         out = (ctypes.c_int * 5)()

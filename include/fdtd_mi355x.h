@@ -223,6 +223,17 @@ int fdtd_set_time_blocking(fdtd_grid* grid, int L, int V, int lanes, int TYg, in
 int fdtd_get_time_blocking(const fdtd_grid* grid, int out[5]);
 int fdtd_autotune_blocking(fdtd_grid* grid, int reps, int out[5], float* best_ms);
 
+/* Temporal blocking with the wavefront-strip kernel: tiles of 64*V columns
+ * by W*R rows (one wavefront per R rows, R rows per lane in registers;
+ * x neighbours by lane shifts, y neighbours across wavefronts through a
+ * one-row LDS exchange), L = 1..4 steps per pass, kchunk planes per z
+ * chunk (0 = default).  Shapes built: V*R*sizeof(compute type) <= 16 bytes
+ * with V in {1,2}, R in {2,4} (FP32/FP16 compute in float, FP64 in double).
+ * Results are bit-identical to unblocked steps.  get: {L, V, R, W, kchunk},
+ * R = W = 0 when the ghost-zone patch kernel (or no blocking) is set. */
+int fdtd_set_time_blocking_strips(fdtd_grid* grid, int L, int V, int R, int W, int kchunk);
+int fdtd_get_time_blocking_strips(const fdtd_grid* grid, int out[5]);
+
 /* Timed run on the grid's stream with HIP events: ms_total covers the whole
  * call (including the B half step of the fused path), ms_main only the
  * nsteps sweep launches (fused) or step triples (3-sweep). */
