@@ -8,22 +8,30 @@ Jz point source at the centre (period 20 dt, envelope sigma 40 dt, peak at
 K steps (default 1000).  A step is one reference update_fields()
 (B/2, E, B/2) over the whole grid.
 
+Each precision runs K steps with the full-row fused sweep, then K steps
+with the configuration the blocking autotuner picks (the fused sweep,
+ghost-zone patch passes or wavefront-strip passes of L steps each; all
+bit-identical to single sweeps).
+
 Reported:
-  value         FP32 fused path, Gcell-updates/s (wall clock over K steps,
-                synchronised on both sides)
-  roofline      the FP32 fused sweep kernel: algorithmic 48 B/cell-update
-                (read and write E and B) / its average launch time from HIP
-                events on the grid's stream, against 8 TB/s; traffic = HBM
-                bytes per launch from the committed rocprofv3 counter runs
-  paths         FP64 fused, FP16-storage fused and the reference-structured
-                FP64 three-sweep path on the same workload
+  value         FP32, autotuned path, Gcell-updates/s (wall clock over K
+                steps, synchronised on both sides)
+  roofline      the kernel of that path.  Blocked passes: 48 B per
+                cell-update x L steps per launch / the average launch time
+                from HIP events on the grid's stream, an effective bandwidth
+                that can exceed 8 TB/s; traffic = measured HBM bytes per
+                launch (rocprofv3 counter runs committed under profiles/)
+  fused_sweep_roofline  the same for the single-step fused sweep
+  paths         FP32/FP64/FP16 autotuned and fused, and the
+                reference-structured FP64 three-sweep path
   errors_vs_fp64  relative L2 and max-abs error of FP32 and FP16 after K
                 steps against the FP64 run, per component
   cpu_baseline  the reference's own OpenMP class (compiled from its
                 sources) on this host's cores, bounded sample
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
-python bench.py --profile fp32|fp64|fp16   (only that path: for rocprofv3)
+python bench.py --profile fp32|fp64|fp16 [--tuning V,TY,kchunk] [--blocking L,V,R,W]
+    (only that path: for rocprofv3)
 
 Other workloads (not the headline; each prints one JSON line):
 python bench.py --workload batch [--grids 256] [--steps 500]
@@ -67,6 +75,7 @@ PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s + %HBM roofline, FP32/FP16 vs FP64 GPU at 512³, L2 err vs FP64"
 WORD = {Precision.FP64: 8, Precision.FP32: 4, Precision.FP16: 2}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_sweep_fp32_512_traffic.json")
+STRIP_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "strip_sweep_fp32_512_traffic.json")
 
 
 def log(*a):
@@ -106,7 +115,7 @@ def run_path(g, steps, warmup, path=_lib.PATH_FUSED):
     wall = time.perf_counter() - t0
     cells = float(g.cells)
     return dict(ms_per_step=1e3 * wall / steps, gcells=cells * steps / wall / 1e9,
-                event_ms_total=ms_total, launch_ms=ms_main / steps)
+                event_ms_total=ms_total, event_main_ms=ms_main, launch_ms=ms_main / steps)
 
 
 def errors(test, ref):
@@ -420,7 +429,9 @@ def bench_pml(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="steps (default: point 1000, batch 500, precision 2000, blocked 4000, "
+                         "pml 1000)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--three-sweep-steps", type=int, default=200)
@@ -428,6 +439,8 @@ def main():
     ap.add_argument("--profile", choices=["fp32", "fp64", "fp16"], default=None)
     ap.add_argument("--tuning", default=None,
                     help="V,TY,kchunk launch shape for --profile (default: autotune)")
+    ap.add_argument("--blocking", default=None,
+                    help="L,V,R,W[,kchunk] strip blocking for --profile (default: none)")
     ap.add_argument("--workload", choices=["point", "batch", "precision", "blocked", "pml"],
                     default="point")
     ap.add_argument("--pml", type=float, default=0.2, help="pml_percent of --workload pml")
@@ -439,6 +452,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): this build has no "
               "multi-GPU path; see the sharding design in DESIGN.md.", file=sys.stderr)
         sys.exit(2)
+    if a.steps is None:
+        a.steps = {"batch": 500, "precision": 2000, "blocked": 4000}.get(a.workload, 1000)
     n, K, W = a.n, a.steps, a.warmup
     if a.workload == "batch":
         return bench_batch(a)
@@ -454,58 +469,114 @@ def main():
         g = make_grid(n, prec, autotune=a.tuning is None)
         if a.tuning:
             g.set_tuning(*[int(x) for x in a.tuning.split(",")])
+        blk = None
+        if a.blocking:
+            g.set_time_blocking_strips(*[int(x) for x in a.blocking.split(",")])
+            blk = g.get_time_blocking_strips()
         r = run_path(g, K, W)
+        if blk:
+            r["pass_ms"] = r["event_main_ms"] / (K // blk["L"]) if K % blk["L"] == 0 else None
         nbytes = 1 << 30
         ms_copy = calibration_copy(nbytes, 5)
-        print(json.dumps(dict(profile=a.profile, n=n, steps=K, tuning=g.get_tuning(), **r,
+        print(json.dumps(dict(profile=a.profile, n=n, steps=K, tuning=g.get_tuning(),
+                              blocking=blk, **r,
                               calibration_copy=dict(bytes=nbytes, ms=ms_copy,
                                                     gbs=2 * nbytes / ms_copy / 1e6))),
               flush=True)
         return
 
     log(f"512^3 workload: n={n} steps={K} warmup={W}")
-    g32 = make_grid(n, Precision.FP32)
-    r32 = run_path(g32, K, W)
-    log(f"fp32 fused: {r32['gcells']:.1f} Gcell/s, launch {r32['launch_ms']:.3f} ms")
-    g64 = make_grid(n, Precision.FP64)
-    r64 = run_path(g64, K, W)
-    log(f"fp64 fused: {r64['gcells']:.1f} Gcell/s, launch {r64['launch_ms']:.3f} ms")
-    g16 = make_grid(n, Precision.FP16)
-    r16 = run_path(g16, K, W)
-    log(f"fp16 fused: {r16['gcells']:.1f} Gcell/s, launch {r16['launch_ms']:.3f} ms")
+    cells = float(n) ** 3
+    grids, fused, best, blocking = {}, {}, {}, {}
+    for prec in (Precision.FP32, Precision.FP64, Precision.FP16):
+        g = make_grid(n, prec)
+        fused[prec] = run_path(g, K, W)
+        log(f"{prec.name} fused: {fused[prec]['gcells']:.1f} Gcell/s, "
+            f"launch {fused[prec]['launch_ms']:.3f} ms")
+        cfg, ms = g.autotune_blocking(reps=3)
+        blocking[prec] = cfg
+        if cfg["L"] > 0:
+            best[prec] = run_path(g, K, W)
+            log(f"{prec.name} blocked {cfg}: {best[prec]['gcells']:.1f} Gcell/s")
+        else:
+            best[prec] = fused[prec]
+        grids[prec] = g
+    g32, g64, g16 = grids[Precision.FP32], grids[Precision.FP64], grids[Precision.FP16]
     e32 = errors(g32, g64)
     e16 = errors(g16, g64)
     en = dict(fp64=energy(g64), fp32=energy(g32), fp16=energy(g16))
     log(f"errors vs fp64: fp32 relL2={e32['rel_l2']:.3e}  fp16 relL2={e16['rel_l2']:.3e}")
+    tunes = {p: grids[p].get_tuning() for p in grids}
     g16.close()
-    g32_tune = g32.get_tuning()
     g32.close()
     # reference-structured FP64 path (three in-place sweeps) on the same grid
     K3 = min(K, a.three_sweep_steps)
+    g64.set_time_blocking(0)
     g64.zero_fields()
     r3 = run_path(g64, K3, 2, path=_lib.PATH_3SWEEP)
     log(f"fp64 3-sweep: {r3['gcells']:.1f} Gcell/s")
-    g64_tune = g64.get_tuning()
     g64.close()
 
-    cells = float(n) ** 3
-    achieved = 12 * WORD[Precision.FP32] * cells / (r32["launch_ms"] * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        t = json.load(open(TRAFFIC_FILE))
-        if t.get("n") == n:
-            traffic = t.get("hbm_bytes_per_launch")
+    def load_traffic(path, **match):
+        if not os.path.exists(path):
+            return None
+        t = json.load(open(path))
+        if any(t.get(k) != v for k, v in match.items()):
+            return None
+        return t.get("hbm_bytes_per_launch")
 
-    def path_rec(r, prec, tune=None, bytes_per_cell=None):
+    def fused_roofline(prec):
+        r = fused[prec]
+        word = WORD[prec]
+        gbs = 12 * word * cells / (r["launch_ms"] * 1e-3) / 1e9
+        sname = {Precision.FP32: "float", Precision.FP64: "double", Precision.FP16: "_Float16"}[prec]
+        cname = "double" if prec == Precision.FP64 else "float"
+        return {"bound": "hbm", "achieved": gbs, "peak": PEAK_GBS, "unit": "GB/s",
+                "frac": gbs / PEAK_GBS,
+                "traffic": load_traffic(TRAFFIC_FILE, n=n) if prec == Precision.FP32 else None,
+                "kernel": f"fdtd::fused_sweep<{sname},{cname},{sname},V,true>",
+                "alg_bytes_per_launch": 12.0 * word * cells, "launch_ms": r["launch_ms"]}
+
+    def blocked_roofline(prec):
+        """Temporally blocked passes: one launch advances L steps; the
+        per-step algorithmic figure (12 words per cell-update) is reported as
+        an effective bandwidth, which may exceed the HBM peak."""
+        cfg, r = blocking[prec], best[prec]
+        L = cfg["L"]
+        npass, rem = K // L, K % L
+        pass_ms = (r["event_main_ms"] - rem * fused[prec]["launch_ms"]) / npass
+        word = WORD[prec]
+        gbs = 12 * word * cells * L / (pass_ms * 1e-3) / 1e9
+        sname = {Precision.FP32: "float", Precision.FP64: "double", Precision.FP16: "_Float16"}[prec]
+        cname = "double" if prec == Precision.FP64 else "float"
+        if cfg.get("R", 0) > 0:
+            kern = f"fdtd::strip_sweep<{sname},{cname},{cfg['V']},{cfg['R']},{L},false>"
+        else:
+            kern = f"fdtd::blocked_sweep<{sname},{cname},{cfg['V']},{L},false>"
+        traffic = None
+        if prec == Precision.FP32:
+            traffic = load_traffic(STRIP_TRAFFIC_FILE, n=n, blocking=[L, cfg["V"], cfg.get("R", 0),
+                                                                    cfg.get("W", 0)])
+        return {"bound": "hbm", "effective": True, "achieved": gbs, "peak": PEAK_GBS,
+                "unit": "GB/s", "frac": gbs / PEAK_GBS, "traffic": traffic, "kernel": kern,
+                "steps_per_launch": L, "alg_bytes_per_launch": 12.0 * word * cells * L,
+                "launch_ms": pass_ms}
+
+    def path_rec(r, prec, tune=None, bytes_per_cell=None, cfg=None):
         bpc = bytes_per_cell or 12 * WORD[prec]
-        gbs = bpc * cells / (r["launch_ms"] * 1e-3) / 1e9
+        gbs = bpc * cells / (r["ms_per_step"] * 1e-3) / 1e9
         rec = dict(gcells=r["gcells"], ms_per_step=r["ms_per_step"], launch_ms=r["launch_ms"],
-                   alg_bytes_per_cell=bpc, alg_gbs=gbs, frac_of_peak=gbs / PEAK_GBS)
+                   alg_bytes_per_cell=bpc, alg_gbs_per_step=gbs, frac_of_peak=gbs / PEAK_GBS)
         if tune:
             rec["launch_shape"] = dict(V=tune[0], TY=tune[1], kchunk=tune[2])
+        if cfg:
+            rec["blocking"] = cfg
         return rec
 
+    blocked32 = blocking[Precision.FP32]["L"] > 0
+    r32 = best[Precision.FP32]
     cpu = None if a.no_cpu_baseline else cpu_baseline(n)
+    cfg32 = blocking[Precision.FP32]
     out = {
         "metric": METRIC,
         "value": r32["gcells"],
@@ -525,29 +596,30 @@ def main():
                          "(period 20dt, sigma 40dt, peak 120dt), zero initial fields, "
                          f"{K} steps"),
             "grid": [n, n, n],
-            "path": "fp32 storage and compute, fused E-then-double-half-B sweep",
-            "launch_shape": dict(V=g32_tune[0], TY=g32_tune[1], kchunk=g32_tune[2]),
+            "path": ("fp32 storage and compute, temporally blocked passes of "
+                     f"{cfg32['L']} fused steps (autotuned; bit-identical to single sweeps)"
+                     if blocked32 else "fp32 storage and compute, fused E-then-double-half-B sweep"),
+            "blocking": cfg32 if blocked32 else None,
+            "fused_launch_shape": dict(V=tunes[Precision.FP32][0], TY=tunes[Precision.FP32][1],
+                                       kchunk=tunes[Precision.FP32][2]),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / PEAK_GBS,
-            "traffic": traffic,
-            "kernel": "fdtd::fused_sweep<float,float,float,V,true>",
-            "alg_bytes_per_launch": 48.0 * cells,
-            "launch_ms": r32["launch_ms"],
-        },
+        "roofline": blocked_roofline(Precision.FP32) if blocked32 else fused_roofline(Precision.FP32),
+        "fused_sweep_roofline": fused_roofline(Precision.FP32),
         "cpu_baseline": cpu,
         "paths": {
-            "fp32_fused": path_rec(r32, Precision.FP32, g32_tune),
-            "fp64_fused": path_rec(r64, Precision.FP64, g64_tune),
-            "fp16_fused": path_rec(r16, Precision.FP16),
+            "fp32": path_rec(best[Precision.FP32], Precision.FP32, cfg=blocking[Precision.FP32]),
+            "fp64": path_rec(best[Precision.FP64], Precision.FP64, cfg=blocking[Precision.FP64]),
+            "fp16": path_rec(best[Precision.FP16], Precision.FP16, cfg=blocking[Precision.FP16]),
+            "fp32_fused": path_rec(fused[Precision.FP32], Precision.FP32, tunes[Precision.FP32]),
+            "fp64_fused": path_rec(fused[Precision.FP64], Precision.FP64, tunes[Precision.FP64]),
+            "fp16_fused": path_rec(fused[Precision.FP16], Precision.FP16, tunes[Precision.FP16]),
             "fp64_3sweep": path_rec(r3, Precision.FP64, bytes_per_cell=240),
         },
-        "speedup_fp32_vs_fp64": r32["gcells"] / r64["gcells"],
+        "speedup_fp32_vs_fp64": r32["gcells"] / best[Precision.FP64]["gcells"],
+        "speedup_fp32_fused_vs_fp64_fused": (fused[Precision.FP32]["gcells"]
+                                             / fused[Precision.FP64]["gcells"]),
         "speedup_fp32_vs_fp64_3sweep": r32["gcells"] / r3["gcells"],
+        "speedup_blocked_vs_fused_fp32": r32["gcells"] / fused[Precision.FP32]["gcells"],
         "errors_vs_fp64": {"fp32": e32, "fp16": e16},
         "energy": en,
     }

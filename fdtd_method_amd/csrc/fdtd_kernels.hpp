@@ -1373,7 +1373,9 @@ strip_sweep(StripArgs<S, C> a) {
 
         C bin[3][R][V];  // B input of the current level at its E plane
         C ein[3][R][V];  // E input of the current level
-        C na[3][R][V];   // the next half step (last level of a FINAL pass)
+        // FINAL passes with a carry buffer also store the next half step
+        // (B + d) + d of the last level's B plane kk-L, as it is computed
+        const bool carry_plane = FINAL && a.ahead[0] != nullptr && kk - L >= k0 && kk - L < k1;
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -1446,10 +1448,11 @@ strip_sweep(StripArgs<S, C> a) {
                         bnew[0][r][v] = bnew[0][r][v] + dbx;
                         bnew[1][r][v] = bnew[1][r][v] + dby;
                         bnew[2][r][v] = bnew[2][r][v] + dbz;
-                    } else {
-                        na[0][r][v] = bnew[0][r][v] + dbx;
-                        na[1][r][v] = bnew[1][r][v] + dby;
-                        na[2][r][v] = bnew[2][r][v] + dbz;
+                    } else if (carry_plane && row_out[r] && col_out[v]) {
+                        const int off = (kk - L) * plane + row_off[r] + v;
+                        __builtin_nontemporal_store(static_cast<S>(bnew[0][r][v] + dbx), a.ahead[0] + off);
+                        __builtin_nontemporal_store(static_cast<S>(bnew[1][r][v] + dby), a.ahead[1] + off);
+                        __builtin_nontemporal_store(static_cast<S>(bnew[2][r][v] + dbz), a.ahead[2] + off);
                     }
 #pragma unroll
                     for (int d = 0; d < 3; ++d) bnew[d][r][v] = as_stored<S>(bnew[d][r][v]);
@@ -1505,19 +1508,6 @@ strip_sweep(StripArgs<S, C> a) {
                                 if (col_out[v])
                                     __builtin_nontemporal_store(static_cast<S>(bin[d][r][v]), dst + v);
                         }
-                if (FINAL && a.ahead[0] != nullptr) {
-#pragma unroll
-                    for (int r = 0; r < R; ++r)
-                        if (row_out[r])
-#pragma unroll
-                            for (int d = 0; d < 3; ++d) {
-                                S* dst = a.ahead[d] + off + row_off[r];
-#pragma unroll
-                                for (int v = 0; v < V; ++v)
-                                    if (col_out[v])
-                                        __builtin_nontemporal_store(static_cast<S>(na[d][r][v]), dst + v);
-                            }
-                }
             }
         }
         if (L == 1) {
