@@ -1203,6 +1203,43 @@ blocked_sweep(BlockedArgs<S, C> a) {
 // reads of the previous value).  One barrier per level and plane (two for
 // L = 1).
 
+// p advanced by a 32-bit unsigned byte offset (zero-extended): lets the
+// compiler address memory as a scalar base plus a 32-bit lane offset.
+template <typename T>
+__device__ __forceinline__ T* at(T* p, unsigned byte_off) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<T*>(reinterpret_cast<B*>(p) + byte_off);
+}
+
+// Global-memory pointers built from wavefront-uniform addresses passed
+// through readfirstlane: the compiler keeps each plane's base in scalar
+// registers and addresses a row as that base plus a 32-bit lane offset,
+// instead of turning per-plane addresses into 64-bit per-lane pointers
+// stepped through the loop (two VGPRs per row and component).
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <typename T>
+__device__ __forceinline__ gptr<T> uniform_gptr(T* p) {
+    const unsigned long long u = reinterpret_cast<unsigned long long>(p);
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(u));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(u >> 32));
+    return reinterpret_cast<gptr<T>>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+template <typename T>
+__device__ __forceinline__ gptr<T> at(gptr<T> p, unsigned byte_off) {
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return reinterpret_cast<gptr<T>>(reinterpret_cast<gptr<B>>(p) + byte_off);
+}
+
+// Back to a generic pointer for the access (an address space cast the
+// compiler folds, keeping the global access).
+template <typename T>
+__device__ __forceinline__ T* generic(gptr<T> p) {
+    return (T*)p;
+}
+
 struct StripGeom {
     int Ni, Nj, Nk;
     int W;            // wavefronts per tile (R rows each)
@@ -1221,8 +1258,7 @@ __host__ __device__ inline size_t strip_lds_bytes(int L, int W, int V, int csize
 // Workgroup size bound of a strip variant: lanes holding more state get
 // fewer threads per workgroup, hence more registers.
 __host__ __device__ constexpr int strip_max_threads(int V, int R, int L, int csize) {
-    return (12 * L + 24) * V * R * csize / 4 > 288 ? 256
-         : ((12 * L + 24) * V * R * csize / 4 > 120 ? 512 : 1024);
+    return (12 * L + 24) * V * R * csize / 4 > 144 ? 512 : 1024;
 }
 
 template <typename S, typename C>
@@ -1289,13 +1325,17 @@ strip_sweep(StripArgs<S, C> a) {
     const int px0 = tx * G.TX;                         // global column of the first stored column
     const int gi0 = wrap_mod(px0 - G.Gx + cx0, Ni);    // V | Gx, V | TX: packs never straddle the wrap
     const int py0 = ty * G.TY;
-    int gj[R], row_off[R];
+    // Row offsets within a plane are unsigned 32-bit byte offsets (a plane
+    // is below 4 GiB), so every access is a uniform (scalar) plane base plus
+    // one 32-bit lane offset per row.
+    int gj[R];
+    unsigned row_off[R];
     bool row_out[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int tr = w * R + r;
         gj[r] = wrap_mod(py0 - L + tr, Nj);
-        row_off[r] = gj[r] * Ni + gi0;
+        row_off[r] = static_cast<unsigned>(gj[r] * Ni + gi0) * static_cast<unsigned>(sizeof(S));
         row_out[r] = tr >= L && tr < L + G.TY && py0 + tr - L < Nj;
     }
     bool col_out[V];
@@ -1332,16 +1372,20 @@ strip_sweep(StripArgs<S, C> a) {
     const int kfirst = k0 - L + 1, klast = k1 + L - 1;
     C en[3][R][V], bn[3][R][V];
     {
-        const int offm = wrap_mod(kfirst - 1, Nk) * plane;
-        const int off = wrap_mod(kfirst, Nk) * plane;
+        const size_t offm = static_cast<size_t>(wrap_mod(kfirst - 1, Nk)) * plane;
+        const size_t off = static_cast<size_t>(wrap_mod(kfirst, Nk)) * plane;
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
+        for (int d = 0; d < 3; ++d) {
+            const gptr<const S> bm = uniform_gptr<const S>(a.in.b[d] + offm);
+            const gptr<const S> b0 = uniform_gptr<const S>(a.in.b[d] + off);
+            const gptr<const S> e0 = uniform_gptr<const S>(a.in.e[d] + off);
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                load_pack<S, C, V>(a.in.b[d] + offm + row_off[r], bP[0][d][r]);
-                load_pack<S, C, V>(a.in.b[d] + off + row_off[r], bn[d][r]);
-                load_pack<S, C, V>(a.in.e[d] + off + row_off[r], en[d][r]);
+                load_pack<S, C, V>(generic(at(bm, row_off[r])), bP[0][d][r]);
+                load_pack<S, C, V>(generic(at(b0, row_off[r])), bn[d][r]);
+                load_pack<S, C, V>(generic(at(e0, row_off[r])), en[d][r]);
             }
+        }
     }
     // E rows are read (for ghost-plane results only) before their first write
 #pragma unroll
@@ -1360,14 +1404,17 @@ strip_sweep(StripArgs<S, C> a) {
         RawPack<S, V> rawE[3][R], rawB[3][R];
         const bool more = kk < klast;
         if (more) {
-            const int off = wrap_mod(kk + 1, Nk) * plane;
+            const size_t off = static_cast<size_t>(wrap_mod(kk + 1, Nk)) * plane;
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
+            for (int d = 0; d < 3; ++d) {
+                const gptr<const S> bb = uniform_gptr<const S>(a.in.b[d] + off);
+                const gptr<const S> eb = uniform_gptr<const S>(a.in.e[d] + off);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    rawB[d][r] = load_raw<S, V>(a.in.b[d] + off + row_off[r]);
-                    rawE[d][r] = load_raw<S, V>(a.in.e[d] + off + row_off[r]);
+                    rawB[d][r] = load_raw<S, V>(generic(at(bb, row_off[r])));
+                    rawE[d][r] = load_raw<S, V>(generic(at(eb, row_off[r])));
                 }
+            }
         }
         __syncthreads();
 
@@ -1449,10 +1496,10 @@ strip_sweep(StripArgs<S, C> a) {
                         bnew[1][r][v] = bnew[1][r][v] + dby;
                         bnew[2][r][v] = bnew[2][r][v] + dbz;
                     } else if (carry_plane && row_out[r] && col_out[v]) {
-                        const int off = (kk - L) * plane + row_off[r] + v;
-                        __builtin_nontemporal_store(static_cast<S>(bnew[0][r][v] + dbx), a.ahead[0] + off);
-                        __builtin_nontemporal_store(static_cast<S>(bnew[1][r][v] + dby), a.ahead[1] + off);
-                        __builtin_nontemporal_store(static_cast<S>(bnew[2][r][v] + dbz), a.ahead[2] + off);
+                        const size_t off = static_cast<size_t>(kk - L) * plane + v;
+                        __builtin_nontemporal_store(static_cast<S>(bnew[0][r][v] + dbx), generic(at(uniform_gptr(a.ahead[0] + off), row_off[r])));
+                        __builtin_nontemporal_store(static_cast<S>(bnew[1][r][v] + dby), generic(at(uniform_gptr(a.ahead[1] + off), row_off[r])));
+                        __builtin_nontemporal_store(static_cast<S>(bnew[2][r][v] + dbz), generic(at(uniform_gptr(a.ahead[2] + off), row_off[r])));
                     }
 #pragma unroll
                     for (int d = 0; d < 3; ++d) bnew[d][r][v] = as_stored<S>(bnew[d][r][v]);
@@ -1482,13 +1529,13 @@ strip_sweep(StripArgs<S, C> a) {
         {
             const int pe = kk - L + 1;
             if (pe >= k0 && pe < k1) {
-                const int off = pe * plane;
+                const size_t off = static_cast<size_t>(pe) * plane;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
                     if (row_out[r])
 #pragma unroll
                         for (int d = 0; d < 3; ++d) {
-                            S* dst = a.out.e[d] + off + row_off[r];
+                            S* dst = generic(at(uniform_gptr(a.out.e[d] + off), row_off[r]));
 #pragma unroll
                             for (int v = 0; v < V; ++v)
                                 if (col_out[v])
@@ -1496,13 +1543,13 @@ strip_sweep(StripArgs<S, C> a) {
                         }
             }
             if (pe - 1 >= k0 && pe - 1 < k1) {
-                const int off = (pe - 1) * plane;
+                const size_t off = static_cast<size_t>(pe - 1) * plane;
 #pragma unroll
                 for (int r = 0; r < R; ++r)
                     if (row_out[r])
 #pragma unroll
                         for (int d = 0; d < 3; ++d) {
-                            S* dst = a.out.b[d] + off + row_off[r];
+                            S* dst = generic(at(uniform_gptr(a.out.b[d] + off), row_off[r]));
 #pragma unroll
                             for (int v = 0; v < V; ++v)
                                 if (col_out[v])
