@@ -518,12 +518,16 @@ def main():
     g64.close()
 
     def load_traffic(path, **match):
+        """HBM bytes per launch from a committed counter summary whose
+        workload/configuration matches (a file may hold several
+        configurations under "configs")."""
         if not os.path.exists(path):
             return None
         t = json.load(open(path))
-        if any(t.get(k) != v for k, v in match.items()):
-            return None
-        return t.get("hbm_bytes_per_launch")
+        for rec in t.get("configs", [t]):
+            if all(rec.get(k) == v for k, v in match.items()):
+                return rec.get("hbm_bytes_per_launch")
+        return None
 
     def fused_roofline(prec):
         r = fused[prec]
