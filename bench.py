@@ -493,7 +493,7 @@ def main():
         fused[prec] = run_path(g, K, W)
         log(f"{prec.name} fused: {fused[prec]['gcells']:.1f} Gcell/s, "
             f"launch {fused[prec]['launch_ms']:.3f} ms")
-        cfg, ms = g.autotune_blocking(reps=3)
+        cfg, ms = g.autotune_blocking(reps=8)
         blocking[prec] = cfg
         if cfg["L"] > 0:
             best[prec] = run_path(g, K, W)
