@@ -873,7 +873,10 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
 // a run never rescales (a PML grid's final sweep leaves its B splits
 // untouched).
 int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
-    if (g->btune.L >= 1 && nsteps >= g->btune.L && !g->pml) return run_blocked(g, nsteps, mid);
+    // Blocked passes take point sources and sparse J values; a dense J array
+    // (uploaded J) runs single sweeps, which read it.
+    if (g->btune.L >= 1 && nsteps >= g->btune.L && !g->pml && !g->j_dense_active)
+        return run_blocked(g, nsteps, mid);
     // A carried half step replaces the B/2 prologue (bit-identical: the
     // final sweep of the previous run computed (B_n + d) + d from the same
     // E_n); otherwise B/2 in place.

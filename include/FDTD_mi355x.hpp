@@ -184,6 +184,21 @@ public:
         check_status(fdtd_zero_currents(grid_));
     }
 
+    // Temporal blocking (no counterpart in the reference): L steps per pass
+    // over HBM with the wavefront-strip kernel (0 = defaults; L = 0 turns it
+    // off), or the fastest of single sweeps and blocked passes timed on
+    // scratch copies of the state.  Bit-identical results either way.
+    void set_time_blocking_strips(int L, int V = 0, int R = 0, int W = 0, int kchunk = 0) {
+        check_status(fdtd_set_time_blocking_strips(grid_, L, V, R, W, kchunk));
+    }
+    float autotune_blocking(int reps = 3) {
+        flush();
+        int cfg[5];
+        float ms = 0.0f;
+        check_status(fdtd_autotune_blocking(grid_, reps, cfg, &ms));
+        return ms;
+    }
+
     std::size_t cells() const { return cells_; }
     fdtd_grid* handle() { return grid_; }
     const Parameters& parameters() const { return parameters_; }

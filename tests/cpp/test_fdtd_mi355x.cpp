@@ -209,6 +209,36 @@ void test_multi_step_equivalence() {
     EXPECT_TRUE(same);
 }
 
+void test_time_blocking() {
+    // Strip passes (4 steps each, plus remainders) and the blocking autotuner
+    // keep the fields bitwise those of single steps.
+    Parameters p{40, 24, 12, 0, 1, 0, 1, 0, 1, 1.0, 1.0, 1.0};
+    const FP dt = 0.5 / (FDTD_const::C * std::sqrt(3.0));
+    FDTD a(p, dt, g_precision), b(p, dt, g_precision), t(p, dt, g_precision);
+    FDTD* all[3] = {&a, &b, &t};
+    for (FDTD* g : all)
+        for (int c = 0; c < 6; ++c) {
+            Field& f = g->get_field(static_cast<Component>(c));
+            for (int i = 0; i < (int)g->cells(); ++i) f[i] = std::cos(0.11 * i + 2 * c);
+        }
+    b.set_time_blocking_strips(4, 1, 2, 8);
+    EXPECT_TRUE(t.autotune_blocking(2) > 0.0f);
+    for (int n : {9, 4, 2}) {
+        for (FDTD* g : all) {
+            g->get_field(Component::JY)[777] = 0.5 * n;
+            g->update_fields(n);
+        }
+    }
+    bool same = true;
+    for (int c = 0; c < 6; ++c)
+        for (int i = 0; i < (int)a.cells(); ++i) {
+            const double x = a.get_field(static_cast<Component>(c))[i];
+            same = same && x == (double)b.get_field(static_cast<Component>(c))[i] &&
+                   x == (double)t.get_field(static_cast<Component>(c))[i];
+        }
+    EXPECT_TRUE(same);
+}
+
 void test_pml_class() {
     // FDTD_PML (src/FDTD/FDTD_PML.cpp:205-365): layer widths, J ignored in
     // the layer, a pulse absorbed, pml_percent validated.
@@ -270,7 +300,8 @@ int main(int argc, char** argv) {
     test_currents();
     test_multi_step_equivalence();
     test_pml_class();
-    ran += 4;
+    test_time_blocking();
+    ran += 5;
     std::printf("%d tests, %d failures\n", ran, g_failures);
     return g_failures == 0 ? 0 : 1;
 }

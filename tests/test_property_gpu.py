@@ -1,8 +1,10 @@
 """Property-based parity (hypothesis, derandomized): random grid shapes,
 launch shapes, step counts, spacings, layer widths and current patterns, the
-FP64 fused path bit-identical to the oracle each time.  Complements the
-hand-picked cases of test_parity_gpu.py / test_pml_gpu.py with shapes nobody
-chose (ragged tiles, chunk boundaries on the last plane, one-cell axes)."""
+FP64 fused path bit-identical to the oracle each time; and random temporal
+blockings (strip passes) bit-identical to unblocked steps in every
+precision.  Complements the hand-picked cases of test_parity_gpu.py /
+test_pml_gpu.py / test_strip_gpu.py with shapes nobody chose (ragged tiles,
+chunk boundaries on the last plane, one-cell axes)."""
 import numpy as np
 import pytest
 from hypothesis import HealthCheck, given, settings, strategies as st
@@ -80,3 +82,56 @@ def test_random_periodic_grids_bit_exact(case):
 @given(cases(pml=True))
 def test_random_pml_grids_bit_exact(case):
     run_case(*case)
+
+
+@st.composite
+def strip_cases(draw):
+    Ni = draw(st.integers(1, 80))
+    Nj = draw(st.integers(1, 40))
+    Nk = draw(st.integers(1, 20))
+    prec = draw(st.sampled_from([Precision.FP64, Precision.FP32, Precision.FP16]))
+    L = draw(st.integers(1, 4))
+    V, R = draw(st.sampled_from([(1, 2)] if prec == Precision.FP64 else [(1, 2), (1, 4), (2, 2)]))
+    W = draw(st.integers(1, 16))
+    kchunk = draw(st.sampled_from([0, 1, 3, 7, 20]))
+    calls = draw(st.lists(st.integers(1, 9), min_size=1, max_size=3))
+    nsrc = draw(st.integers(0, 4))
+    seed = draw(st.integers(0, 2 ** 31 - 1))
+    return (Ni, Nj, Nk), prec, (L, V, R, W, kchunk), calls, nsrc, seed
+
+
+@SETTINGS
+@given(strip_cases())
+def test_random_strip_blocking_bit_exact(case):
+    """Temporally blocked strip passes on random shapes, precisions,
+    blockings, call sequences and point sources: bitwise the unblocked
+    fused steps (shapes the kernel rejects are skipped)."""
+    shape, prec, blocking, calls, nsrc, seed = case
+    n = int(np.prod(shape))
+    dt = courant_dt(1, 1, 1)
+    rng = np.random.default_rng(seed)
+    state = [rng.standard_normal(n) * 0.25 for _ in range(6)]
+    src = [dict(i=int(rng.integers(shape[0])), j=int(rng.integers(shape[1])),
+                k=int(rng.integers(shape[2])), comp=int(rng.integers(3)), kind=int(rng.integers(2)),
+                amp=float(rng.uniform(0.1, 1.0)), period=float(rng.uniform(8, 30)) * dt,
+                t0=0.0, sigma=0.0) for _ in range(nsrc)]
+    grids = []
+    for blocked in (False, True):
+        g = FDTD(Parameters(*shape), dt, prec)
+        for c in range(6):
+            g.upload(c, state[c])
+        if src:
+            g.set_point_sources(src)
+        grids.append(g)
+    try:
+        grids[1].set_time_blocking_strips(*blocking)
+    except _lib.FDTDError:
+        return  # not a valid strip shape for this grid
+    print(f"strip case shape={shape} prec={prec.name} blocking={grids[1].get_time_blocking_strips()} "
+          f"calls={calls} nsrc={nsrc} seed={seed}", flush=True)
+    for nsteps in calls:
+        for g in grids:
+            g.update_fields(nsteps)
+        assert grids[0].checksum() == grids[1].checksum(), nsteps
+    for g in grids:
+        g.close()

@@ -194,3 +194,29 @@ def test_strip_blocking_carries_the_half_step(prec):
         b.update_fields(nsteps)
     a.update_fields(13)
     assert a.checksum() == b.checksum()
+
+
+@pytest.mark.parametrize("kind", ["strips", "patch"])
+def test_blocking_with_dense_currents_runs_single_sweeps(kind):
+    """A dense J array (a whole J component uploaded) is read by single
+    sweeps: with blocking set, such runs still equal unblocked ones, and
+    blocking resumes once the dense J is dropped."""
+    shape = (32, 20, 12)
+    n = int(np.prod(shape))
+    dt = courant_dt(1, 1, 1)
+    st = random_state(n, 47)
+    J = np.random.default_rng(5).standard_normal(n) * 1e-3
+    a = grid(shape, dt, st, Precision.FP64)
+    b = grid(shape, dt, st, Precision.FP64)
+    if kind == "strips":
+        b.set_time_blocking_strips(2, 1, 2, 8, 0)
+    else:
+        b.set_time_blocking(2, 1, 32, 12, 0)
+    for g in (a, b):
+        g.upload(7, J)  # dense JY
+        g.update_fields(5)
+    assert a.checksum() == b.checksum()
+    for g in (a, b):
+        g.zeroed_currents()
+        g.update_fields(4)
+    assert a.checksum() == b.checksum()
