@@ -76,10 +76,24 @@ METRIC = "Gcell-updates/s + %HBM roofline, FP32/FP16 vs FP64 GPU at 512³, L2 er
 WORD = {Precision.FP64: 8, Precision.FP32: 4, Precision.FP16: 2}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_sweep_fp32_512_traffic.json")
 STRIP_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "strip_sweep_fp32_512_traffic.json")
+PML_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_pml_sweep_fp32_512_pml0.2_traffic.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def committed_traffic(path, **match):
+    """HBM bytes per launch from a committed counter summary under
+    profiles/ whose workload/configuration matches (a file may hold several
+    configurations under "configs"); None if there is none."""
+    if not os.path.exists(path):
+        return None
+    t = json.load(open(path))
+    for rec in t.get("configs", [t]):
+        if all(rec.get(k) == v for k, v in match.items()):
+            return rec.get("hbm_bytes_per_launch")
+    return None
 
 
 def make_grid(n, precision, autotune=True):
@@ -415,7 +429,9 @@ def bench_pml(a):
                                 f"centre, zero initial fields, {K} steps"),
                    "main_box_cells": main, "layer_cells": layer},
         "roofline": {"bound": "hbm", "achieved": r32["alg_gbs"], "peak": PEAK_GBS, "unit": "GB/s",
-                     "frac": r32["alg_gbs"] / PEAK_GBS, "traffic": None,
+                     "frac": r32["alg_gbs"] / PEAK_GBS,
+                     "traffic": committed_traffic(PML_TRAFFIC_FILE, n=n, pml=pml,
+                                                  tuning=list(r32["launch_shape"])),
                      "kernel": "fdtd::fused_pml_sweep<float,float,float,V,true>",
                      "alg_bytes_per_launch": nbytes, "launch_ms": r32["launch_ms"]},
         "cpu_baseline": cpu,
@@ -517,17 +533,6 @@ def main():
     log(f"fp64 3-sweep: {r3['gcells']:.1f} Gcell/s")
     g64.close()
 
-    def load_traffic(path, **match):
-        """HBM bytes per launch from a committed counter summary whose
-        workload/configuration matches (a file may hold several
-        configurations under "configs")."""
-        if not os.path.exists(path):
-            return None
-        t = json.load(open(path))
-        for rec in t.get("configs", [t]):
-            if all(rec.get(k) == v for k, v in match.items()):
-                return rec.get("hbm_bytes_per_launch")
-        return None
 
     def fused_roofline(prec):
         r = fused[prec]
@@ -537,7 +542,7 @@ def main():
         cname = "double" if prec == Precision.FP64 else "float"
         return {"bound": "hbm", "achieved": gbs, "peak": PEAK_GBS, "unit": "GB/s",
                 "frac": gbs / PEAK_GBS,
-                "traffic": load_traffic(TRAFFIC_FILE, n=n) if prec == Precision.FP32 else None,
+                "traffic": committed_traffic(TRAFFIC_FILE, n=n) if prec == Precision.FP32 else None,
                 "kernel": f"fdtd::fused_sweep<{sname},{cname},{sname},V,true>",
                 "alg_bytes_per_launch": 12.0 * word * cells, "launch_ms": r["launch_ms"]}
 
@@ -559,7 +564,7 @@ def main():
             kern = f"fdtd::blocked_sweep<{sname},{cname},{cfg['V']},{L},false>"
         traffic = None
         if prec == Precision.FP32:
-            traffic = load_traffic(STRIP_TRAFFIC_FILE, n=n, blocking=[L, cfg["V"], cfg.get("R", 0),
+            traffic = committed_traffic(STRIP_TRAFFIC_FILE, n=n, blocking=[L, cfg["V"], cfg.get("R", 0),
                                                                     cfg.get("W", 0)])
         return {"bound": "hbm", "effective": True, "achieved": gbs, "peak": PEAK_GBS,
                 "unit": "GB/s", "frac": gbs / PEAK_GBS, "traffic": traffic, "kernel": kern,

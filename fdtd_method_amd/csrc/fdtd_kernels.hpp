@@ -588,6 +588,41 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
         widen<S, C, V>(m0, bm);
     }
 
+    // PML sweeps: the split fields of iteration kn (E splits of its plane,
+    // B splits of plane kn-1 for steady sweeps), for lanes holding layer
+    // cells, loaded one iteration ahead as raw words so that their latency
+    // overlaps a whole iteration.  A chunk's B splits are updated only by its
+    // own workgroup, one plane behind these loads.
+    PK hsr[6], hbr[6];
+    auto load_splits = [&](int kn) {
+        if constexpr (PML) {
+            const Pml<S, C>& P = a.pml;
+            const int kwn = (kn == Nk) ? 0 : kn;
+            const int offn = kwn * plane + row_off;
+            const bool rowm = in_main(kwn, Nk, P.pk) && in_main(j, Nj, P.pj);
+            if (active && !(rowm && ib >= P.pi && ib + V <= Ni - P.pi)) {
+                hsr[kExy] = load_raw<S, V>(P.es_in[kExy] + offn);
+                hsr[kExz] = load_raw<S, V>(P.es_in[kExz] + offn);
+                hsr[kEzx] = load_raw<S, V>(P.es_in[kEzx] + offn);
+                hsr[kEzy] = load_raw<S, V>(P.es_in[kEzy] + offn);
+                if (own) {
+                    hsr[kEyx] = load_raw<S, V>(P.es_in[kEyx] + offn);
+                    hsr[kEyz] = load_raw<S, V>(P.es_in[kEyz] + offn);
+                }
+            }
+            if (STEADY && own && kn > k0) {
+                const int kb = kn - 1;
+                const bool rowb = in_main(kb, Nk, P.pk) && in_main(j, Nj, P.pj);
+                if (!(rowb && ib >= P.pi && ib + V <= Ni - P.pi)) {
+                    const int offb = kb * plane + row_off;
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) hbr[q] = load_raw<S, V>(P.bs[q] + offb);
+                }
+            }
+        }
+    };
+    if constexpr (PML) load_splits(k0);
+
     for (int kk = k0; kk <= k1; ++kk) {
         const int kw = (kk == Nk) ? 0 : kk;
         const int off = kw * plane + row_off;
@@ -609,33 +644,16 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
         PK en2[3], bn2[3], bm2;
         if (active && kk < k1) load_plane((kk + 1 == Nk) ? 0 : kk + 1, en2, bn2, bm2);
         // PML sweeps: the split fields this iteration updates (E splits of
-        // plane kw, B splits of plane kk-1), for lanes holding layer cells,
-        // are loaded here, before the E update, so that their latency
-        // overlaps it instead of stalling the workgroup at the next barrier.
+        // plane kw, B splits of plane kk-1), prefetched by the previous one.
         C hs[6][V];
         C hb[6][V];
         if constexpr (PML) {
-            const Pml<S, C>& P = a.pml;
-            const bool rowm = in_main(kw, Nk, P.pk) && in_main(j, Nj, P.pj);
-            if (active && !(rowm && ib >= P.pi && ib + V <= Ni - P.pi)) {
-                load_pack<S, C, V>(P.es_in[kExy] + off, hs[kExy]);
-                load_pack<S, C, V>(P.es_in[kExz] + off, hs[kExz]);
-                load_pack<S, C, V>(P.es_in[kEzx] + off, hs[kEzx]);
-                load_pack<S, C, V>(P.es_in[kEzy] + off, hs[kEzy]);
-                if (own) {
-                    load_pack<S, C, V>(P.es_in[kEyx] + off, hs[kEyx]);
-                    load_pack<S, C, V>(P.es_in[kEyz] + off, hs[kEyz]);
-                }
-            }
-            if (STEADY && own && kk > k0) {
-                const int kb = kk - 1;
-                const bool rowb = in_main(kb, Nk, P.pk) && in_main(j, Nj, P.pj);
-                if (!(rowb && ib >= P.pi && ib + V <= Ni - P.pi)) {
-                    const int offb = kb * plane + row_off;
 #pragma unroll
-                    for (int q = 0; q < 6; ++q) load_pack<S, C, V>(P.bs[q] + offb, hb[q]);
-                }
+            for (int q = 0; q < 6; ++q) {
+                widen<S, C, V>(hsr[q], hs[q]);
+                widen<S, C, V>(hbr[q], hb[q]);
             }
+            if (kk < k1) load_splits(kk + 1);
         }
 
         C e1[3][V];
