@@ -243,15 +243,16 @@ def bench_precision(a):
             g.set_autoscale_interval(100, 12)
         g.set_point_sources([src])
         g.autotune(reps=3)
+        blocking, _ = g.autotune_blocking(reps=4)  # bit-identical either way
         e0 = energy(g)
         g.synchronize()
         t0 = time.perf_counter()
         ms_total, ms_main = g.time_update_fields(K)
         wall = time.perf_counter() - t0
         res[prec.name] = dict(gcells=float(n) ** 3 * K / wall / 1e9, ms_per_step=1e3 * wall / K,
-                              energy_start=e0, energy_end=energy(g))
+                              energy_start=e0, energy_end=energy(g), blocking=blocking)
         grids[prec.name] = g
-        log(f"{prec.name}: {res[prec.name]['gcells']:.1f} Gcell/s")
+        log(f"{prec.name}: {res[prec.name]['gcells']:.1f} Gcell/s, blocking {blocking}")
     e32 = errors(grids["FP32"], grids["FP64"])
     e16 = errors(grids["FP16"], grids["FP64"])
     e64 = res["FP64"]["energy_end"]
@@ -449,7 +450,8 @@ def main():
                     help="steps (default: point 1000, batch 500, precision 2000, blocked 4000, "
                          "pml 1000)")
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--n", type=int, default=None,
+                    help="grid edge (default: precision 768, otherwise 512)")
     ap.add_argument("--three-sweep-steps", type=int, default=200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", choices=["fp32", "fp64", "fp16"], default=None)
@@ -470,6 +472,8 @@ def main():
         sys.exit(2)
     if a.steps is None:
         a.steps = {"batch": 500, "precision": 2000, "blocked": 4000}.get(a.workload, 1000)
+    if a.n is None:
+        a.n = 768 if a.workload == "precision" else 512
     n, K, W = a.n, a.steps, a.warmup
     if a.workload == "batch":
         return bench_batch(a)
