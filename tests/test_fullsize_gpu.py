@@ -1,0 +1,128 @@
+"""Size-independent properties at the benchmark's full sizes.
+
+The oracle finishes small grids in seconds; at the benchmark sizes (512^3,
+the headline workload; 1024x1024x256, the temporal-blocking workload) the
+HIP path is checked through properties the Yee step has exactly, whatever
+the size:
+
+* translation equivariance: on a periodic grid with uniform coefficients,
+  moving the point source by one cell moves every field by one cell,
+  bitwise;
+* linearity under a power-of-two scale: doubling the source doubles every
+  field exactly in FP64 (all operations are linear, scaling by 2 is exact,
+  and the fields stay far from the subnormal range);
+* temporally blocked passes equal single sweeps bitwise, and FP32 single
+  sweeps equal the three-sweep structure bitwise.
+
+Planes are compared on the host (one plane is Ni*Nj values); whole grids
+by checksum.
+"""
+import numpy as np
+import pytest
+
+from fdtd_method_amd import FDTD, Component, Parameters, Precision
+from oracle.oracle import C
+
+pytestmark = pytest.mark.gpu
+
+DT = 0.5 / (C * np.sqrt(3.0))
+
+
+def headline_source(i, j, k, amp=1.0):
+    """The 512^3 workload's Gaussian-modulated Jz point source."""
+    return dict(i=i, j=j, k=k, comp=2, kind=1, amp=amp, period=20 * DT, t0=120 * DT,
+                sigma=40 * DT, phase=0.0)
+
+
+def make(shape, src, prec=Precision.FP32, blocking=None):
+    g = FDTD(Parameters(*shape), DT, prec)
+    g.set_point_sources([src])
+    if prec == Precision.FP16:
+        # fields start at zero: scales sized for one step's injection, as the
+        # benchmark does, so that FP16 holds the fields
+        g.autoscale(target_log2=10, magnitude_floor=4.0 * np.pi * DT * 1e-2)
+    if blocking:
+        g.set_time_blocking_strips(*blocking)
+    return g
+
+
+PLANES = (250, 256, 257, 300)
+COMPS = (Component.EX, Component.EZ, Component.BY)
+
+
+def test_translation_equivariance_512():
+    n = 512
+    steps = 130  # past the envelope's peak at 120 dt
+    a = make((n, n, n), headline_source(256, 256, 256), blocking=(4, 1, 2, 16, 0))
+    b = make((n, n, n), headline_source(257, 256, 256), blocking=(4, 1, 2, 16, 0))
+    print("blocking", a.get_time_blocking_strips(), flush=True)
+    a.update_fields(steps)
+    b.update_fields(steps)
+    for c in COMPS:
+        for k in PLANES:
+            pa = a.download_plane(c, k).reshape(n, n)  # [j, i]
+            pb = b.download_plane(c, k).reshape(n, n)
+            assert np.array_equal(np.roll(pa, 1, axis=1), pb), (c, k)
+    assert np.abs(a.download_plane(Component.EZ, 256)).max() > 0
+    a.close()
+    b.close()
+
+
+def test_linearity_512():
+    # FP64: the fields stay far above the subnormal range, where rounding
+    # would not commute with the doubling
+    n = 512
+    steps = 60
+    a = make((n, n, n), headline_source(256, 256, 256, amp=1.0), Precision.FP64, (3, 1, 2, 8, 0))
+    b = make((n, n, n), headline_source(256, 256, 256, amp=2.0), Precision.FP64, (3, 1, 2, 8, 0))
+    a.update_fields(steps)
+    b.update_fields(steps)
+    for c in COMPS:
+        for k in PLANES:
+            pa = a.download_plane(c, k)
+            pb = b.download_plane(c, k)
+            assert np.array_equal(2.0 * pa, pb), (c, k)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec,blocking", [(Precision.FP32, (4, 1, 2, 16, 0)),
+                                           (Precision.FP32, (3, 1, 4, 8, 0)),
+                                           (Precision.FP64, (3, 1, 2, 8, 0)),
+                                           (Precision.FP16, (4, 1, 2, 16, 0))])
+def test_blocked_equals_single_sweeps_512(prec, blocking):
+    n = 512
+    src = headline_source(256, 256, 256)
+    a = make((n, n, n), src, prec)
+    b = make((n, n, n), src, prec, blocking)
+    L = blocking[0]
+    for steps in (4 * L + 1, L):  # remainder sweeps, then whole passes with the carried step
+        a.update_fields(steps)
+        b.update_fields(steps)
+        assert a.checksum() == b.checksum(), steps
+    a.close()
+    b.close()
+
+
+def test_blocked_equals_single_sweeps_1024x1024x256():
+    shape = (1024, 1024, 256)
+    src = headline_source(300, 700, 100)
+    a = make(shape, src)
+    b = make(shape, src, blocking=(4, 1, 2, 16, 0))
+    a.update_fields(13)
+    b.update_fields(13)
+    assert a.checksum() == b.checksum()
+    a.close()
+    b.close()
+
+
+def test_fused_equals_three_sweeps_fp32_512():
+    n = 512
+    src = headline_source(256, 256, 256)
+    a = make((n, n, n), src)
+    b = make((n, n, n), src)
+    a.update_fields(7)
+    b.update_fields_3sweep(7)
+    assert a.checksum() == b.checksum()
+    a.close()
+    b.close()
