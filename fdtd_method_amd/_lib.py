@@ -37,6 +37,7 @@ EXPORTED = [
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_update_fields",
     "fdtd_batch_update_fields_graph", "fdtd_batch_synchronize", "fdtd_batch_time",
     "fdtd_batch_set_shape", "fdtd_batch_get_shape", "fdtd_batch_autotune",
+    "fdtd_batch_set_time_blocking_strips", "fdtd_batch_get_time_blocking_strips",
 ]
 
 
@@ -136,6 +137,8 @@ def load():
         "fdtd_batch_get_shape": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _i),
         "fdtd_batch_autotune": ([_vp, _i, ctypes.POINTER(ctypes.c_longlong),
                                  ctypes.POINTER(ctypes.c_float)], _i),
+        "fdtd_batch_set_time_blocking_strips": ([_vp, _i, _i, _i, _i], _i),
+        "fdtd_batch_get_time_blocking_strips": ([_vp, ctypes.POINTER(_i)], _i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -82,6 +82,18 @@ class Batch:
         _lib.check(self._L.fdtd_batch_autotune(self._h, int(reps), out, ctypes.byref(ms)))
         return dict(V=out[0], threads=out[1], tile_cells=out[2], work_items=out[3]), ms.value
 
+    def set_time_blocking_strips(self, L: int, V: int = 0, R: int = 0, W: int = 0) -> None:
+        """Runs of at least L steps advance in persistent strip passes of L
+        steps (0 picks V=1, R=2, W=16; L=0 turns it off); bit-identical to
+        single-grid runs."""
+        _lib.check(self._L.fdtd_batch_set_time_blocking_strips(self._h, int(L), int(V), int(R),
+                                                               int(W)))
+
+    def get_time_blocking_strips(self):
+        out = (ctypes.c_int * 5)()
+        _lib.check(self._L.fdtd_batch_get_time_blocking_strips(self._h, out))
+        return dict(L=out[0], V=out[1], R=out[2], W=out[3], work_items=out[4])
+
     @property
     def cells(self) -> int:
         return sum(g.cells for g in self.grids)

@@ -53,7 +53,20 @@ struct fdtd_batch {
     long long* d_step = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
+    int graph_swaps = 0;
     hipEvent_t ev[4] = {};
+    // temporal blocking: strip passes of sL steps (0: single sweeps only)
+    int sL = 0, sV = 1, sR = 2, sW = 8;
+    std::vector<fdtd::StripGeom> sgeom;
+    int2* d_swork = nullptr;
+    int snwork = 0;
+    int snblocks = 0;
+    size_t slds = 0;
+    void* d_sgrids = nullptr;
+    size_t sgrid_desc_bytes = 0;
+    std::vector<unsigned char> h_sgrids;
+    int* d_kstart = nullptr;
+    size_t kstart_cap = 0;
 };
 
 namespace {
@@ -137,6 +150,27 @@ void fill_descriptors(fdtd_batch* b, const std::vector<int>& src_off) {
     }
 }
 
+template <typename S>
+void fill_strip_descriptors(fdtd_batch* b, const std::vector<int>& src_off,
+                            const std::vector<size_t>& kst_off) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    b->h_sgrids.resize(sizeof(fdtd::BatchStripGrid<S, C>) * b->grids.size());
+    auto* d = reinterpret_cast<fdtd::BatchStripGrid<S, C>*>(b->h_sgrids.data());
+    for (size_t q = 0; q < b->grids.size(); ++q) {
+        fdtd_grid* g = b->grids[q];
+        fdtd::BatchStripGrid<S, C> bg;
+        bg.buf[0] = fields_of<S>(g, g->cur);
+        bg.buf[1] = fields_of<S>(g, 1 - g->cur);
+        bg.src = b->d_src ? b->d_src + src_off[q] : nullptr;
+        bg.kstart = b->d_kstart + kst_off[q];
+        bg.src_off = src_off[q];
+        bg.nsrc = src_off[q + 1] - src_off[q];
+        bg.c = make_coefs<C>(g);
+        bg.g = b->sgeom[q];
+        d[q] = bg;
+    }
+}
+
 template <typename S, int V, bool STEADY>
 int occupancy_v(const fdtd_batch* b, int nt, size_t lds, int* blocks) {
     using C = typename fdtd::ComputeOf<S>::type;
@@ -189,6 +223,189 @@ int launch_batch(fdtd_batch* b, bool steady, int parity, const double* vals_row)
             return steady ? launch_batch_s<_Float16, true>(b, parity, vals_row)
                           : launch_batch_s<_Float16, false>(b, parity, vals_row);
     }
+}
+
+// ---- strip passes --------------------------------------------------------
+
+constexpr bool batch_strip_built(int csize, int V, int R) {
+    return (V == 1 || V == 2) && (R == 2 || R == 4) && V * R * csize <= 16;
+}
+
+template <typename S, int V, int R, int L, bool FINAL>
+const void* batch_strip_kernel() {
+    using C = typename fdtd::ComputeOf<S>::type;
+    if constexpr (!batch_strip_built(sizeof(C), V, R)) {
+        return nullptr;
+    } else {
+        return reinterpret_cast<const void*>(fdtd::batch_strip_sweep<S, C, V, R, L, FINAL>);
+    }
+}
+
+template <typename S, int V, int R, int L, bool FINAL>
+int launch_batch_strip_v(fdtd_batch* b, int parity, const double* vals) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    if constexpr (!batch_strip_built(sizeof(C), V, R)) {
+        return fail(FDTD_ERR_UNSUPPORTED, "strip variant V=%d R=%d not built", V, R);
+    } else {
+        fdtd::BatchStripArgs<S, C> a;
+        a.grids = static_cast<const fdtd::BatchStripGrid<S, C>*>(b->d_sgrids);
+        a.work = b->d_swork;
+        a.nwork = b->snwork;
+        a.parity = parity;
+        a.vals = vals;
+        a.vstride = b->total_src;
+        a.pad = 0;
+        a.counters = b->d_counters;
+        hipLaunchKernelGGL((fdtd::batch_strip_sweep<S, C, V, R, L, FINAL>), dim3(b->snblocks),
+                           dim3(64 * b->sW), b->slds, b->stream, a);
+        HIP_TRY(hipGetLastError());
+        return FDTD_OK;
+    }
+}
+
+template <typename S, int V, int R, bool FINAL>
+int launch_batch_strip_l(fdtd_batch* b, int parity, const double* vals) {
+    switch (b->sL) {
+        case 1: return launch_batch_strip_v<S, V, R, 1, FINAL>(b, parity, vals);
+        case 2: return launch_batch_strip_v<S, V, R, 2, FINAL>(b, parity, vals);
+        case 3: return launch_batch_strip_v<S, V, R, 3, FINAL>(b, parity, vals);
+        default: return launch_batch_strip_v<S, V, R, 4, FINAL>(b, parity, vals);
+    }
+}
+
+template <typename S, bool FINAL>
+int launch_batch_strip_s(fdtd_batch* b, int parity, const double* vals) {
+    if (b->sV == 1 && b->sR == 2) return launch_batch_strip_l<S, 1, 2, FINAL>(b, parity, vals);
+    if (b->sV == 1 && b->sR == 4) return launch_batch_strip_l<S, 1, 4, FINAL>(b, parity, vals);
+    return launch_batch_strip_l<S, 2, 2, FINAL>(b, parity, vals);
+}
+
+// One strip pass of sL steps over every grid; source values are the rows
+// starting at vals (null: no sources).
+int launch_batch_strip(fdtd_batch* b, bool final, int parity, const double* vals) {
+    switch (b->precision) {
+        case FDTD_FP64:
+            return final ? launch_batch_strip_s<double, true>(b, parity, vals)
+                         : launch_batch_strip_s<double, false>(b, parity, vals);
+        case FDTD_FP32:
+            return final ? launch_batch_strip_s<float, true>(b, parity, vals)
+                         : launch_batch_strip_s<float, false>(b, parity, vals);
+        default:
+            return final ? launch_batch_strip_s<_Float16, true>(b, parity, vals)
+                         : launch_batch_strip_s<_Float16, false>(b, parity, vals);
+    }
+}
+
+template <typename S, int V, int R>
+const void* strip_kernel_l(int L, bool final) {
+    switch (L) {
+        case 1: return final ? batch_strip_kernel<S, V, R, 1, true>() : batch_strip_kernel<S, V, R, 1, false>();
+        case 2: return final ? batch_strip_kernel<S, V, R, 2, true>() : batch_strip_kernel<S, V, R, 2, false>();
+        case 3: return final ? batch_strip_kernel<S, V, R, 3, true>() : batch_strip_kernel<S, V, R, 3, false>();
+        default: return final ? batch_strip_kernel<S, V, R, 4, true>() : batch_strip_kernel<S, V, R, 4, false>();
+    }
+}
+
+template <typename S>
+const void* strip_kernel_s(int V, int R, int L, bool final) {
+    if (V == 1 && R == 2) return strip_kernel_l<S, 1, 2>(L, final);
+    if (V == 1 && R == 4) return strip_kernel_l<S, 1, 4>(L, final);
+    if (V == 2 && R == 2) return strip_kernel_l<S, 2, 2>(L, final);
+    return nullptr;
+}
+
+const void* strip_kernel(int precision, int V, int R, int L, bool final) {
+    switch (precision) {
+        case FDTD_FP64: return strip_kernel_s<double>(V, R, L, final);
+        case FDTD_FP32: return strip_kernel_s<float>(V, R, L, final);
+        default: return strip_kernel_s<_Float16>(V, R, L, final);
+    }
+}
+
+// Strip tiling of every grid (L, V, R rows per lane, W wavefronts), its
+// work list (most expensive tiles first) and the persistent grid size.
+// Each grid's z chunk aims at about 128 Ki computed cells per tile.
+int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
+    const int cs = (int)compute_size(b->precision);
+    if (L < 1 || L > 4 || !batch_strip_built(cs, V, R) || W < 1 || W > 16 ||
+        64 * W > fdtd::strip_max_threads(V, R, L, cs))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "strip blocking L=%d V=%d R=%d W=%d not available",
+                    L, V, R, W);
+    const int Gx = (L + V - 1) / V * V;
+    const int TX = 64 * V - 2 * Gx, TY = W * R - 2 * L;
+    if (TX < 1 || TY < 1)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "strip tiles of W=%d R=%d leave no stored rows", W, R);
+    std::vector<fdtd::StripGeom> geom(b->grids.size());
+    std::vector<int2> work;
+    std::vector<long long> cost;
+    for (size_t q = 0; q < b->grids.size(); ++q) {
+        const fdtd_grid* g = b->grids[q];
+        if (g->Ni % V || (unsigned long long)g->Ni * g->Nj * g->esize >= (1ull << 32))
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "grid %zu cannot take strips with V=%d", q, V);
+        fdtd::StripGeom& G = geom[q];
+        G.Ni = g->Ni;
+        G.Nj = g->Nj;
+        G.Nk = g->Nk;
+        G.W = W;
+        G.Gx = Gx;
+        G.TX = TX;
+        G.TY = TY;
+        const long long area = 64LL * V * W * R;
+        int kc = (int)std::max<long long>(16, (128LL * 1024) / area);
+        kc = std::min(kc, g->Nk);
+        const int nchunk = (g->Nk + kc - 1) / kc;
+        G.kchunk = (g->Nk + nchunk - 1) / nchunk;
+        G.ntx = (g->Ni + TX - 1) / TX;
+        G.nty = (g->Nj + TY - 1) / TY;
+        G.ntiles = G.ntx * G.nty * nchunk;
+        G.per_xcd = (G.ntiles + 7) / 8;
+        for (int t = 0; t < G.ntiles; ++t) {
+            const int chunk = t / (G.ntx * G.nty);
+            const int kcount = std::min(G.kchunk, g->Nk - chunk * G.kchunk) + 2 * (L - 1);
+            work.push_back(make_int2((int)q, t));
+            cost.push_back(area * kcount);
+        }
+    }
+    std::vector<size_t> order(work.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+    std::vector<int2> sorted(work.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = work[order[i]];
+    const size_t lds = fdtd::strip_lds_bytes(L, W, V, cs);
+    if (lds > kLdsLimit) return fail(FDTD_ERR_INVALID_ARGUMENT, "strip tiles need too much LDS");
+    int per_cu = 0;
+    for (bool final : {false, true}) {
+        const void* kern = strip_kernel(b->precision, V, R, L, final);
+        if (!kern) return fail(FDTD_ERR_UNSUPPORTED, "strip variant not built");
+        if (int rc = allow_dynamic_lds(kern, lds)) return rc;
+        if (!final) HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, lds));
+    }
+    if (per_cu < 1) return fail(FDTD_ERR_UNSUPPORTED, "strip tiles do not fit a CU");
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->d_swork) HIP_TRY(hipFree(b->d_swork));
+    b->d_swork = nullptr;
+    b->sL = 0;  // until the new work list exists
+    HIP_TRY(hipMalloc(&b->d_swork, sizeof(int2) * sorted.size()));
+    HIP_TRY(hipMemcpy(b->d_swork, sorted.data(), sizeof(int2) * sorted.size(),
+                      hipMemcpyHostToDevice));
+    b->sgeom = geom;
+    b->snwork = (int)sorted.size();
+    b->snblocks = std::max(1, std::min(b->snwork, per_cu * cus));
+    b->slds = lds;
+    b->sL = L;
+    b->sV = V;
+    b->sR = R;
+    b->sW = W;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        b->graph_steps = 0;
+    }
+    return FDTD_OK;
 }
 
 template <typename S>
@@ -244,8 +461,19 @@ int apply_shape(fdtd_batch* b, const BatchShape& sh) {
 int prepare(fdtd_batch* b) {
     std::vector<fdtd::PointSource> all;
     std::vector<int> off(b->grids.size() + 1, 0);
+    std::vector<int> kst;  // per-grid plane ranges of the (plane-sorted) sources
+    std::vector<size_t> kst_off(b->grids.size(), 0);
     for (size_t q = 0; q < b->grids.size(); ++q) {
-        const std::vector<fdtd::PointSource> s = host_sources(b->grids[q]);
+        std::vector<fdtd::PointSource> s = host_sources(b->grids[q]);
+        // sorted by plane, stably (sources on one cell keep their order)
+        std::stable_sort(s.begin(), s.end(), [](const fdtd::PointSource& x,
+                                                const fdtd::PointSource& y) { return x.k < y.k; });
+        const int Nk = b->grids[q]->Nk;
+        kst_off[q] = kst.size();
+        kst.resize(kst.size() + Nk + 1, 0);
+        int* ks = kst.data() + kst_off[q];
+        for (const auto& x : s) ks[x.k + 1] += 1;
+        for (int k = 0; k < Nk; ++k) ks[k + 1] += ks[k];
         off[q] = (int)all.size();
         all.insert(all.end(), s.begin(), s.end());
         if (b->grids[q]->j_dense_active)
@@ -274,10 +502,39 @@ int prepare(fdtd_batch* b) {
         HIP_TRY(hipMemcpy(b->d_src, all.data(), sizeof(fdtd::PointSource) * all.size(),
                           hipMemcpyHostToDevice));
     b->total_src = (int)all.size();
+    if (kst.size() > b->kstart_cap) {
+        if (b->d_kstart) HIP_TRY(hipFree(b->d_kstart));
+        b->d_kstart = nullptr;
+        b->kstart_cap = 0;
+        HIP_TRY(hipMalloc(&b->d_kstart, sizeof(int) * kst.size()));
+        b->kstart_cap = kst.size();
+        if (b->graph) {
+            (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+            b->graph_steps = 0;
+        }
+    }
+    HIP_TRY(hipMemcpy(b->d_kstart, kst.data(), sizeof(int) * kst.size(), hipMemcpyHostToDevice));
     switch (b->precision) {
         case FDTD_FP64: fill_descriptors<double>(b, off); break;
         case FDTD_FP32: fill_descriptors<float>(b, off); break;
         default: fill_descriptors<_Float16>(b, off);
+    }
+    if (b->sL > 0) {
+        switch (b->precision) {
+            case FDTD_FP64: fill_strip_descriptors<double>(b, off, kst_off); break;
+            case FDTD_FP32: fill_strip_descriptors<float>(b, off, kst_off); break;
+            default: fill_strip_descriptors<_Float16>(b, off, kst_off);
+        }
+        if (b->h_sgrids.size() != b->sgrid_desc_bytes) {
+            if (b->d_sgrids) HIP_TRY(hipFree(b->d_sgrids));
+            b->d_sgrids = nullptr;
+            b->sgrid_desc_bytes = 0;
+            HIP_TRY(hipMalloc(&b->d_sgrids, b->h_sgrids.size()));
+            b->sgrid_desc_bytes = b->h_sgrids.size();
+        }
+        HIP_TRY(hipMemcpy(b->d_sgrids, b->h_sgrids.data(), b->h_sgrids.size(),
+                          hipMemcpyHostToDevice));
     }
     if (b->h_grids.size() != b->grid_desc_bytes) {
         if (b->d_grids) HIP_TRY(hipFree(b->d_grids));
@@ -317,11 +574,40 @@ int prologue_B(fdtd_batch* b) {
 // batched sweeps.  Source values come from step0 (host) or, when
 // step_dev is non-null, from the device counter, which is advanced at the
 // end (graph replays then continue in time).
-int enqueue_steps(fdtd_batch* b, int nsteps, long long step0, bool device_step) {
+int enqueue_steps(fdtd_batch* b, int nsteps, long long step0, bool device_step, int* swaps) {
     if (int rc = prologue_B(b)) return rc;
-    for (int s = 0; s < nsteps; ++s) {
-        if (b->total_src > 0 && s % kValTable == 0) {
-            const int chunk = std::min(kValTable, nsteps - s);
+    // strip passes of sL steps first (when set and the run is long enough),
+    // then the remaining steps as single sweeps; every launch swaps buffers
+    const int L = b->sL;
+    const int npass = L > 0 ? nsteps / L : 0;
+    const int rem = nsteps - npass * L;
+    int parity = 0;
+    int s = 0;
+    const int pchunk = L > 0 ? (kValTable / L) * L : kValTable;
+    for (int p = 0; p < npass; ++p, s += L) {
+        if (b->total_src > 0 && s % pchunk == 0) {
+            const int chunk = std::min(pchunk, npass * L - s);
+            const int n = b->total_src * chunk;
+            hipLaunchKernelGGL(fdtd::eval_sources, dim3((n + 255) / 256), dim3(256), 0, b->stream,
+                               b->d_src, b->total_src, chunk,
+                               device_step ? (const long long*)b->d_step : nullptr,
+                               device_step ? 0LL : step0 + s, b->grids[0]->dt, b->d_vals);
+            HIP_TRY(hipGetLastError());
+            if (device_step) {
+                hipLaunchKernelGGL(fdtd::advance_step, dim3(1), dim3(1), 0, b->stream, b->d_step,
+                                   (long long)chunk);
+                HIP_TRY(hipGetLastError());
+            }
+        }
+        const double* vals = b->total_src > 0 ? b->d_vals + (size_t)(s % pchunk) * b->total_src
+                                              : nullptr;
+        if (int rc = launch_batch_strip(b, rem == 0 && p + 1 == npass, parity, vals)) return rc;
+        parity ^= 1;
+    }
+    for (int q = 0; q < rem; ++q, ++s) {
+        const int r = q;  // step index within the single sweeps
+        if (b->total_src > 0 && r % kValTable == 0) {
+            const int chunk = std::min(kValTable, rem - r);
             const int n = b->total_src * chunk;
             // values of steps step0+s ...; with a device counter the kernel
             // reads the run's first step and this launch adds s to it
@@ -336,16 +622,18 @@ int enqueue_steps(fdtd_batch* b, int nsteps, long long step0, bool device_step) 
                 HIP_TRY(hipGetLastError());
             }
         }
-        const double* row = b->total_src > 0 ? b->d_vals + (size_t)(s % kValTable) * b->total_src
+        const double* row = b->total_src > 0 ? b->d_vals + (size_t)(r % kValTable) * b->total_src
                                              : nullptr;
-        if (int rc = launch_batch(b, s + 1 < nsteps, s & 1, row)) return rc;
+        if (int rc = launch_batch(b, q + 1 < rem, parity, row)) return rc;
+        parity ^= 1;
     }
+    if (swaps) *swaps = npass + rem;
     return FDTD_OK;
 }
 
-void finish_steps(fdtd_batch* b, int nsteps) {
+void finish_steps(fdtd_batch* b, int nsteps, int swaps) {
     for (fdtd_grid* g : b->grids) {
-        g->cur = (g->cur + nsteps) & 1;
+        g->cur = (g->cur + swaps) & 1;
         g->step += nsteps;
         g->ba_cur = -1;  // a batch run leaves no carried half step
     }
@@ -417,6 +705,9 @@ int fdtd_batch_destroy(fdtd_batch* b) {
     if (b->d_src) (void)hipFree(b->d_src);
     if (b->d_vals) (void)hipFree(b->d_vals);
     if (b->d_step) (void)hipFree(b->d_step);
+    if (b->d_swork) (void)hipFree(b->d_swork);
+    if (b->d_sgrids) (void)hipFree(b->d_sgrids);
+    if (b->d_kstart) (void)hipFree(b->d_kstart);
     for (auto& e : b->ev)
         if (e) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -429,9 +720,10 @@ int fdtd_batch_update_fields(fdtd_batch* b, int nsteps) {
     if (nsteps == 0) return FDTD_OK;
     if (int rc = check_steps(b)) return rc;
     if (int rc = prepare(b)) return rc;
-    if (int rc = enqueue_steps(b, nsteps, b->grids[0]->step, false)) return rc;
+    int swaps = 0;
+    if (int rc = enqueue_steps(b, nsteps, b->grids[0]->step, false, &swaps)) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
-    finish_steps(b, nsteps);
+    finish_steps(b, nsteps, swaps);
     return FDTD_OK;
 }
 
@@ -446,7 +738,8 @@ int fdtd_batch_update_fields_graph(fdtd_batch* b, int nsteps, int replays) {
         }
         hipGraph_t graph = nullptr;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = enqueue_steps(b, nsteps, 0, true);
+        int swaps = 0;
+        const int rc = enqueue_steps(b, nsteps, 0, true, &swaps);
         hipError_t e = hipStreamEndCapture(b->stream, &graph);
         if (rc) {
             if (graph) (void)hipGraphDestroy(graph);
@@ -457,27 +750,52 @@ int fdtd_batch_update_fields_graph(fdtd_batch* b, int nsteps, int replays) {
         (void)hipGraphDestroy(graph);
         if (e != hipSuccess) return fail(FDTD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
         b->graph_steps = nsteps;
+        b->graph_swaps = swaps;
     }
     const long long step0 = b->grids[0]->step;
     HIP_TRY(hipMemcpy(b->d_step, &step0, sizeof step0, hipMemcpyHostToDevice));
     for (int r = 0; r < replays; ++r) {
         if (r > 0) {
-            finish_steps(b, nsteps);
-            // an odd step count swaps every grid's buffers: re-point the
-            // descriptors (prepare() waits for the previous replay first)
-            if (nsteps & 1)
+            finish_steps(b, nsteps, b->graph_swaps);
+            // an odd number of launches swaps every grid's buffers: re-point
+            // the descriptors (prepare() waits for the previous replay first)
+            if (b->graph_swaps & 1)
                 if (int rc = prepare(b)) return rc;
         }
         HIP_TRY(hipGraphLaunch(b->graph, b->stream));
     }
     HIP_TRY(hipStreamSynchronize(b->stream));
-    if (replays > 0) finish_steps(b, nsteps);
+    if (replays > 0) finish_steps(b, nsteps, b->graph_swaps);
     return FDTD_OK;
 }
 
 int fdtd_batch_synchronize(fdtd_batch* b) {
     if (!b) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
     HIP_TRY(hipStreamSynchronize(b->stream));
+    return FDTD_OK;
+}
+
+int fdtd_batch_set_time_blocking_strips(fdtd_batch* b, int L, int V, int R, int W) {
+    if (!b) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (L <= 0) {
+        b->sL = 0;
+        if (b->graph) {
+            (void)hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+            b->graph_steps = 0;
+        }
+        return FDTD_OK;
+    }
+    return apply_strips(b, L, V > 0 ? V : 1, R > 0 ? R : 2, W > 0 ? W : 16);
+}
+
+int fdtd_batch_get_time_blocking_strips(const fdtd_batch* b, int out[5]) {
+    if (!b || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    out[0] = b->sL;
+    out[1] = b->sL > 0 ? b->sV : 0;
+    out[2] = b->sL > 0 ? b->sR : 0;
+    out[3] = b->sL > 0 ? b->sW : 0;
+    out[4] = b->sL > 0 ? b->snwork : 0;
     return FDTD_OK;
 }
 
@@ -501,10 +819,11 @@ int fdtd_batch_time(fdtd_batch* b, int nsteps, float* ms_batched, float* ms_sequ
     if (int rc = check_steps(b)) return rc;
     if (int rc = prepare(b)) return rc;
     HIP_TRY(hipEventRecord(b->ev[0], b->stream));
-    if (int rc = enqueue_steps(b, nsteps, b->grids[0]->step, false)) return rc;
+    int swaps = 0;
+    if (int rc = enqueue_steps(b, nsteps, b->grids[0]->step, false, &swaps)) return rc;
     HIP_TRY(hipEventRecord(b->ev[1], b->stream));
     HIP_TRY(hipEventSynchronize(b->ev[1]));
-    finish_steps(b, nsteps);
+    finish_steps(b, nsteps, swaps);
     float t = 0;
     HIP_TRY(hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
     if (ms_batched) *ms_batched = t / nsteps;

@@ -1285,9 +1285,9 @@ struct StripArgs {
     Fields<S> out;
     const PointSource* src;   // sorted by plane
     const int* kstart;        // sources of plane k: [kstart[k], kstart[k+1])
-    const double* vals;       // source values of the pass's first step; step l at vals + l*nsrc
+    const double* vals;       // source values of the pass's first step; step l at vals + l*vstride
     int nsrc;
-    int pad;
+    int vstride;              // value-table row length (nsrc for one grid)
     S* ahead[3];              // FINAL passes: also store the next half step (B + d) + d here
     Coefs<C> c;
     StripGeom g;
@@ -1320,15 +1320,12 @@ __device__ __forceinline__ double lane_next(double x) {
     return __builtin_bit_cast(double, h);
 }
 
+// One tile (index < a.g.ntiles) of a strip pass; every thread of the
+// workgroup calls it with the same tile.
 template <typename S, typename C, int V, int R, int L, bool FINAL>
-__global__ void __launch_bounds__(strip_max_threads(V, R, L, sizeof(C)))
-strip_sweep(StripArgs<S, C> a) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    C* __restrict__ lds = reinterpret_cast<C*>(lds_raw);
+__device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int tile,
+                                           C* __restrict__ lds) {
     const StripGeom& G = a.g;
-    const int blk = blockIdx.x;
-    const int tile = (blk & 7) * G.per_xcd + (blk >> 3);
-    if (tile >= G.ntiles) return;  // whole workgroup
     const int txy = G.ntx * G.nty;
     const int chunk = tile / txy;
     const int rem = tile - chunk * txy;
@@ -1482,7 +1479,7 @@ strip_sweep(StripArgs<S, C> a) {
                     t[2][v] = c.ez_by * (bin[1][r][v] - by_im);
                 }
                 if (q1 > q0)
-                    add_point_sources<C, V>(t, c, a.src, a.vals + l * a.nsrc, q0, q1, gi0, gj[r],
+                    add_point_sources<C, V>(t, c, a.src, a.vals + l * a.vstride, q0, q1, gi0, gj[r],
                                             wrap_mod(pe, Nk));
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
@@ -1595,6 +1592,16 @@ strip_sweep(StripArgs<S, C> a) {
     }
 }
 
+template <typename S, typename C, int V, int R, int L, bool FINAL>
+__global__ void __launch_bounds__(strip_max_threads(V, R, L, sizeof(C)))
+strip_sweep(StripArgs<S, C> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int blk = blockIdx.x;
+    const int tile = (blk & 7) * a.g.per_xcd + (blk >> 3);
+    if (tile >= a.g.ntiles) return;  // whole workgroup
+    strip_tile<S, C, V, R, L, FINAL>(a, tile, reinterpret_cast<C*>(lds_raw));
+}
+
 // ---------------------------------------------------------------------------
 // Batched sweep: many independent grids, one persistent launch per step.
 //
@@ -1698,6 +1705,88 @@ batch_fused_sweep(BatchArgs<S, C> a) {
         const int4 w = make_int4(uni(wv.x), uni(wv.y), uni(wv.z), 0);
         const SweepArgs<S, C, C> sa = load_batch_grid(a.grids + w.x, a.parity, a.vals_row);
         fused_tile<S, C, C, V, STEADY>(sa, w.y, w.z, reinterpret_cast<C*>(lds_raw));
+    }
+    if (threadIdx.x == 0) {
+        const unsigned done = atomicAdd(&a.counters[1], 1u);
+        if (done == gridDim.x - 1) {
+            atomicExch(&a.counters[0], 0u);
+            atomicExch(&a.counters[1], 0u);
+        }
+    }
+}
+
+// Batched temporal blocking: one persistent launch per pass of L steps
+// over the strip tiles of every grid (the same strip_tile as a single grid).
+template <typename S, typename C> struct BatchStripGrid {
+    Fields<S> buf[2];        // buf[p]: the grid's buffer pair used at parity p
+    const PointSource* src;  // this grid's sources, sorted by plane
+    const int* kstart;       // its per-plane ranges (Nk + 1 entries, local indices)
+    int src_off;             // offset of its source values in a value-table row
+    int nsrc;
+    Coefs<C> c;
+    StripGeom g;             // its strip tiling
+};
+
+template <typename S, typename C> struct BatchStripArgs {
+    const BatchStripGrid<S, C>* grids;
+    const int2* work;        // (grid, tile), most expensive first
+    int nwork;
+    int parity;              // the pass reads buf[parity], writes buf[1-parity]
+    const double* vals;      // source values of the pass's first step (null: no sources)
+    int vstride;             // value-table row length (all sources of the batch)
+    int pad;
+    unsigned int* counters;  // [0] next work item, [1] finished workgroups
+};
+
+template <typename S, typename C>
+__device__ __forceinline__ StripArgs<S, C> load_batch_strip_grid(const BatchStripGrid<S, C>* p,
+                                                                 int parity, const double* vals,
+                                                                 int vstride) {
+    StripArgs<S, C> a;
+    const Fields<S>* in = p->buf + parity;
+    const Fields<S>* out = p->buf + (1 - parity);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        a.in.e[d] = uni(in->e[d]);
+        a.in.b[d] = uni(in->b[d]);
+        a.out.e[d] = uni(out->e[d]);
+        a.out.b[d] = uni(out->b[d]);
+        a.ahead[d] = nullptr;
+    }
+    a.src = uni(p->src);
+    a.kstart = uni(p->kstart);
+    a.vals = vals ? vals + uni(p->src_off) : nullptr;
+    a.nsrc = uni(p->nsrc);
+    a.vstride = vstride;
+    const Coefs<C>& c = p->c;
+    a.c.ex_bz = uni(c.ex_bz); a.c.ex_by = uni(c.ex_by); a.c.ey_bx = uni(c.ey_bx);
+    a.c.ey_bz = uni(c.ey_bz); a.c.ez_by = uni(c.ez_by); a.c.ez_bx = uni(c.ez_bx);
+    a.c.jx = uni(c.jx); a.c.jy = uni(c.jy); a.c.jz = uni(c.jz);
+    a.c.bx_ey = uni(c.bx_ey); a.c.bx_ez = uni(c.bx_ez); a.c.by_ez = uni(c.by_ez);
+    a.c.by_ex = uni(c.by_ex); a.c.bz_ex = uni(c.bz_ex); a.c.bz_ey = uni(c.bz_ey);
+    const StripGeom& q = p->g;
+    a.g.Ni = uni(q.Ni); a.g.Nj = uni(q.Nj); a.g.Nk = uni(q.Nk); a.g.W = uni(q.W);
+    a.g.Gx = uni(q.Gx); a.g.TX = uni(q.TX); a.g.TY = uni(q.TY); a.g.kchunk = uni(q.kchunk);
+    a.g.ntx = uni(q.ntx); a.g.nty = uni(q.nty); a.g.ntiles = uni(q.ntiles);
+    a.g.per_xcd = uni(q.per_xcd);
+    return a;
+}
+
+template <typename S, typename C, int V, int R, int L, bool FINAL>
+__global__ void __launch_bounds__(strip_max_threads(V, R, L, sizeof(C)))
+batch_strip_sweep(BatchStripArgs<S, C> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ int item;
+    for (;;) {
+        if (threadIdx.x == 0) item = static_cast<int>(atomicAdd(&a.counters[0], 1u));
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(item);
+        __syncthreads();
+        if (t >= a.nwork) break;
+        const int2 wv = a.work[t];
+        const StripArgs<S, C> sa = load_batch_strip_grid(a.grids + uni(wv.x), a.parity, a.vals,
+                                                         a.vstride);
+        strip_tile<S, C, V, R, L, FINAL>(sa, uni(wv.y), reinterpret_cast<C*>(lds_raw));
     }
     if (threadIdx.x == 0) {
         const unsigned done = atomicAdd(&a.counters[1], 1u);

@@ -659,7 +659,7 @@ int launch_strip_v(fdtd_grid* g, int slot, int in, int out, void* const* in_b, v
         a.kstart = g->d_kstart;
         a.vals = g->d_vals ? g->d_vals + (size_t)slot * g->nsrc : nullptr;
         a.nsrc = g->nsrc;
-        a.pad = 0;
+        a.vstride = g->nsrc;
         a.c = make_coefs<C>(g);
         a.g = make_strip_geom(g, t);
         const size_t lds = fdtd::strip_lds_bytes(L, a.g.W, V, sizeof(C));

@@ -266,6 +266,14 @@ int fdtd_batch_time(fdtd_batch* batch, int nsteps, float* ms_batched, float* ms_
 int fdtd_batch_set_shape(fdtd_batch* batch, int V, int threads, long long tile_cells);
 int fdtd_batch_get_shape(const fdtd_batch* batch, long long out[4]);
 int fdtd_batch_autotune(fdtd_batch* batch, int reps, long long out[4], float* best_ms);
+/* Temporal blocking for a batch (no counterpart in the reference): runs of
+ * at least L steps advance in persistent strip passes of L steps over the
+ * strip tiles of every grid (64*V columns by W*R rows, as
+ * fdtd_set_time_blocking_strips; 0 picks V=1, R=2, W=16), the remaining
+ * steps in batched single sweeps; bit-identical to single-grid runs.
+ * L = 0 turns it off.  get: out = {L, V, R, W, work items}. */
+int fdtd_batch_set_time_blocking_strips(fdtd_batch* batch, int L, int V, int R, int W);
+int fdtd_batch_get_time_blocking_strips(const fdtd_batch* batch, int out[5]);
 
 #ifdef __cplusplus
 }

@@ -103,3 +103,62 @@ def test_plane_wave_batch_workload_small():
     ms_b, ms_s = b.time(4)
     print("batched ms/step", ms_b, "sequential", ms_s)
     assert ms_b > 0 and ms_s > 0
+
+
+@pytest.mark.parametrize("prec,strips", [(Precision.FP64, (2, 1, 2, 8)),
+                                         (Precision.FP32, (4, 1, 2, 16)),
+                                         (Precision.FP32, (3, 1, 4, 4)),
+                                         (Precision.FP32, (4, 2, 2, 8)),
+                                         (Precision.FP32, (1, 1, 2, 16)),
+                                         (Precision.FP16, (3, 1, 2, 16))])
+def test_batch_strips_equal_single_grid_runs(prec, strips):
+    """Batched strip passes (L steps per persistent launch, plane-sorted
+    sources per grid) plus remainder sweeps give every grid its single-grid
+    fields bitwise."""
+    singles = [make(s, prec, q) for q, s in enumerate(SHAPES)]
+    batched = [make(s, prec, q) for q, s in enumerate(SHAPES)]
+    for g in singles:
+        g.set_step_carry(False)  # a batch call starts from the B/2 prologue
+    b = Batch(batched)
+    b.set_time_blocking_strips(*strips)
+    cfg = b.get_time_blocking_strips()
+    print("strips", cfg, flush=True)
+    assert (cfg["L"], cfg["V"], cfg["R"], cfg["W"]) == strips
+    L = strips[0]
+    for nsteps in (2 * L + 1, L):
+        for g in singles:
+            g.update_fields(nsteps)
+        b.update_fields(nsteps)
+        for g1, g2 in zip(singles, batched):
+            assert g1.checksum() == g2.checksum(), nsteps
+
+
+@pytest.mark.parametrize("nsteps,replays,L", [(8, 3, 4), (5, 3, 2)])
+def test_batch_strips_graph_replays(nsteps, replays, L):
+    """Captured strip passes replayed; an odd number of launches per replay
+    (passes plus remainder sweeps) swaps the buffers between replays."""
+    singles = [make(s, Precision.FP32, q) for q, s in enumerate(SHAPES)]
+    batched = [make(s, Precision.FP32, q) for q, s in enumerate(SHAPES)]
+    b = Batch(batched)
+    b.set_time_blocking_strips(L, 1, 2, 8)
+    b.update_fields_graph(nsteps, replays)
+    for g in singles:
+        g.update_fields(nsteps * replays)
+    for g1, g2 in zip(singles, batched):
+        assert g1.checksum() == g2.checksum()
+        assert g2.step == nsteps * replays
+    b.set_time_blocking_strips(0)
+    b.update_fields(3)
+    for g in singles:
+        g.update_fields(3)
+    for g1, g2 in zip(singles, batched):
+        assert g1.checksum() == g2.checksum()
+
+
+def test_batch_strips_reject_invalid_shapes():
+    grids = [make(s, Precision.FP64, q) for q, s in enumerate(SHAPES[:2])]
+    b = Batch(grids)
+    for args in [(2, 1, 4, 8), (5, 1, 2, 8), (2, 1, 2, 1), (2, 1, 3, 8)]:
+        with pytest.raises((ValueError, RuntimeError)):
+            b.set_time_blocking_strips(*args)
+    assert b.get_time_blocking_strips()["L"] == 0
