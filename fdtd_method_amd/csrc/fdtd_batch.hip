@@ -340,7 +340,7 @@ int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
     std::vector<long long> cost;
     for (size_t q = 0; q < b->grids.size(); ++q) {
         const fdtd_grid* g = b->grids[q];
-        if (g->Ni % V || (unsigned long long)g->Ni * g->Nj * g->esize >= (1ull << 32))
+        if (g->Ni % V || (unsigned long long)g->Ni * g->Nj * g->esize >= (1ull << 31))
             return fail(FDTD_ERR_INVALID_ARGUMENT, "grid %zu cannot take strips with V=%d", q, V);
         fdtd::StripGeom& G = geom[q];
         G.Ni = g->Ni;

@@ -119,6 +119,62 @@ __device__ __forceinline__ void store_pack(S* __restrict__ p, const C (&in)[V]) 
     for (int v = 0; v < V; ++v) __builtin_nontemporal_store(static_cast<S>(in[v]), p + v);
 }
 
+// Raw buffer access to one plane: a resource over the plane's bytes and a
+// 32-bit byte offset per lane.  The hardware range check makes an access at
+// or beyond the resource's size read 0 and write nothing, so a lane outside
+// the stored region takes kBufSkip as its offset and a plane that is not
+// read or written gets a size-0 resource.  The memory instructions are then
+// issued on every path through a loop, and the compiler can count them:
+// waiting for a prefetch does not wait for stores issued after it.  Planes
+// are at most 2 GiB, so kBufSkip lies beyond every resource.
+constexpr int kBufDword3 = 0x00020000;   // raw (untyped) buffer, gfx9 family
+constexpr unsigned kBufSkip = 0x80000000u;
+constexpr int kBufNT = 2;                 // non-temporal store
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes),
+                                             kBufDword3);
+}
+
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt unconstrained): every vector
+// memory access of the wavefront has completed.  Ends a prologue whose loads
+// would otherwise still be outstanding where it joins a loop, making the
+// loop's first uses of those registers wait on the loop's own stores.
+__device__ __forceinline__ void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+template <typename S, int V>
+__device__ __forceinline__ RawPack<S, V> buf_load_raw(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    using Words = typename RawPack<S, V>::Words;
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    constexpr int bytes = static_cast<int>(sizeof(S)) * V;
+    RawPack<S, V> p;
+    if constexpr (bytes == 2)
+        p.words = __builtin_bit_cast(Words, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+    else if constexpr (bytes == 4)
+        p.words = __builtin_bit_cast(Words, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    else if constexpr (bytes == 8)
+        p.words = __builtin_bit_cast(Words, static_cast<u32x2>(__builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0)));
+    else {
+        static_assert(bytes == 16, "pack of 2, 4, 8 or 16 bytes");
+        p.words = __builtin_bit_cast(Words, static_cast<u32x4>(__builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0)));
+    }
+    return p;
+}
+
+template <typename S>
+__device__ __forceinline__ void buf_store_nt(S x, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    if constexpr (sizeof(S) == 2)
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, x), r, off, 0, kBufNT);
+    else if constexpr (sizeof(S) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, off, 0, kBufNT);
+    else {
+        static_assert(sizeof(S) == 8, "2, 4 or 8 byte values");
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, off, 0, kBufNT);
+    }
+}
+
 // A field value as the storage type holds it.  Blocked levels keep their
 // intermediate fields in registers; with FP16 storage a level's output is
 // rounded where the next unblocked sweep would read it back from memory
@@ -1362,6 +1418,22 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
     const int k0 = chunk * G.kchunk;
     const int k1 = min(k0 + G.kchunk, Nk);
     const int plane = Ni * Nj;
+    // Memory path.  FP64 and FP16 storage prefetch and store through plane
+    // buffer resources, issued on every path, so that waiting for the
+    // prefetch does not wait for the stores issued after it (measured:
+    // FP64 passes 12%, FP16 passes 16% faster than with global accesses).
+    // FP32 keeps global accesses under branches, whose waits do include the
+    // previous plane's stores: its buffer form measured 4-5% slower.
+    constexpr bool kBuf = sizeof(S) != 4;
+    const unsigned pbytes = static_cast<unsigned>(plane) * static_cast<unsigned>(sizeof(S));
+    // store offsets: kBufSkip outside the stored rows and columns
+    unsigned st_off[R][V];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            st_off[r][v] = kBuf && row_out[r] && col_out[v]
+                               ? row_off[r] + static_cast<unsigned>(v * sizeof(S)) : kBufSkip;
     // wavefront whose last row lies below / first row above this one's
     // (the end wavefronts read their own: those rows are ghost rows)
     const int wm = w > 0 ? w - 1 : w, wp = w + 1 < W ? w + 1 : w;
@@ -1409,25 +1481,70 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
         for (int q = 0; q < 2; ++q)
 #pragma unroll
             for (int v = 0; v < V; ++v) sE(l, q, w)[v] = C(0);
-
-    for (int kk = kfirst; kk <= klast; ++kk) {
+    // level 0's B input row for the wavefront below: the buffer path
+    // publishes it at the end of the previous iteration (after the barriers
+    // that follow its last reads), once the prefetch has been waited for
+    if constexpr (kBuf) {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             sB(0, 0, w)[v] = bn[0][R - 1][v];
             sB(0, 1, w)[v] = bn[2][R - 1][v];
         }
+        wait_vmem_all();
+    }
+
+    // Point-source ranges [qlo, qhi) of the planes pk the levels update.
+    // Level l at iteration kk updates plane kk-l, which level l-1 updated one
+    // iteration earlier: the ranges pass down a uniform shift register, with
+    // one table read per iteration, issued a plane ahead.
+    int pk[L], qlo[L], qhi[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        pk[l] = wrap_mod(kfirst - l, Nk);
+        qlo[l] = a.nsrc > 0 ? kst[pk[l]] : 0;
+        qhi[l] = a.nsrc > 0 ? kst[pk[l] + 1] : 0;
+    }
+
+    for (int kk = kfirst; kk <= klast; ++kk) {
+        const int pnext = pk[0] + 1 == Nk ? 0 : pk[0] + 1;
+        const int nlo = a.nsrc > 0 ? kst[pnext] : 0;
+        const int nhi = a.nsrc > 0 ? kst[pnext + 1] : 0;
+        if constexpr (!kBuf) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                sB(0, 0, w)[v] = bn[0][R - 1][v];
+                sB(0, 1, w)[v] = bn[2][R - 1][v];
+            }
+        }
+        // prefetch of plane kk+1 (buffer path: a size-0 resource after the
+        // last plane)
         RawPack<S, V> rawE[3][R], rawB[3][R];
         const bool more = kk < klast;
-        if (more) {
+        if constexpr (!kBuf) {
+            if (more) {
+                const size_t off = static_cast<size_t>(wrap_mod(kk + 1, Nk)) * plane;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const gptr<const S> bb = uniform_gptr<const S>(a.in.b[d] + off);
+                    const gptr<const S> eb = uniform_gptr<const S>(a.in.e[d] + off);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        rawB[d][r] = load_raw<S, V>(generic(at(bb, row_off[r])));
+                        rawE[d][r] = load_raw<S, V>(generic(at(eb, row_off[r])));
+                    }
+                }
+            }
+        } else {
+            const unsigned nb = more ? pbytes : 0u;
             const size_t off = static_cast<size_t>(wrap_mod(kk + 1, Nk)) * plane;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const gptr<const S> bb = uniform_gptr<const S>(a.in.b[d] + off);
-                const gptr<const S> eb = uniform_gptr<const S>(a.in.e[d] + off);
+                const __amdgpu_buffer_rsrc_t bb = plane_rsrc(a.in.b[d] + off, nb);
+                const __amdgpu_buffer_rsrc_t eb = plane_rsrc(a.in.e[d] + off, nb);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    rawB[d][r] = load_raw<S, V>(generic(at(bb, row_off[r])));
-                    rawE[d][r] = load_raw<S, V>(generic(at(eb, row_off[r])));
+                    rawB[d][r] = buf_load_raw<S, V>(bb, row_off[r]);
+                    rawE[d][r] = buf_load_raw<S, V>(eb, row_off[r]);
                 }
             }
         }
@@ -1438,6 +1555,7 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
         // FINAL passes with a carry buffer also store the next half step
         // (B + d) + d of the last level's B plane kk-L, as it is computed
         const bool carry_plane = FINAL && a.ahead[0] != nullptr && kk - L >= k0 && kk - L < k1;
+        const size_t carry_off = static_cast<size_t>(carry_plane ? kk - L : k0) * plane;
 #pragma unroll
         for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -1449,7 +1567,7 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                 }
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            const int pe = kk - l;  // this level's E plane; its B plane is pe - 1
+            // this level's E plane is kk - l (pk[l] wrapped); its B plane is kk - l - 1
             if (L > 1) {
                 // publish the first-row Ex, Ez of the level whose readers
                 // have all passed the last barrier
@@ -1460,12 +1578,7 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                     sE(lp, 1, w)[v] = eP[lp][2][0][v];
                 }
             }
-            int q0 = 0, q1 = 0;
-            if (a.nsrc > 0) {
-                const int pk = wrap_mod(pe, Nk);
-                q0 = kst[pk];
-                q1 = kst[pk + 1];
-            }
+            const int q0 = qlo[l], q1 = qhi[l];
             C enew[3][R][V];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -1480,7 +1593,7 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                 }
                 if (q1 > q0)
                     add_point_sources<C, V>(t, c, a.src, a.vals + l * a.vstride, q0, q1, gi0, gj[r],
-                                            wrap_mod(pe, Nk));
+                                            pk[l]);
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     const C bx_jm = r > 0 ? bin[0][r - 1][v] : sB(l, 0, wm)[v];
@@ -1510,8 +1623,13 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                         bnew[0][r][v] = bnew[0][r][v] + dbx;
                         bnew[1][r][v] = bnew[1][r][v] + dby;
                         bnew[2][r][v] = bnew[2][r][v] + dbz;
-                    } else if (carry_plane && row_out[r] && col_out[v]) {
-                        const size_t off = static_cast<size_t>(kk - L) * plane + v;
+                    } else if (kBuf && a.ahead[0] != nullptr) {
+                        const unsigned nb = carry_plane ? pbytes : 0u;
+                        buf_store_nt(static_cast<S>(bnew[0][r][v] + dbx), plane_rsrc(a.ahead[0] + carry_off, nb), st_off[r][v]);
+                        buf_store_nt(static_cast<S>(bnew[1][r][v] + dby), plane_rsrc(a.ahead[1] + carry_off, nb), st_off[r][v]);
+                        buf_store_nt(static_cast<S>(bnew[2][r][v] + dbz), plane_rsrc(a.ahead[2] + carry_off, nb), st_off[r][v]);
+                    } else if (!kBuf && carry_plane && row_out[r] && col_out[v]) {
+                        const size_t off = carry_off + v;
                         __builtin_nontemporal_store(static_cast<S>(bnew[0][r][v] + dbx), generic(at(uniform_gptr(a.ahead[0] + off), row_off[r])));
                         __builtin_nontemporal_store(static_cast<S>(bnew[1][r][v] + dby), generic(at(uniform_gptr(a.ahead[1] + off), row_off[r])));
                         __builtin_nontemporal_store(static_cast<S>(bnew[2][r][v] + dbz), generic(at(uniform_gptr(a.ahead[2] + off), row_off[r])));
@@ -1541,7 +1659,8 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
             }
         }
         // stores of the last level: E_{L-1} at plane kk-L+1, B_{L-1} at kk-L
-        {
+        // (buffer path: size-0 resources for planes outside the chunk)
+        if constexpr (!kBuf) {
             const int pe = kk - L + 1;
             if (pe >= k0 && pe < k1) {
                 const size_t off = static_cast<size_t>(pe) * plane;
@@ -1571,6 +1690,23 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                                     __builtin_nontemporal_store(static_cast<S>(bin[d][r][v]), dst + v);
                         }
             }
+        } else {
+            const int pe = kk - L + 1;
+            const bool se = pe >= k0 && pe < k1, sb = pe - 1 >= k0 && pe - 1 < k1;
+            const size_t offe = static_cast<size_t>(se ? pe : k0) * plane;
+            const size_t offb = static_cast<size_t>(sb ? pe - 1 : k0) * plane;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const __amdgpu_buffer_rsrc_t re = plane_rsrc(a.out.e[d] + offe, se ? pbytes : 0u);
+                const __amdgpu_buffer_rsrc_t rb = plane_rsrc(a.out.b[d] + offb, sb ? pbytes : 0u);
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        buf_store_nt(static_cast<S>(eP[L - 1][d][r][v]), re, st_off[r][v]);
+                        buf_store_nt(static_cast<S>(bin[d][r][v]), rb, st_off[r][v]);
+                    }
+            }
         }
         if (L == 1) {
             __syncthreads();
@@ -1580,7 +1716,7 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                 sE(0, 1, w)[v] = eP[0][2][0][v];
             }
         }
-        if (more) {
+        if (kBuf || more) {
 #pragma unroll
             for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -1589,6 +1725,22 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                     widen<S, C, V>(rawE[d][r], en[d][r]);
                 }
         }
+        if constexpr (kBuf) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                sB(0, 0, w)[v] = bn[0][R - 1][v];
+                sB(0, 1, w)[v] = bn[2][R - 1][v];
+            }
+        }
+#pragma unroll
+        for (int l = L - 1; l > 0; --l) {
+            pk[l] = pk[l - 1];
+            qlo[l] = qlo[l - 1];
+            qhi[l] = qhi[l - 1];
+        }
+        pk[0] = pnext;
+        qlo[0] = nlo;
+        qhi[0] = nhi;
     }
 }
 

@@ -315,7 +315,7 @@ bool strip_valid(const fdtd_grid* g, const BlockTuning& t) {
     const int cs = (int)compute_size(g->precision);
     if (t.L < 1 || t.L > 4 || !strip_built(cs, t.V, t.R)) return false;
     if (g->Ni % t.V || t.TYg <= 0 || t.TYg % t.R || t.kchunk <= 0) return false;
-    if ((unsigned long long)g->Ni * g->Nj * g->esize >= (1ull << 32)) return false;  // 32-bit plane offsets
+    if ((unsigned long long)g->Ni * g->Nj * g->esize >= (1ull << 31)) return false;  // 32-bit plane offsets, buffer range checks
     const int W = t.TYg / t.R;
     if (W < 1 || W > 16 || 64 * W > fdtd::strip_max_threads(t.V, t.R, t.L, cs)) return false;
     const int Gx = (t.L + t.V - 1) / t.V * t.V;
