@@ -1397,7 +1397,7 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
     const int gi0 = wrap_mod(px0 - G.Gx + cx0, Ni);    // V | Gx, V | TX: packs never straddle the wrap
     const int py0 = ty * G.TY;
     // Row offsets within a plane are unsigned 32-bit byte offsets (a plane
-    // is below 4 GiB), so every access is a uniform (scalar) plane base plus
+    // is below 2 GiB), so every access is a uniform (scalar) plane base plus
     // one 32-bit lane offset per row.
     int gj[R];
     unsigned row_off[R];

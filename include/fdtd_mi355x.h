@@ -229,6 +229,7 @@ int fdtd_autotune_blocking(fdtd_grid* grid, int reps, int out[5], float* best_ms
  * one-row LDS exchange), L = 1..4 steps per pass, kchunk planes per z
  * chunk (0 = default).  Shapes built: V*R*sizeof(compute type) <= 16 bytes
  * with V in {1,2}, R in {2,4} (FP32/FP16 compute in float, FP64 in double).
+ * Planes (Ni*Nj stored values) must be below 2 GiB (32-bit plane offsets).
  * Results are bit-identical to unblocked steps.  get: {L, V, R, W, kchunk},
  * R = W = 0 when the ghost-zone patch kernel (or no blocking) is set. */
 int fdtd_set_time_blocking_strips(fdtd_grid* grid, int L, int V, int R, int W, int kchunk);
