@@ -1421,7 +1421,8 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
     // Memory path.  FP64 and FP16 storage prefetch and store through plane
     // buffer resources, issued on every path, so that waiting for the
     // prefetch does not wait for the stores issued after it (measured:
-    // FP64 passes 12%, FP16 passes 16% faster than with global accesses).
+    // FP64 passes about 11%, FP16 passes 12-16% faster than with global
+    // accesses).
     // FP32 keeps global accesses under branches, whose waits do include the
     // previous plane's stores: its buffer form measured 4-5% slower.
     constexpr bool kBuf = sizeof(S) != 4;
