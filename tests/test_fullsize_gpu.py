@@ -12,7 +12,10 @@ the size:
   field exactly in FP64 (all operations are linear, scaling by 2 is exact,
   and the fields stay far from the subnormal range);
 * temporally blocked passes equal single sweeps bitwise, and FP32 single
-  sweeps equal the three-sweep structure bitwise.
+  sweeps equal the three-sweep structure bitwise;
+* the B update is minus a forward-difference curl of E, whose
+  forward-difference divergence vanishes identically: from zero fields
+  with the source in E only, div B stays at rounding level.
 
 Planes are compared on the host (one plane is Ni*Nj values); whole grids
 by checksum.
@@ -126,3 +129,21 @@ def test_fused_equals_three_sweeps_fp32_512():
     assert a.checksum() == b.checksum()
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec,blocking,tol", [(Precision.FP64, (4, 1, 2, 8, 0), 1e-12),
+                                               (Precision.FP32, (4, 1, 2, 16, 0), 1e-5)])
+def test_divergence_of_B_stays_zero_512(prec, blocking, tol):
+    n = 512
+    g = make((n, n, n), headline_source(256, 256, 256), prec, blocking)
+    g.update_fields(130)  # the wave front is ~37 cells out
+    for k in (240, 255, 256, 280):
+        bx = g.download_plane(Component.BX, k)  # [j, i], d = 1
+        by = g.download_plane(Component.BY, k)
+        bz0 = g.download_plane(Component.BZ, k)
+        bz1 = g.download_plane(Component.BZ, k + 1)
+        div = (np.roll(bx, -1, axis=1) - bx) + (np.roll(by, -1, axis=0) - by) + (bz1 - bz0)
+        scale = max(np.abs(bx).max(), np.abs(by).max(), np.abs(bz0).max())
+        assert scale > 0, k
+        assert np.abs(div).max() <= tol * scale, (k, np.abs(div).max() / scale)
+    g.close()
