@@ -13,6 +13,8 @@ the size:
   and the fields stay far from the subnormal range);
 * temporally blocked passes equal single sweeps bitwise, and FP32 single
   sweeps equal the three-sweep structure bitwise;
+* a plane wave along x stays bitwise uniform in y and z, across strip
+  tiles, wavefront boundaries, z chunks and the periodic wrap;
 * the B update is minus a forward-difference curl of E, whose
   forward-difference divergence vanishes identically: from zero fields
   with the source in E only, div B stays at rounding level.
@@ -146,4 +148,33 @@ def test_divergence_of_B_stays_zero_512(prec, blocking, tol):
         scale = max(np.abs(bx).max(), np.abs(by).max(), np.abs(bz0).max())
         assert scale > 0, k
         assert np.abs(div).max() <= tol * scale, (k, np.abs(div).max() / scale)
+    g.close()
+
+
+@pytest.mark.parametrize("prec,blocking", [(Precision.FP32, (4, 1, 2, 16, 0)),
+                                           (Precision.FP64, (4, 1, 2, 8, 0)),
+                                           (Precision.FP32, None)])
+def test_plane_wave_stays_transversely_uniform_512(prec, blocking):
+    """A plane wave along x (E along y, B along z) depends on i only: every
+    cell of a row (and every row of every plane) takes the same operations,
+    so the fields stay bitwise uniform in j and k, and Ex, Ez, Bx, By stay
+    exactly zero."""
+    n = 512
+    g = FDTD(Parameters(n, n, n), DT, prec)
+    g.add_plane_wave((3, 0, 0), 1.0, 0.3, (0.0, 1.0, 0.0))
+    if blocking:
+        g.set_time_blocking_strips(*blocking)
+    g.update_fields(37)  # strip passes and a remainder sweep
+    for c in (Component.EY, Component.BZ):
+        first = None
+        for k in (0, 63, 64, 255, 300, 511):
+            p = g.download_plane(c, k)  # [j, i]
+            assert (p == p[0:1, :]).all(), (c, k)
+            if first is None:
+                first = p[0]
+                assert np.abs(first).max() > 0.5 * (1.0 if c == Component.EY else 0.0), c
+            assert np.array_equal(p[0], first), (c, k)
+    for c in (Component.EX, Component.EZ, Component.BX, Component.BY):
+        for k in (0, 255, 511):
+            assert not g.download_plane(c, k).any(), (c, k)
     g.close()
