@@ -2004,14 +2004,39 @@ template <typename S>
 __global__ void __launch_bounds__(kReduceThreads)
 sumsq_maxabs_partial(const S* __restrict__ x, size_t n, double inv_scale,
                      double* __restrict__ part) {
+    // x is an allocation's start (16-byte aligned): 16-byte packs (8 FP16,
+    // 4 FP32 or 2 FP64 values), four in flight per thread, then the tail
+    // element-wise.  A fixed grid keeps the summation order fixed.
     __shared__ double sh[kReduceThreads];
+    constexpr int P = 16 / static_cast<int>(sizeof(S));
+    using Q = Pack<S, P>;
+    const Q* __restrict__ xq = reinterpret_cast<const Q*>(x);
+    const size_t npk = n / P;
+    const size_t stride = static_cast<size_t>(gridDim.x) * kReduceThreads;
     double s = 0.0, m = 0.0;
-    for (size_t i = blockIdx.x * static_cast<size_t>(kReduceThreads) + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * kReduceThreads) {
-        const double v = static_cast<double>(x[i]) * inv_scale;
+    auto add = [&](S e) {
+        const double v = static_cast<double>(e) * inv_scale;
         s += v * v;
         m = fmax(m, fabs(v));
+    };
+    size_t i = blockIdx.x * static_cast<size_t>(kReduceThreads) + threadIdx.x;
+    for (; i + 3 * stride < npk; i += 4 * stride) {
+        Q q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = xq[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int e = 0; e < P; ++e) add(q[u].v[e]);
     }
+    for (; i < npk; i += stride) {
+        const Q q = xq[i];
+#pragma unroll
+        for (int e = 0; e < P; ++e) add(q.v[e]);
+    }
+    for (size_t t = npk * P + blockIdx.x * static_cast<size_t>(kReduceThreads) + threadIdx.x;
+         t < n; t += stride)
+        add(x[t]);
     s = block_sum(s, sh);
     m = block_max(m, sh);
     if (threadIdx.x == 0) {
@@ -2115,9 +2140,21 @@ add_plane_wave(Fields<S> f, int Ni, int Nj, int Nk, double kx, double ky, double
 template <typename S>
 __global__ void __launch_bounds__(256)
 scale_array(S* __restrict__ x, size_t n, float factor) {
-    for (size_t i = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; i < n;
-         i += static_cast<size_t>(gridDim.x) * 256)
-        x[i] = static_cast<S>(static_cast<float>(x[i]) * factor);
+    // x is an allocation's start: 16-byte packs, then the tail
+    constexpr int P = 16 / static_cast<int>(sizeof(S));
+    using Q = Pack<S, P>;
+    Q* __restrict__ xq = reinterpret_cast<Q*>(x);
+    const size_t npk = n / P;
+    const size_t stride = static_cast<size_t>(gridDim.x) * 256;
+    for (size_t i = blockIdx.x * static_cast<size_t>(256) + threadIdx.x; i < npk; i += stride) {
+        Q q = xq[i];
+#pragma unroll
+        for (int e = 0; e < P; ++e) q.v[e] = static_cast<S>(static_cast<float>(q.v[e]) * factor);
+        xq[i] = q;
+    }
+    for (size_t t = npk * P + blockIdx.x * static_cast<size_t>(256) + threadIdx.x; t < n;
+         t += stride)
+        x[t] = static_cast<S>(static_cast<float>(x[t]) * factor);
 }
 
 // Fill with zeros (used instead of hipMemsetAsync so that step sequences can
