@@ -104,14 +104,20 @@ def test_strip_blocking_many_tiles_and_chunks(shape):
     assert a.checksum() == b.checksum()
 
 
-def test_strip_blocking_tiny_grids():
+@pytest.mark.parametrize("storage", ["wide", "fp16"])
+def test_strip_blocking_tiny_grids(storage):
     """Grids smaller than one tile: the x ghost columns and y ghost rows wrap
-    several times around the grid."""
+    several times around the grid (FP32 and FP64, or FP16 storage, whose
+    strips read and write through plane buffer resources)."""
     for shape, V in [((4, 3, 5), 2), ((2, 2, 2), 2), ((6, 1, 3), 1), ((1, 5, 4), 1)]:
         n = int(np.prod(shape))
         dt = courant_dt(1, 1, 1)
         st = random_state(n, 42)
-        prec = Precision.FP32 if V == 2 else Precision.FP64  # FP64 strips: V = 1
+        if storage == "fp16":
+            prec = Precision.FP16
+            st = [x * 0.25 for x in st]
+        else:
+            prec = Precision.FP32 if V == 2 else Precision.FP64  # FP64 strips: V = 1
         a = grid(shape, dt, st, prec)
         b = grid(shape, dt, st, prec)
         b.set_time_blocking_strips(2, V, 2, 4, 1)
