@@ -215,7 +215,8 @@ int fdtd_autotune(fdtd_grid* grid, int reps, int out[3], float* best_ms);
  * TYg rows (ghosts included) marching kchunk planes in z; L = 1 is the same
  * tiled kernel one step per pass (for rows too wide for the full-row
  * sweep); L = 0 returns to the full-row fused sweep.  0 picks a default for
- * V, lanes, TYg, kchunk.  Results are bit-identical to unblocked steps.
+ * V, lanes, TYg, kchunk.  Results are bit-identical to unblocked steps
+ * (FP16 with in-run rescaling: to FP16 rounding, as it rescales between passes).
  * get: {L, V, lanes, TYg, kchunk}.  fdtd_autotune_blocking times the
  * full-row sweep and tiled/blocked candidates on scratch copies of the
  * state and keeps the fastest per step (best_ms: ms per step). */
@@ -230,7 +231,8 @@ int fdtd_autotune_blocking(fdtd_grid* grid, int reps, int out[5], float* best_ms
  * chunk (0 = default).  Shapes built: V*R*sizeof(compute type) <= 16 bytes
  * with V in {1,2}, R in {2,4} (FP32/FP16 compute in float, FP64 in double).
  * Planes (Ni*Nj stored values) must be below 2 GiB (32-bit plane offsets).
- * Results are bit-identical to unblocked steps.  get: {L, V, R, W, kchunk},
+ * Results are bit-identical to unblocked steps (FP16 with in-run rescaling:
+ * to FP16 rounding, as it rescales between passes).  get: {L, V, R, W, kchunk},
  * R = W = 0 when the ghost-zone patch kernel (or no blocking) is set. */
 int fdtd_set_time_blocking_strips(fdtd_grid* grid, int L, int V, int R, int W, int kchunk);
 int fdtd_get_time_blocking_strips(const fdtd_grid* grid, int out[5]);

@@ -226,3 +226,46 @@ def test_blocking_with_dense_currents_runs_single_sweeps(kind):
         g.zeroed_currents()
         g.update_fields(4)
     assert a.checksum() == b.checksum()
+
+
+def test_strip_blocking_fp16_in_run_rescaling():
+    """FP16 strip passes with in-run rescaling (a decision every 8 steps, two
+    passes of L=4): a blocked run rescales the state in place between
+    passes, where an unblocked run scales a sweep's output, so the two agree
+    to FP16 rounding rather than bitwise.  The scales follow a growing
+    point-source field, and the result stays within 2e-2 of FP64 and close to
+    the unblocked rescaling run."""
+    N = 40
+    dt = courant_dt(1, 1, 1, factor=0.5)
+    src = [dict(i=20, j=20, k=20, comp=2, kind=1, amp=1.0, period=20 * dt, t0=60 * dt,
+                sigma=20 * dt)]
+
+    def make(prec, blocking):
+        g = FDTD(Parameters(N, N, N), dt, prec)
+        g.set_point_sources(src)
+        if prec == Precision.FP16:
+            g.autoscale(target_log2=10, magnitude_floor=4.0 * np.pi * dt * 1e-2)
+            g.set_autoscale_interval(8, 10)
+        if blocking:
+            g.set_time_blocking_strips(*blocking)
+        return g
+
+    ref = make(Precision.FP64, None)
+    ub = make(Precision.FP16, None)
+    bl = make(Precision.FP16, (4, 1, 2, 8, 0))
+    s0 = bl.get_scales()
+    for g in (ref, ub, bl):
+        g.update_fields(90)
+    s1 = bl.get_scales()
+    assert s1 != s0
+    mx = bl.max_abs()
+    for c in (0, 1, 2, 3, 4):  # Bz stays ~0 under a Jz source
+        assert 2.0 ** 6 <= mx[c] * s1[c] <= 2.0 ** 14, (c, mx[c] * s1[c])
+
+    err = bl.compare(ref)
+    r_ref = np.sqrt(sum(e[0] for e in err) / sum(e[1] for e in err))
+    r_ub = rel_l2(bl.fields(), ub.fields())
+    print("fp16 blocked rescaling: relL2 vs FP64", r_ref, "vs unblocked FP16", r_ub, s0, s1)
+    assert r_ref <= 2e-2
+    assert r_ub <= 1e-2
+
