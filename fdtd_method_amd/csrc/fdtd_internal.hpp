@@ -94,6 +94,11 @@ struct fdtd_grid {
     bool fused_ok = false;
     int autoscale_interval = 0;  // FP16: rescale every this many sweeps (0 = off)
     int autoscale_log2 = 12;
+    // FP16 strip passes: the next pass scales what it reads by these exact
+    // powers of two (the in-place rescaling of the state, folded into the
+    // pass's loads); set by run_blocked for one launch.
+    bool pass_rescale = false;
+    float pass_iscale[6] = {1, 1, 1, 1, 1, 1};
     double* d_part = nullptr;
     double* d_out = nullptr;
     unsigned* d_max = nullptr;  // per-component max |stored| tracked by a sweep (float bits)

@@ -1347,6 +1347,10 @@ struct StripArgs {
     S* ahead[3];              // FINAL passes: also store the next half step (B + d) + d here
     Coefs<C> c;
     StripGeom g;
+    // FP16 storage, RESCALE passes: every value read (E: iscale[0..2],
+    // B: iscale[3..5], exact powers of two) is scaled and rounded to FP16
+    // before use -- the same values as rescaling the state in place first
+    float iscale[6];
 };
 
 // Value of the same register in lane-1 (lane_prev) or lane+1 (lane_next) of
@@ -1377,8 +1381,9 @@ __device__ __forceinline__ double lane_next(double x) {
 }
 
 // One tile (index < a.g.ntiles) of a strip pass; every thread of the
-// workgroup calls it with the same tile.
-template <typename S, typename C, int V, int R, int L, bool FINAL>
+// workgroup calls it with the same tile.  RESCALE (FP16 storage only): the
+// pass after an in-run rescaling decision, which scales what it reads.
+template <typename S, typename C, int V, int R, int L, bool FINAL, bool RESCALE = false>
 __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int tile,
                                            C* __restrict__ lds) {
     const StripGeom& G = a.g;
@@ -1474,6 +1479,18 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                 load_pack<S, C, V>(generic(at(e0, row_off[r])), en[d][r]);
             }
         }
+    }
+    if constexpr (RESCALE) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    bP[0][d][r][v] = as_stored<S>(bP[0][d][r][v] * a.iscale[3 + d]);
+                    bn[d][r][v] = as_stored<S>(bn[d][r][v] * a.iscale[3 + d]);
+                    en[d][r][v] = as_stored<S>(en[d][r][v] * a.iscale[d]);
+                }
     }
     // E rows are read (for ghost-plane results only) before their first write
 #pragma unroll
@@ -1725,6 +1742,17 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                     widen<S, C, V>(rawB[d][r], bn[d][r]);
                     widen<S, C, V>(rawE[d][r], en[d][r]);
                 }
+            if constexpr (RESCALE) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d)
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+#pragma unroll
+                        for (int v = 0; v < V; ++v) {
+                            bn[d][r][v] = as_stored<S>(bn[d][r][v] * a.iscale[3 + d]);
+                            en[d][r][v] = as_stored<S>(en[d][r][v] * a.iscale[d]);
+                        }
+            }
         }
         if constexpr (kBuf) {
 #pragma unroll
@@ -1753,6 +1781,19 @@ strip_sweep(StripArgs<S, C> a) {
     const int tile = (blk & 7) * a.g.per_xcd + (blk >> 3);
     if (tile >= a.g.ntiles) return;  // whole workgroup
     strip_tile<S, C, V, R, L, FINAL>(a, tile, reinterpret_cast<C*>(lds_raw));
+}
+
+// FP16 storage: the strip pass after an in-run rescaling decision, reading
+// the state scaled by StripArgs::iscale (a kernel of its own, so that the
+// other passes carry none of it).
+template <typename S, typename C, int V, int R, int L, bool FINAL>
+__global__ void __launch_bounds__(strip_max_threads(V, R, L, sizeof(C)))
+strip_rescale_sweep(StripArgs<S, C> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int blk = blockIdx.x;
+    const int tile = (blk & 7) * a.g.per_xcd + (blk >> 3);
+    if (tile >= a.g.ntiles) return;  // whole workgroup
+    strip_tile<S, C, V, R, L, FINAL, true>(a, tile, reinterpret_cast<C*>(lds_raw));
 }
 
 // ---------------------------------------------------------------------------
@@ -1911,6 +1952,8 @@ __device__ __forceinline__ StripArgs<S, C> load_batch_strip_grid(const BatchStri
     a.vals = vals ? vals + uni(p->src_off) : nullptr;
     a.nsrc = uni(p->nsrc);
     a.vstride = vstride;
+#pragma unroll
+    for (int c6 = 0; c6 < 6; ++c6) a.iscale[c6] = 1.0f;
     const Coefs<C>& c = p->c;
     a.c.ex_bz = uni(c.ex_bz); a.c.ex_by = uni(c.ex_by); a.c.ey_bx = uni(c.ey_bx);
     a.c.ey_bz = uni(c.ey_bz); a.c.ez_by = uni(c.ez_by); a.c.ez_bx = uni(c.ez_bx);

@@ -647,7 +647,10 @@ int launch_strip_v(fdtd_grid* g, int slot, int in, int out, void* const* in_b, v
     if constexpr (!strip_built(sizeof(C), V, R)) {
         return fail(FDTD_ERR_UNSUPPORTED, "strip variant V=%d R=%d not built", V, R);
     } else {
+        // FP16: the pass after an in-run rescaling decision scales its reads
         auto kern = fdtd::strip_sweep<S, C, V, R, L, FINAL>;
+        if constexpr (sizeof(S) == 2)
+            if (g->pass_rescale) kern = fdtd::strip_rescale_sweep<S, C, V, R, L, FINAL>;
         const BlockTuning& t = g->btune;
         fdtd::StripArgs<S, C> a;
         a.in = fields_of<S>(g, in);
@@ -662,6 +665,7 @@ int launch_strip_v(fdtd_grid* g, int slot, int in, int out, void* const* in_b, v
         a.vstride = g->nsrc;
         a.c = make_coefs<C>(g);
         a.g = make_strip_geom(g, t);
+        for (int c = 0; c < 6; ++c) a.iscale[c] = g->pass_iscale[c];
         const size_t lds = fdtd::strip_lds_bytes(L, a.g.W, V, sizeof(C));
         if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
         hipLaunchKernelGGL(kern, dim3(8 * a.g.per_xcd), dim3(64 * a.g.W), lds, g->stream, a);
@@ -836,8 +840,23 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
     const int chunk = (kValTable / L) * L;
     int s = 0;
     for (int p = 0; p < npass; ++p, s += L) {
-        if (g->autoscale_interval > 0 && s > 0 && (s / L) % std::max(1, g->autoscale_interval / L) == 0)
-            if ((rc = rescale_to_target(g, true))) return rc;
+        if (g->autoscale_interval > 0 && s > 0 && (s / L) % std::max(1, g->autoscale_interval / L) == 0) {
+            if (strips) {
+                // strip passes scale what they read (bit-identical to
+                // rescaling the current state in place first)
+                double mx[6], snew[6];
+                if ((rc = fdtd_max_abs(g, mx))) return rc;
+                target_scales(g, mx, g->autoscale_log2, 0.0, snew);
+                for (int c = 0; c < 6; ++c) {
+                    const double f = snew[c] / g->scales[c];
+                    g->pass_iscale[c] = static_cast<float>(f);
+                    g->pass_rescale = g->pass_rescale || f != 1.0;
+                    g->scales[c] = snew[c];
+                }
+            } else if ((rc = rescale_to_target(g, true))) {
+                return rc;
+            }
+        }
         if (s % chunk == 0)
             if ((rc = eval_sources(g, g->step + s, std::min(chunk, npass * L - s)))) return rc;
         const int in = g->cur, out = 1 - g->cur;
@@ -845,6 +864,8 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
         void* const* in_b = p == 0 && carried >= 0 ? g->BA[carried] : nullptr;
         void* const* ahead = last && carry_out ? g->BA[ahead_buf] : nullptr;
         rc = launch_blocked(g, last, s % chunk, in, out, in_b, ahead);
+        g->pass_rescale = false;
+        for (int c = 0; c < 6; ++c) g->pass_iscale[c] = 1.0f;
         if (rc) return rc;
         g->cur = out;
     }
