@@ -1511,22 +1511,7 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
         wait_vmem_all();
     }
 
-    // Point-source ranges [qlo, qhi) of the planes pk the levels update.
-    // Level l at iteration kk updates plane kk-l, which level l-1 updated one
-    // iteration earlier: the ranges pass down a uniform shift register, with
-    // one table read per iteration, issued a plane ahead.
-    int pk[L], qlo[L], qhi[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-        pk[l] = wrap_mod(kfirst - l, Nk);
-        qlo[l] = a.nsrc > 0 ? kst[pk[l]] : 0;
-        qhi[l] = a.nsrc > 0 ? kst[pk[l] + 1] : 0;
-    }
-
     for (int kk = kfirst; kk <= klast; ++kk) {
-        const int pnext = pk[0] + 1 == Nk ? 0 : pk[0] + 1;
-        const int nlo = a.nsrc > 0 ? kst[pnext] : 0;
-        const int nhi = a.nsrc > 0 ? kst[pnext + 1] : 0;
         if constexpr (!kBuf) {
 #pragma unroll
             for (int v = 0; v < V; ++v) {
@@ -1585,7 +1570,7 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                 }
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            // this level's E plane is kk - l (pk[l] wrapped); its B plane is kk - l - 1
+            // this level's E plane is kk - l (pkl wrapped); its B plane is kk - l - 1
             if (L > 1) {
                 // publish the first-row Ex, Ez of the level whose readers
                 // have all passed the last barrier
@@ -1596,7 +1581,13 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                     sE(lp, 1, w)[v] = eP[lp][2][0][v];
                 }
             }
-            const int q0 = qlo[l], q1 = qhi[l];
+            // point sources of plane pkl: [kstart[pkl], kstart[pkl + 1])
+            const int pkl = wrap_mod(kk - l, Nk);
+            int q0 = 0, q1 = 0;
+            if (a.nsrc > 0) {
+                q0 = kst[pkl];
+                q1 = kst[pkl + 1];
+            }
             C enew[3][R][V];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -1611,7 +1602,7 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                 }
                 if (q1 > q0)
                     add_point_sources<C, V>(t, c, a.src, a.vals + l * a.vstride, q0, q1, gi0, gj[r],
-                                            pk[l]);
+                                            pkl);
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     const C bx_jm = r > 0 ? bin[0][r - 1][v] : sB(l, 0, wm)[v];
@@ -1761,15 +1752,6 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                 sB(0, 1, w)[v] = bn[2][R - 1][v];
             }
         }
-#pragma unroll
-        for (int l = L - 1; l > 0; --l) {
-            pk[l] = pk[l - 1];
-            qlo[l] = qlo[l - 1];
-            qhi[l] = qhi[l - 1];
-        }
-        pk[0] = pnext;
-        qlo[0] = nlo;
-        qhi[0] = nhi;
     }
 }
 
