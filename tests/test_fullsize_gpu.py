@@ -7,7 +7,7 @@ the size:
 
 * translation equivariance: on a periodic grid with uniform coefficients,
   moving the point source by one cell moves every field by one cell,
-  bitwise;
+  bitwise (FP32, and FP16 with in-run rescaling);
 * linearity under a power-of-two scale: doubling the source doubles every
   field exactly in FP64 (all operations are linear, scaling by 2 is exact,
   and the fields stay far from the subnormal range);
@@ -178,3 +178,29 @@ def test_plane_wave_stays_transversely_uniform_512(prec, blocking):
         for k in (0, 255, 511):
             assert not g.download_plane(c, k).any(), (c, k)
     g.close()
+
+
+def test_translation_equivariance_fp16_rescaling_512():
+    """FP16 storage with in-run rescaling every 20 steps through strip passes:
+    the grid-wide maxima, hence every scale decision, are the same for a
+    shifted source, so moving the source one cell moves every field one cell,
+    bitwise."""
+    n = 512
+    steps = 130
+    grids = []
+    for i in (256, 257):
+        g = make((n, n, n), headline_source(i, 256, 256), Precision.FP16, (4, 1, 2, 16, 0))
+        g.set_autoscale_interval(20, 10)
+        g.update_fields(steps)
+        grids.append(g)
+    a, b = grids
+    assert a.get_scales() == b.get_scales()
+    for c in COMPS:
+        for k in PLANES:
+            pa = a.download_plane(c, k).reshape(n, n)
+            pb = b.download_plane(c, k).reshape(n, n)
+            assert np.array_equal(np.roll(pa, 1, axis=1), pb), (c, k)
+    assert np.abs(a.download_plane(Component.EZ, 256)).max() > 0
+    a.close()
+    b.close()
+
