@@ -191,7 +191,9 @@ def test_translation_equivariance_fp16_rescaling_512():
     for i in (256, 257):
         g = make((n, n, n), headline_source(i, 256, 256), Precision.FP16, (4, 1, 2, 16, 0))
         g.set_autoscale_interval(20, 10)
+        s0 = g.get_scales()
         g.update_fields(steps)
+        assert g.get_scales() != s0  # the rescaling passes ran
         grids.append(g)
     a, b = grids
     assert a.get_scales() == b.get_scales()
