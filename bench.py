@@ -510,8 +510,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): this build has no "
-              "multi-GPU path; see the sharding design in DESIGN.md.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}): there is no "
+              "multi-GPU path (the grid would shard into z-slabs with halo-plane exchange, "
+              "not implemented).", file=sys.stderr)
         sys.exit(2)
     if a.steps is None:
         a.steps = {"batch": 500, "precision": 2000, "blocked": 4000}.get(a.workload, 1000)
