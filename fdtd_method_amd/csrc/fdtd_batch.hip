@@ -324,7 +324,10 @@ const void* strip_kernel(int precision, int V, int R, int L, bool final) {
 
 // Strip tiling of every grid (L, V, R rows per lane, W wavefronts), its
 // work list (most expensive tiles first) and the persistent grid size.
-// Each grid's z chunk aims at about 128 Ki computed cells per tile.
+// Every z chunk of a tile repeats the pipeline's 2(L-1) fill planes, so the
+// chunks are as long as the batch allows while its work list still holds at
+// least four tiles per resident workgroup (whole grids when there are many),
+// and never shorter than about 128 Ki computed cells per tile.
 int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
     const int cs = (int)compute_size(b->precision);
     if (L < 1 || L > 4 || !batch_strip_built(cs, V, R) || W < 1 || W > 16 ||
@@ -335,43 +338,11 @@ int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
     const int TX = 64 * V - 2 * Gx, TY = W * R - 2 * L;
     if (TX < 1 || TY < 1)
         return fail(FDTD_ERR_INVALID_ARGUMENT, "strip tiles of W=%d R=%d leave no stored rows", W, R);
-    std::vector<fdtd::StripGeom> geom(b->grids.size());
-    std::vector<int2> work;
-    std::vector<long long> cost;
     for (size_t q = 0; q < b->grids.size(); ++q) {
         const fdtd_grid* g = b->grids[q];
         if (g->Ni % V || (unsigned long long)g->Ni * g->Nj * g->esize >= (1ull << 31))
             return fail(FDTD_ERR_INVALID_ARGUMENT, "grid %zu cannot take strips with V=%d", q, V);
-        fdtd::StripGeom& G = geom[q];
-        G.Ni = g->Ni;
-        G.Nj = g->Nj;
-        G.Nk = g->Nk;
-        G.W = W;
-        G.Gx = Gx;
-        G.TX = TX;
-        G.TY = TY;
-        const long long area = 64LL * V * W * R;
-        int kc = (int)std::max<long long>(16, (128LL * 1024) / area);
-        kc = std::min(kc, g->Nk);
-        const int nchunk = (g->Nk + kc - 1) / kc;
-        G.kchunk = (g->Nk + nchunk - 1) / nchunk;
-        G.ntx = (g->Ni + TX - 1) / TX;
-        G.nty = (g->Nj + TY - 1) / TY;
-        G.ntiles = G.ntx * G.nty * nchunk;
-        G.per_xcd = (G.ntiles + 7) / 8;
-        for (int t = 0; t < G.ntiles; ++t) {
-            const int chunk = t / (G.ntx * G.nty);
-            const int kcount = std::min(G.kchunk, g->Nk - chunk * G.kchunk) + 2 * (L - 1);
-            work.push_back(make_int2((int)q, t));
-            cost.push_back(area * kcount);
-        }
     }
-    std::vector<size_t> order(work.size());
-    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t x, size_t y) { return cost[x] > cost[y]; });
-    std::vector<int2> sorted(work.size());
-    for (size_t i = 0; i < order.size(); ++i) sorted[i] = work[order[i]];
     const size_t lds = fdtd::strip_lds_bytes(L, W, V, cs);
     if (lds > kLdsLimit) return fail(FDTD_ERR_INVALID_ARGUMENT, "strip tiles need too much LDS");
     int per_cu = 0;
@@ -385,6 +356,48 @@ int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
     int dev = 0, cus = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t want = 4ull * per_cu * cus;
+    const long long area = 64LL * V * W * R;
+    std::vector<fdtd::StripGeom> geom(b->grids.size());
+    std::vector<int2> work;
+    std::vector<long long> cost;
+    const int kmin = (int)std::max<long long>(16, (128LL * 1024) / area);
+    for (int kmax : {1 << 30, 1024, 512, 256, 128, kmin}) {
+        if (kmax < kmin) continue;
+        work.clear();
+        cost.clear();
+        for (size_t q = 0; q < b->grids.size(); ++q) {
+            const fdtd_grid* g = b->grids[q];
+            fdtd::StripGeom& G = geom[q];
+            G.Ni = g->Ni;
+            G.Nj = g->Nj;
+            G.Nk = g->Nk;
+            G.W = W;
+            G.Gx = Gx;
+            G.TX = TX;
+            G.TY = TY;
+            const int kc = std::min(kmax, g->Nk);
+            const int nchunk = (g->Nk + kc - 1) / kc;
+            G.kchunk = (g->Nk + nchunk - 1) / nchunk;
+            G.ntx = (g->Ni + TX - 1) / TX;
+            G.nty = (g->Nj + TY - 1) / TY;
+            G.ntiles = G.ntx * G.nty * nchunk;
+            G.per_xcd = (G.ntiles + 7) / 8;
+            for (int t = 0; t < G.ntiles; ++t) {
+                const int chunk = t / (G.ntx * G.nty);
+                const int kcount = std::min(G.kchunk, g->Nk - chunk * G.kchunk) + 2 * (L - 1);
+                work.push_back(make_int2((int)q, t));
+                cost.push_back(area * kcount);
+            }
+        }
+        if (work.size() >= want || kmax == kmin) break;
+    }
+    std::vector<size_t> order(work.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return cost[x] > cost[y]; });
+    std::vector<int2> sorted(work.size());
+    for (size_t i = 0; i < order.size(); ++i) sorted[i] = work[order[i]];
     HIP_TRY(hipStreamSynchronize(b->stream));
     if (b->d_swork) HIP_TRY(hipFree(b->d_swork));
     b->d_swork = nullptr;
