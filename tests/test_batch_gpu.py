@@ -133,6 +133,36 @@ def test_batch_strips_equal_single_grid_runs(prec, strips):
             assert g1.checksum() == g2.checksum(), nsteps
 
 
+def test_batch_strips_whole_grid_chunks():
+    """A batch with enough tiles to fill the device four times over runs
+    each grid as one z chunk (here Nk = 96, which a batch of a few grids cuts
+    into two 48-plane chunks); the first grids still get their single-grid
+    fields bitwise."""
+    shape, ngrids, nsrc = (64, 160, 96), 100, 3
+    singles = [make(shape, Precision.FP32, q) for q in range(nsrc)]
+    batched = [make(shape, Precision.FP32, q) for q in range(nsrc)]
+    batched += [FDTD(Parameters(*shape), courant_dt(1, 1, 1, factor=0.5), Precision.FP32)
+                for _ in range(ngrids - nsrc)]
+    for g in singles:
+        g.set_step_carry(False)
+    b = Batch(batched)
+    b.set_time_blocking_strips(4, 1, 2, 16)
+    cfg = b.get_time_blocking_strips()
+    print("strips", cfg, flush=True)
+    # 56 stored columns and 24 stored rows per tile: 2 x 7 tiles per chunk
+    assert cfg["work_items"] == ngrids * 2 * 7
+    for g in singles:
+        g.update_fields(9)
+    b.update_fields(9)
+    for g1, g2 in zip(singles, batched):
+        assert g1.checksum() == g2.checksum()
+    for g in batched[nsrc:]:
+        assert g.checksum() == batched[-1].checksum()
+    b.close()
+    for g in singles + batched:
+        g.close()
+
+
 @pytest.mark.parametrize("nsteps,replays,L", [(8, 3, 4), (5, 3, 2)])
 def test_batch_strips_graph_replays(nsteps, replays, L):
     """Captured strip passes replayed; an odd number of launches per replay
