@@ -34,7 +34,7 @@ python bench.py --profile fp32|fp64|fp16 [--tuning V,TY,kchunk] [--blocking L,V,
     (only that path: for rocprofv3)
 
 Other workloads (not the headline; each prints one JSON line):
-python bench.py --workload batch [--grids 256] [--steps 500]
+python bench.py --workload batch [--grids 256] [--steps 500] [--blocking L,V,R,W]
     256 independent FP32 grids, each dimension from {32,48,...,160} (seeded),
     one plane wave (wavelength 8-32 cells) and one Gaussian-modulated Jz
     source each: one persistent launch per step, HIP-graph captured,
@@ -215,7 +215,10 @@ def bench_batch(a):
     ms_b, ms_s = b.time(min(K, 48))
     # persistent strip passes of L steps over every grid's tiles
     best = None
-    for cfg in ((4, 1, 2, 16), (4, 1, 2, 8), (3, 1, 2, 8), (2, 1, 2, 8), (4, 1, 4, 4)):
+    cands = ((4, 1, 2, 16), (4, 1, 2, 8), (3, 1, 2, 8), (2, 1, 2, 8), (4, 1, 4, 4))
+    if a.blocking:
+        cands = (tuple(int(x) for x in a.blocking.split(","))[:4],)
+    for cfg in cands:
         try:
             b.set_time_blocking_strips(*cfg)
         except RuntimeError as e:
@@ -501,7 +504,8 @@ def main():
     ap.add_argument("--tuning", default=None,
                     help="V,TY,kchunk launch shape for --profile (default: autotune)")
     ap.add_argument("--blocking", default=None,
-                    help="L,V,R,W[,kchunk] strip blocking for --profile (default: none)")
+                    help="L,V,R,W[,kchunk] strip blocking for --profile (default: none); "
+                         "L,V,R,W for --workload batch (default: the best of five)")
     ap.add_argument("--workload", choices=["point", "batch", "precision", "blocked", "pml"],
                     default="point")
     ap.add_argument("--pml", type=float, default=0.2, help="pml_percent of --workload pml")
