@@ -10,6 +10,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
+#include <queue>
 #include <vector>
 
 #include "fdtd_internal.hpp"
@@ -324,10 +326,13 @@ const void* strip_kernel(int precision, int V, int R, int L, bool final) {
 
 // Strip tiling of every grid (L, V, R rows per lane, W wavefronts), its
 // work list (most expensive tiles first) and the persistent grid size.
-// Every z chunk of a tile repeats the pipeline's 2(L-1) fill planes, so the
-// chunks are as long as the batch allows while its work list still holds at
-// least four tiles per resident workgroup (whole grids when there are many),
-// and never shorter than about 128 Ki computed cells per tile.
+// Every z chunk of a tile repeats the pipeline's 2(L-1) fill planes, while
+// fewer, longer tiles fill the device's resident workgroups in coarser
+// rounds.  For each candidate chunk length (whole grids down to about
+// 128 Ki computed cells per tile) the queue's schedule is simulated, costliest
+// tile first onto the workgroup that frees first; a shorter chunk length is
+// taken only when it shortens the makespan by more than 2%, since each tile
+// also has costs outside the model (descriptor loads, the queue, ramp-up).
 int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
     const int cs = (int)compute_size(b->precision);
     if (L < 1 || L > 4 || !batch_strip_built(cs, V, R) || W < 1 || W > 16 ||
@@ -356,19 +361,20 @@ int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
     int dev = 0, cus = 0;
     HIP_TRY(hipGetDevice(&dev));
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t want = 4ull * per_cu * cus;
+    const size_t slots = (size_t)per_cu * cus;
     const long long area = 64LL * V * W * R;
-    std::vector<fdtd::StripGeom> geom(b->grids.size());
-    std::vector<int2> work;
-    std::vector<long long> cost;
     const int kmin = (int)std::max<long long>(16, (128LL * 1024) / area);
-    for (int kmax : {1 << 30, 1024, 512, 256, 128, kmin}) {
+    std::vector<fdtd::StripGeom> geom(b->grids.size()), try_geom(b->grids.size());
+    std::vector<int2> work, try_work;
+    std::vector<long long> cost, try_cost;
+    long long best_span = -1;
+    for (int kmax : {1 << 30, 512, 256, 192, 128, 96, kmin}) {
         if (kmax < kmin) continue;
-        work.clear();
-        cost.clear();
+        try_work.clear();
+        try_cost.clear();
         for (size_t q = 0; q < b->grids.size(); ++q) {
             const fdtd_grid* g = b->grids[q];
-            fdtd::StripGeom& G = geom[q];
+            fdtd::StripGeom& G = try_geom[q];
             G.Ni = g->Ni;
             G.Nj = g->Nj;
             G.Nk = g->Nk;
@@ -386,11 +392,27 @@ int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
             for (int t = 0; t < G.ntiles; ++t) {
                 const int chunk = t / (G.ntx * G.nty);
                 const int kcount = std::min(G.kchunk, g->Nk - chunk * G.kchunk) + 2 * (L - 1);
-                work.push_back(make_int2((int)q, t));
-                cost.push_back(area * kcount);
+                try_work.push_back(make_int2((int)q, t));
+                try_cost.push_back(area * kcount);
             }
         }
-        if (work.size() >= want || kmax == kmin) break;
+        std::vector<long long> sorted_cost = try_cost;
+        std::sort(sorted_cost.begin(), sorted_cost.end(), std::greater<long long>());
+        std::priority_queue<long long, std::vector<long long>, std::greater<long long>> free_at;
+        for (size_t i = 0; i < std::min(slots, sorted_cost.size()); ++i) free_at.push(0);
+        long long span = 0;
+        for (long long c : sorted_cost) {
+            const long long start = free_at.top();
+            free_at.pop();
+            free_at.push(start + c);
+            span = std::max(span, start + c);
+        }
+        if (best_span < 0 || span * 50 < best_span * 49) {
+            best_span = span;
+            geom.swap(try_geom);
+            work.swap(try_work);
+            cost.swap(try_cost);
+        }
     }
     std::vector<size_t> order(work.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;

@@ -134,11 +134,12 @@ def test_batch_strips_equal_single_grid_runs(prec, strips):
 
 
 def test_batch_strips_whole_grid_chunks():
-    """A batch with enough tiles to fill the device four times over runs
-    each grid as one z chunk (here Nk = 96, which a batch of a few grids cuts
-    into two 48-plane chunks); the first grids still get their single-grid
-    fields bitwise."""
-    shape, ngrids, nsrc = (64, 160, 96), 100, 3
+    """A batch whose whole-grid tiles just fill four rounds of the device's
+    256 resident workgroups (73 grids x 14 tiles = 1022) runs each grid as
+    one z chunk (Nk = 96, which a batch of a few grids cuts into two
+    48-plane chunks); the first grids still get their single-grid fields
+    bitwise."""
+    shape, ngrids, nsrc = (64, 160, 96), 73, 3
     singles = [make(shape, Precision.FP32, q) for q in range(nsrc)]
     batched = [make(shape, Precision.FP32, q) for q in range(nsrc)]
     batched += [FDTD(Parameters(*shape), courant_dt(1, 1, 1, factor=0.5), Precision.FP32)
