@@ -27,7 +27,8 @@ EXPORTED = [
     "fdtd_download_plane",
     "fdtd_zero_currents", "fdtd_set_current_values", "fdtd_set_point_sources",
     "fdtd_update_fields", "fdtd_update_fields_3sweep", "fdtd_get_step", "fdtd_set_step",
-    "fdtd_synchronize", "fdtd_get_info", "fdtd_sum_squares", "fdtd_max_abs", "fdtd_compare",
+    "fdtd_synchronize", "fdtd_get_info", "fdtd_sum_squares", "fdtd_sum_squares_noalloc",
+    "fdtd_max_abs", "fdtd_compare",
     "fdtd_checksum", "fdtd_add_plane_wave",
     "fdtd_set_scales", "fdtd_get_scales", "fdtd_autoscale", "fdtd_set_autoscale_interval",
     "fdtd_zero_fields", "fdtd_set_step_carry", "fdtd_set_time_blocking", "fdtd_get_time_blocking",
@@ -104,6 +105,7 @@ def load():
         "fdtd_get_info": ([_vp, ctypes.POINTER(_i), ctypes.POINTER(_i),
                            ctypes.POINTER(ctypes.c_longlong)], _i),
         "fdtd_sum_squares": ([_vp, _dp], _i),
+        "fdtd_sum_squares_noalloc": ([_vp, _dp], _i),
         "fdtd_max_abs": ([_vp, _dp], _i),
         "fdtd_compare": ([_vp, _vp, _dp], _i),
         "fdtd_checksum": ([_vp, ctypes.POINTER(ctypes.c_ulonglong)], _i),

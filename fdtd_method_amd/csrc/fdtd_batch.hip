@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <functional>
 #include <queue>
@@ -56,6 +57,11 @@ struct fdtd_batch {
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     int graph_swaps = 0;
+    // What a captured graph bakes in besides its step count: source count
+    // (value-table stride and eval_sources size), launch sizes and every
+    // device buffer its launches read.  A replay re-captures when any of
+    // them differs from the batch's current values.
+    std::vector<long long> graph_key;
     hipEvent_t ev[4] = {};
     // temporal blocking: strip passes of sL steps (0: single sweeps only)
     int sL = 0, sV = 1, sR = 2, sW = 8;
@@ -72,6 +78,23 @@ struct fdtd_batch {
 };
 
 namespace {
+
+void drop_graph(fdtd_batch* b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    b->graph_steps = 0;
+    b->graph_key.clear();
+}
+
+// The values a graph captured for nsteps steps depends on (see graph_key).
+std::vector<long long> graph_key_of(const fdtd_batch* b, int nsteps) {
+    auto addr = [](const void* p) { return static_cast<long long>(reinterpret_cast<uintptr_t>(p)); };
+    return {nsteps, b->total_src, b->sL, b->sV, b->sR, b->sW, b->snwork, b->snblocks,
+            b->nwork, b->nblocks, b->shape.V, b->shape.NT,
+            addr(b->d_src), addr(b->d_vals), addr(b->d_kstart), addr(b->d_grids),
+            addr(b->d_sgrids), addr(b->d_work), addr(b->d_swork), addr(b->d_counters),
+            addr(b->d_step)};
+}
 
 int batch_v_candidates(int precision, int* out) {
     if (precision == FDTD_FP64) { out[0] = 2; out[1] = 1; return 2; }
@@ -421,6 +444,7 @@ int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
     std::vector<int2> sorted(work.size());
     for (size_t i = 0; i < order.size(); ++i) sorted[i] = work[order[i]];
     HIP_TRY(hipStreamSynchronize(b->stream));
+    drop_graph(b);  // before freeing anything a captured graph reads
     if (b->d_swork) HIP_TRY(hipFree(b->d_swork));
     b->d_swork = nullptr;
     b->sL = 0;  // until the new work list exists
@@ -435,11 +459,6 @@ int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
     b->sV = V;
     b->sR = R;
     b->sW = W;
-    if (b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-        b->graph_steps = 0;
-    }
     return FDTD_OK;
 }
 
@@ -474,6 +493,7 @@ int apply_shape(fdtd_batch* b, const BatchShape& sh) {
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const std::vector<int4> work = make_work(b, plan);
     HIP_TRY(hipStreamSynchronize(b->stream));
+    drop_graph(b);  // before freeing anything a captured graph reads
     if (b->d_work) HIP_TRY(hipFree(b->d_work));
     b->d_work = nullptr;
     HIP_TRY(hipMalloc(&b->d_work, sizeof(int4) * work.size()));
@@ -483,11 +503,6 @@ int apply_shape(fdtd_batch* b, const BatchShape& sh) {
     b->lds_bytes = lds;
     b->nwork = (int)work.size();
     b->nblocks = std::max(1, std::min(b->nwork, per_cu * cus));
-    if (b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-        b->graph_steps = 0;
-    }
     return FDTD_OK;
 }
 
@@ -518,6 +533,7 @@ int prepare(fdtd_batch* b) {
     off[b->grids.size()] = (int)all.size();
     HIP_TRY(hipStreamSynchronize(b->stream));
     if ((int)all.size() > b->src_cap) {
+        drop_graph(b);
         if (b->d_src) HIP_TRY(hipFree(b->d_src));
         if (b->d_vals) HIP_TRY(hipFree(b->d_vals));
         b->d_src = nullptr;
@@ -527,27 +543,18 @@ int prepare(fdtd_batch* b) {
         HIP_TRY(hipMalloc(&b->d_src, sizeof(fdtd::PointSource) * cap));
         HIP_TRY(hipMalloc(&b->d_vals, sizeof(double) * cap * kValTable));
         b->src_cap = cap;
-        if (b->graph) {
-            (void)hipGraphExecDestroy(b->graph);
-            b->graph = nullptr;
-            b->graph_steps = 0;
-        }
     }
     if (!all.empty())
         HIP_TRY(hipMemcpy(b->d_src, all.data(), sizeof(fdtd::PointSource) * all.size(),
                           hipMemcpyHostToDevice));
     b->total_src = (int)all.size();
     if (kst.size() > b->kstart_cap) {
+        drop_graph(b);
         if (b->d_kstart) HIP_TRY(hipFree(b->d_kstart));
         b->d_kstart = nullptr;
         b->kstart_cap = 0;
         HIP_TRY(hipMalloc(&b->d_kstart, sizeof(int) * kst.size()));
         b->kstart_cap = kst.size();
-        if (b->graph) {
-            (void)hipGraphExecDestroy(b->graph);
-            b->graph = nullptr;
-            b->graph_steps = 0;
-        }
     }
     HIP_TRY(hipMemcpy(b->d_kstart, kst.data(), sizeof(int) * kst.size(), hipMemcpyHostToDevice));
     switch (b->precision) {
@@ -562,6 +569,7 @@ int prepare(fdtd_batch* b) {
             default: fill_strip_descriptors<_Float16>(b, off, kst_off);
         }
         if (b->h_sgrids.size() != b->sgrid_desc_bytes) {
+            drop_graph(b);
             if (b->d_sgrids) HIP_TRY(hipFree(b->d_sgrids));
             b->d_sgrids = nullptr;
             b->sgrid_desc_bytes = 0;
@@ -572,6 +580,7 @@ int prepare(fdtd_batch* b) {
                           hipMemcpyHostToDevice));
     }
     if (b->h_grids.size() != b->grid_desc_bytes) {
+        drop_graph(b);
         if (b->d_grids) HIP_TRY(hipFree(b->d_grids));
         b->d_grids = nullptr;
         b->grid_desc_bytes = 0;
@@ -766,11 +775,8 @@ int fdtd_batch_update_fields_graph(fdtd_batch* b, int nsteps, int replays) {
     if (!b || nsteps <= 0 || replays < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
     if (int rc = check_steps(b)) return rc;
     if (int rc = prepare(b)) return rc;
-    if (!b->graph || b->graph_steps != nsteps) {
-        if (b->graph) {
-            HIP_TRY(hipGraphExecDestroy(b->graph));
-            b->graph = nullptr;
-        }
+    if (!b->graph || graph_key_of(b, nsteps) != b->graph_key) {
+        drop_graph(b);
         hipGraph_t graph = nullptr;
         HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         int swaps = 0;
@@ -786,6 +792,7 @@ int fdtd_batch_update_fields_graph(fdtd_batch* b, int nsteps, int replays) {
         if (e != hipSuccess) return fail(FDTD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
         b->graph_steps = nsteps;
         b->graph_swaps = swaps;
+        b->graph_key = graph_key_of(b, nsteps);
     }
     const long long step0 = b->grids[0]->step;
     HIP_TRY(hipMemcpy(b->d_step, &step0, sizeof step0, hipMemcpyHostToDevice));
@@ -794,8 +801,11 @@ int fdtd_batch_update_fields_graph(fdtd_batch* b, int nsteps, int replays) {
             finish_steps(b, nsteps, b->graph_swaps);
             // an odd number of launches swaps every grid's buffers: re-point
             // the descriptors (prepare() waits for the previous replay first)
-            if (b->graph_swaps & 1)
+            if (b->graph_swaps & 1) {
                 if (int rc = prepare(b)) return rc;
+                if (graph_key_of(b, nsteps) != b->graph_key)
+                    return fail(FDTD_ERR_HIP, "batch buffers changed between graph replays");
+            }
         }
         HIP_TRY(hipGraphLaunch(b->graph, b->stream));
     }
@@ -814,11 +824,7 @@ int fdtd_batch_set_time_blocking_strips(fdtd_batch* b, int L, int V, int R, int 
     if (!b) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
     if (L <= 0) {
         b->sL = 0;
-        if (b->graph) {
-            (void)hipGraphExecDestroy(b->graph);
-            b->graph = nullptr;
-            b->graph_steps = 0;
-        }
+        drop_graph(b);
         return FDTD_OK;
     }
     return apply_strips(b, L, V > 0 ? V : 1, R > 0 ? R : 2, W > 0 ? W : 16);

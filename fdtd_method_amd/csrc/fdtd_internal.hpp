@@ -71,6 +71,7 @@ struct fdtd_grid {
     double* staging = nullptr;
     double scales[6] = {1, 1, 1, 1, 1, 1};
     hipStream_t stream = nullptr;
+    int device = 0;  // the HIP device current when the grid was created
     long long step = 0;
     std::map<long long, double> sparse_j;  // key = index*3 + comp
     std::vector<fdtd::PointSource> waveforms;

@@ -1018,12 +1018,15 @@ void* field_ptr(fdtd_grid* g, int c);
 
 // sum x^2 and max|x| of the six E and B components (physical units): all
 // partial and final passes queued back to back, one copy and one wait.
-int reduce_fields(fdtd_grid* g, double sumsq[6], double maxabs[6]) {
+// parts: 12 * kReduceBlocks doubles of per-workgroup partials (each one
+// written before it is read); res: the 12 final values.
+int reduce_fields_into(fdtd_grid* g, double* parts, double* res, double sumsq[6],
+                       double maxabs[6]) {
     const int nb = std::min<int>(kReduceBlocks, grid_blocks(g->N));
     for (int c = 0; c < 6; ++c) {
         const void* p = field_ptr(g, c);
         const double inv = 1.0 / g->scales[c];
-        double* part = g->d_part + (size_t)2 * kReduceBlocks * c;
+        double* part = parts + (size_t)2 * kReduceBlocks * c;
         switch (g->precision) {
             case FDTD_FP64:
                 hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<double>, dim3(nb), dim3(256), 0,
@@ -1038,17 +1041,21 @@ int reduce_fields(fdtd_grid* g, double sumsq[6], double maxabs[6]) {
                                    g->stream, static_cast<const _Float16*>(p), g->N, inv, part);
         }
         hipLaunchKernelGGL(fdtd::reduce_partials, dim3(1), dim3(256), 0, g->stream, part, nb, 2,
-                           2u, g->d_out + 2 * c);
+                           2u, res + 2 * c);
     }
     HIP_TRY(hipGetLastError());
     double r[12];
-    HIP_TRY(hipMemcpyAsync(r, g->d_out, sizeof r, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(r, res, sizeof r, hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
     for (int c = 0; c < 6; ++c) {
         if (sumsq) sumsq[c] = r[2 * c];
         if (maxabs) maxabs[c] = r[2 * c + 1];
     }
     return FDTD_OK;
+}
+
+int reduce_fields(fdtd_grid* g, double sumsq[6], double maxabs[6]) {
+    return reduce_fields_into(g, g->d_part, g->d_out, sumsq, maxabs);
 }
 
 int check_component(int c) {
@@ -1061,6 +1068,44 @@ void* field_ptr(fdtd_grid* g, int c) {
 }
 
 }  // namespace fdtd::rt
+
+// ---- allocation-free diagnostics --------------------------------------------
+// Scratch of fdtd_sum_squares_noalloc: per-workgroup partial sums and maxima
+// of the six components (at most kReduceBlocks workgroups each, every one
+// written before it is read) and the 12 final values: 98,432 bytes.  Each GPU
+// holds its own copy until the process ends; every call on a GPU shares it,
+// so calls that overlap on one GPU (two host threads, or grids on different
+// streams) race.  Host code uses the per-device address from
+// hipGetSymbolAddress, never the array's name.
+namespace {
+
+constexpr size_t kNoAllocScratchDoubles = 12 * fdtd::rt::kReduceBlocks + 16;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on device dev, which must be current; looked
+// up on the first call on each device.
+int scratch_on_device(int dev, double** out) {
+    if (dev < 0 || dev >= kMaxDevices)
+        return fdtd::rt::fail(FDTD_ERR_UNSUPPORTED, "device %d beyond the %d supported", dev,
+                              kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fdtd::rt::fail(FDTD_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                                  hipGetErrorString(e));
+        g_scratch_addr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_addr[dev];
+    return FDTD_OK;
+}
+
+}  // namespace
 
 // =============================================================================
 // C-ABI
@@ -1111,6 +1156,8 @@ int create_grid(const fdtd_parameters* p, double dt, double pml_percent, int pre
         fdtd_destroy(g);
         return rc;
     };
+    if (hipGetDevice(&g->device) != hipSuccess)
+        return cleanup(fail(FDTD_ERR_HIP, "hipGetDevice failed"));
     if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess)
         return cleanup(fail(FDTD_ERR_HIP, "hipStreamCreate failed"));
     const size_t bytes = g->N * g->esize + 16;
@@ -1497,6 +1544,23 @@ int fdtd_sum_squares(fdtd_grid* g, double out[6]) {
 int fdtd_max_abs(fdtd_grid* g, double out[6]) {
     if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
     return reduce_fields(g, nullptr, out);
+}
+
+int fdtd_sum_squares_noalloc(fdtd_grid* g, double out[6]) {
+    if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    const int nb = std::min<int>(kReduceBlocks, grid_blocks(g->N));
+    if ((size_t)nb * 12 + 12 > kNoAllocScratchDoubles)
+        return fail(FDTD_ERR_INVALID_ARGUMENT,
+                    "%d partial sums exceed the allocation-free scratch of %zu doubles", nb * 12,
+                    kNoAllocScratchDoubles);
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != g->device) HIP_TRY(hipSetDevice(g->device));
+    double* s = nullptr;
+    int rc = scratch_on_device(g->device, &s);
+    if (!rc) rc = reduce_fields_into(g, s, s + 12 * kReduceBlocks, out, nullptr);
+    if (cur != g->device) (void)hipSetDevice(cur);
+    return rc;
 }
 
 int fdtd_compare(fdtd_grid* t, fdtd_grid* r, double out[18]) {

@@ -161,6 +161,16 @@ int fdtd_get_info(const fdtd_grid* grid, int dims[3], int* precision, long long*
  * sum of squares (field energy is their total / (8 pi) * dx*dy*dz) and max |x|. */
 int fdtd_sum_squares(fdtd_grid* grid, double out[6]);
 int fdtd_max_abs(fdtd_grid* grid, double out[6]);
+/* fdtd_sum_squares without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory, such as tight
+ * diagnostic loops and code that must not call hipMalloc.  Bit-identical to
+ * fdtd_sum_squares.  Its partial sums live in one fixed 98,432-byte device
+ * array per GPU (the library's, held until the process ends), which fits
+ * every grid size (at most 1024 workgroups of partials per component; a
+ * larger need would be rejected with FDTD_ERR_INVALID_ARGUMENT).  All calls
+ * on one GPU share that array: calls that overlap on one GPU (from several
+ * host threads) race.  Synchronises the grid's stream. */
+int fdtd_sum_squares_noalloc(fdtd_grid* grid, double out[6]);
 /* Error of grid `test` (any precision) against FP64 grid `ref` of the same
  * shape, per component c: out[3c] = sum (t-r)^2, out[3c+1] = sum r^2,
  * out[3c+2] = max |t-r|. */

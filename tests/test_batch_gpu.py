@@ -193,3 +193,98 @@ def test_batch_strips_reject_invalid_shapes():
         with pytest.raises((ValueError, RuntimeError)):
             b.set_time_blocking_strips(*args)
     assert b.get_time_blocking_strips()["L"] == 0
+
+
+@pytest.mark.parametrize("strips", [None, (4, 1, 2, 8)])
+def test_batch_graph_recaptures_after_source_change(strips):
+    """A captured graph bakes in the batch's source count (value-table
+    stride, evaluation size): changing one grid's point sources between two
+    graph calls of the same length (staying under the source-buffer
+    capacity) must re-capture, and every grid keeps its single-grid fields
+    bitwise."""
+    singles = [make(s, Precision.FP32, q) for q, s in enumerate(SHAPES)]
+    batched = [make(s, Precision.FP32, q) for q, s in enumerate(SHAPES)]
+    b = Batch(batched)
+    if strips:
+        b.set_time_blocking_strips(*strips)
+        for g in singles:
+            g.set_step_carry(False)
+    b.update_fields_graph(8, 2)
+    for g in singles:
+        g.update_fields(16)
+    dt = courant_dt(1, 1, 1, factor=0.5)
+    extra = [dict(i=1, j=2, k=0, comp=0, kind=1, amp=0.5, period=12 * dt, t0=4 * dt, sigma=0.0),
+             dict(i=3, j=0, k=1, comp=1, kind=0, amp=0.25)]
+    for grids in (singles, batched):
+        s0 = SHAPES[0]
+        grids[0].set_point_sources([dict(i=s0[0] // 2, j=s0[1] // 3, k=s0[2] - 1, comp=2, kind=1,
+                                         amp=1.0, period=20 * dt, t0=10 * dt, sigma=8 * dt)]
+                                   + extra)
+        grids[3].set_point_sources([])
+    b.update_fields_graph(8, 2)
+    for g in singles:
+        g.update_fields(16)
+    for q, (g1, g2) in enumerate(zip(singles, batched)):
+        assert g1.checksum() == g2.checksum(), q
+        assert g1.step == g2.step == 32
+
+
+@pytest.mark.parametrize("ngrids", [3, 100])
+def test_batch_strips_chunked_grids(ngrids):
+    """Batched strip passes with more than one z chunk per grid: a few
+    (64, 160, 96) grids are cut into z chunks so that their tiles fill the
+    device's resident workgroups, and so are 100 of them (the schedule
+    model prefers two 48-plane chunks there).  The first grids carry data
+    and must equal their single-grid runs bitwise."""
+    shape, nsrc = (64, 160, 96), 3
+    singles = [make(shape, Precision.FP32, q) for q in range(nsrc)]
+    batched = [make(shape, Precision.FP32, q) for q in range(nsrc)]
+    batched += [FDTD(Parameters(*shape), courant_dt(1, 1, 1, factor=0.5), Precision.FP32)
+                for _ in range(ngrids - nsrc)]
+    for g in singles:
+        g.set_step_carry(False)
+    b = Batch(batched)
+    b.set_time_blocking_strips(4, 1, 2, 16)
+    cfg = b.get_time_blocking_strips()
+    print("strips", cfg, flush=True)
+    # 2 x 7 tiles per z chunk, at least two chunks per grid
+    assert cfg["work_items"] % (ngrids * 2 * 7) == 0
+    assert cfg["work_items"] >= ngrids * 2 * 7 * 2
+    for g in singles:
+        g.update_fields(11)
+    b.update_fields(11)
+    for g1, g2 in zip(singles, batched):
+        assert g1.checksum() == g2.checksum()
+    b.close()
+    for g in singles + batched:
+        g.close()
+
+
+def test_batch_256_grid_workload_graph_strips():
+    """The 256-grid seeded workload (dims in {32, 48, ..., 160}, seed 2024,
+    one plane wave and one Gaussian-modulated Jz source per grid), 500
+    steps as graph-captured batched strip passes (L = 4, 20-step graph
+    replayed 25 times): the 8 largest and 8 smallest grids equal their own
+    single-grid runs bitwise."""
+    grids, desc = plane_wave_batch(n_grids=256, seed=2024)
+    cells = [g.cells for g in grids]
+    order = np.argsort(cells, kind="stable")
+    picked = [int(q) for q in list(order[:8]) + list(order[-8:])]
+    ref, _ = plane_wave_batch(n_grids=256, seed=2024)
+    keep = [ref[q] for q in picked]
+    for q, g in enumerate(ref):
+        if q not in picked:
+            g.close()
+    b = Batch(grids)
+    b.set_time_blocking_strips(4, 1, 2, 16)
+    print("strips", b.get_time_blocking_strips(), "cells", sum(cells), flush=True)
+    b.update_fields_graph(20, 25)
+    for g in keep:
+        g.set_step_carry(False)
+        g.update_fields(500)
+    for q, g1 in zip(picked, keep):
+        assert g1.checksum() == grids[q].checksum(), (q, desc[q]["shape"])
+        assert grids[q].step == 500
+    b.close()
+    for g in grids + keep:
+        g.close()

@@ -116,4 +116,6 @@ def test_host_writes_scales_and_batches_drop_the_carried_step():
     ref = grid(shape, dt, Precision.FP64, [a * 1e-3 for a in st])
     ref.update_fields(3)
     err = f.compare(ref)
-    assert np.sqrt(sum(e[0] for e in err) / sum(e[1] for e in err)) < 2e-2
+    rel = np.sqrt(sum(e[0] for e in err) / sum(e[1] for e in err))
+    print("fp16 after a scale change: relL2 vs FP64", rel)
+    assert rel < 5e-3

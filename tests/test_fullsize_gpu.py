@@ -39,12 +39,26 @@ def headline_source(i, j, k, amp=1.0):
                 sigma=40 * DT, phase=0.0)
 
 
-def make(shape, src, prec=Precision.FP32, blocking=None):
+# Modes seeded before stepping in the bitwise blocked-vs-single comparisons,
+# so that every tile, wavefront boundary, z chunk and the periodic wrap
+# carry non-zero data (a point source alone reaches a few dozen cells).
+SEED_WAVES = [dict(m=(3, 1, 2), amp=0.5, phase=0.3, pol=(0.0, 1.0, 0.2)),
+              dict(m=(-2, 5, 1), amp=0.3, phase=1.1, pol=(1.0, 0.0, 0.5)),
+              dict(m=(1, -4, 7), amp=0.2, phase=2.0, pol=(0.3, 0.6, 1.0))]
+
+
+def make(shape, src, prec=Precision.FP32, blocking=None, waves=False):
     g = FDTD(Parameters(*shape), DT, prec)
+    if waves:
+        if prec == Precision.FP16:
+            g.set_scales([2.0 ** 10] * 6)
+        for w in SEED_WAVES:
+            g.add_plane_wave(w["m"], w["amp"], w["phase"], w["pol"])
     g.set_point_sources([src])
     if prec == Precision.FP16:
-        # fields start at zero: scales sized for one step's injection, as the
-        # benchmark does, so that FP16 holds the fields
+        # scales sized for the seeded modes, or (fields starting at zero) for
+        # one step's injection, as the benchmark does, so that FP16 holds the
+        # fields
         g.autoscale(target_log2=10, magnitude_floor=4.0 * np.pi * DT * 1e-2)
     if blocking:
         g.set_time_blocking_strips(*blocking)
@@ -98,13 +112,19 @@ def test_linearity_512():
 def test_blocked_equals_single_sweeps_512(prec, blocking):
     n = 512
     src = headline_source(256, 256, 256)
-    a = make((n, n, n), src, prec)
-    b = make((n, n, n), src, prec, blocking)
+    a = make((n, n, n), src, prec, waves=True)
+    b = make((n, n, n), src, prec, blocking, waves=True)
+    assert a.checksum() == b.checksum()
     L = blocking[0]
     for steps in (4 * L + 1, L):  # remainder sweeps, then whole passes with the carried step
         a.update_fields(steps)
         b.update_fields(steps)
         assert a.checksum() == b.checksum(), steps
+    # every tile and z chunk carried data: no plane of any component is zero
+    for c in (Component.EX, Component.EY, Component.EZ, Component.BX, Component.BY,
+              Component.BZ):
+        for k in (0, 63, 64, 200, 511):
+            assert np.count_nonzero(b.download_plane(c, k)) > 0.99 * n * n, (c, k)
     a.close()
     b.close()
 
@@ -112,8 +132,8 @@ def test_blocked_equals_single_sweeps_512(prec, blocking):
 def test_blocked_equals_single_sweeps_1024x1024x256():
     shape = (1024, 1024, 256)
     src = headline_source(300, 700, 100)
-    a = make(shape, src)
-    b = make(shape, src, blocking=(4, 1, 2, 16, 0))
+    a = make(shape, src, waves=True)
+    b = make(shape, src, blocking=(4, 1, 2, 16, 0), waves=True)
     a.update_fields(13)
     b.update_fields(13)
     assert a.checksum() == b.checksum()

@@ -329,7 +329,7 @@ def test_fp32_fused_equals_fp32_three_sweep():
 
 def test_fp16_plane_wave_error():
     """FP16 storage (FP32 compute) on a smooth plane wave: relative L2 error
-    against FP64 after 100 steps stays below 2e-2."""
+    against FP64 after 100 steps stays below 5e-3 (measured 1.5e-3)."""
     N = 48
     n = N ** 3
     st = config1_state(N)
@@ -342,7 +342,7 @@ def test_fp16_plane_wave_error():
     err = g16.compare(g64)
     rel = np.sqrt(sum(e[0] for e in err) / sum(e[1] for e in err))
     print("fp16 plane wave relL2", rel)
-    assert rel <= 2e-2
+    assert rel <= 5e-3
 
 
 def test_fp16_scales_are_exact_and_restored():
@@ -374,9 +374,7 @@ def test_fp16_in_run_rescaling():
     decision point tracks max |stored|, the next one stores its output
     multiplied by new/old power-of-two factors): a growing point-source field
     keeps every component's stored max in range (scales move with it) and the
-    result stays within 2e-2 of FP64 -- the same bound as without rescaling
-    when the scales are fixed well (checked on the same run with one manual
-    autoscale)."""
+    result stays within 5e-3 of FP64, about three times the measured error."""
     N = 40
     dt = courant_dt(1, 1, 1, factor=0.5)
     src = [dict(i=20, j=20, k=20, comp=2, kind=1, amp=1.0, period=20 * dt, t0=60 * dt,
@@ -401,7 +399,7 @@ def test_fp16_in_run_rescaling():
     err = g.compare(ref)
     rel = np.sqrt(sum(e[0] for e in err) / sum(e[1] for e in err))
     print("fp16 in-run rescaling relL2", rel, s0, s1)
-    assert rel <= 2e-2
+    assert rel <= 5e-3
 
 
 def test_diagnostics_match_oracle():

@@ -5,7 +5,7 @@ FP64 is bit-identical to the oracle (which matches the reference's compiled
 PML class bit for bit) for the fused and three-sweep paths: totals and all
 twelve split fields, on ragged shapes, layer widths 0..N/2 and every launch
 shape.  FP32 is bit-identical between its fused and three-sweep paths and
-within 1e-5 relative L2 of FP64; FP16 within 2e-2.
+within 1e-5 relative L2 of FP64; FP16 within 5e-3.
 """
 import numpy as np
 import pytest
@@ -233,7 +233,7 @@ def test_pml_fp32_fused_equals_3sweep_and_tracks_fp64():
 
 def test_pml_fp16_with_scales_tracks_fp64():
     """FP16 storage with per-component power-of-two scales (autoscaled in the
-    run; split fields share their totals' scales): error within 2e-2 of the
+    run; split fields share their totals' scales): error within 5e-3 of the
     FP64 field norm while the pulse is in the grid, and the layer still
     absorbs."""
     shape, pml, spacing = (48, 40, 36), 0.2, (1.0, 1.0, 1.0)
@@ -253,7 +253,7 @@ def test_pml_fp16_with_scales_tracks_fp64():
     h.update_fields(300)
     e1 = sum(h.sum_squares())
     print("fp16 vs fp64 rel L2", err, "energy", e0, e1, h.get_scales())
-    assert err < 2e-2
+    assert err < 5e-3
     assert e1 < 1e-2 * e0
 
 

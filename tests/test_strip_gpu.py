@@ -233,7 +233,7 @@ def test_strip_blocking_fp16_in_run_rescaling():
     passes of L=4): a blocked run rescales the state in place between
     passes, where an unblocked run scales a sweep's output, so the two agree
     to FP16 rounding rather than bitwise.  The scales follow a growing
-    point-source field, and the result stays within 2e-2 of FP64 and close to
+    point-source field, and the result stays within 5e-3 of FP64 and close to
     the unblocked rescaling run."""
     N = 40
     dt = courant_dt(1, 1, 1, factor=0.5)
@@ -266,6 +266,6 @@ def test_strip_blocking_fp16_in_run_rescaling():
     r_ref = np.sqrt(sum(e[0] for e in err) / sum(e[1] for e in err))
     r_ub = rel_l2(bl.fields(), ub.fields())
     print("fp16 blocked rescaling: relL2 vs FP64", r_ref, "vs unblocked FP16", r_ub, s0, s1)
-    assert r_ref <= 2e-2
-    assert r_ub <= 1e-2
+    assert r_ref <= 5e-3
+    assert r_ub <= 5e-3
 
