@@ -362,8 +362,9 @@ int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
         64 * W > fdtd::strip_max_threads(V, R, L, cs))
         return fail(FDTD_ERR_INVALID_ARGUMENT, "strip blocking L=%d V=%d R=%d W=%d not available",
                     L, V, R, W);
-    const int Gx = (L + V - 1) / V * V;
-    const int TX = 64 * V - 2 * Gx, TY = W * R - 2 * L;
+    int Gx = 0, TX = 0;
+    strip_x_geometry(L, V, storage_size(b->precision), &Gx, &TX);
+    const int TY = W * R - 2 * L;
     if (TX < 1 || TY < 1)
         return fail(FDTD_ERR_INVALID_ARGUMENT, "strip tiles of W=%d R=%d leave no stored rows", W, R);
     for (size_t q = 0; q < b->grids.size(); ++q) {

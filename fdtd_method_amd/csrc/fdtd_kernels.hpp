@@ -142,6 +142,8 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, u
 // loop's first uses of those registers wait on the loop's own stores.
 __device__ __forceinline__ void wait_vmem_all() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+
+
 template <typename S, int V>
 __device__ __forceinline__ RawPack<S, V> buf_load_raw(__amdgpu_buffer_rsrc_t r, unsigned off) {
     using Words = typename RawPack<S, V>::Words;
@@ -1443,8 +1445,16 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
     // wavefront whose last row lies below / first row above this one's
     // (the end wavefronts read their own: those rows are ghost rows)
     const int wm = w > 0 ? w - 1 : w, wp = w + 1 < W ? w + 1 : w;
-    auto sB = [&](int l, int c, int ww) { return lds + ((l * 2 + c) * W + ww) * RW + cx0; };
-    auto sE = [&](int l, int c, int ww) { return lds + (((L + l) * 2 + c) * W + ww) * RW + cx0; };
+    // LDS rows grouped by wavefront, so that the level and component parts
+    // of an address are compile-time offsets from one of three bases (this,
+    // the previous and the next wavefront's block)
+    using lds_c = __attribute__((address_space(3))) C*;
+    auto sB = [&](int l, int c, int ww) {
+        return (lds_c)lds + ww * (4 * L * RW) + (l * 2 + c) * RW + cx0;
+    };
+    auto sE = [&](int l, int c, int ww) {
+        return (lds_c)lds + ww * (4 * L * RW) + ((L + l) * 2 + c) * RW + cx0;
+    };
     const Coefs<C> c = a.c;
     const cptr<int> kst = as_const(a.kstart);
 
@@ -1508,10 +1518,22 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
             sB(0, 0, w)[v] = bn[0][R - 1][v];
             sB(0, 1, w)[v] = bn[2][R - 1][v];
         }
-        wait_vmem_all();
     }
+    // the first plane's loads have completed, so that the loop's waits are
+    // for its own prefetches (and, in the global path, not for its stores)
+    wait_vmem_all();
 
     for (int kk = kfirst; kk <= klast; ++kk) {
+        // lane offsets of this iteration's global accesses, redefined each
+        // iteration (an empty asm) so that the compiler forms every address
+        // as a scalar plane base plus a 32-bit lane offset instead of
+        // hoisting 64-bit per-lane sums out of the loop
+        unsigned lo[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            lo[r] = row_off[r];
+            asm volatile("" : "+v"(lo[r]));
+        }
         if constexpr (!kBuf) {
 #pragma unroll
             for (int v = 0; v < V; ++v) {
@@ -1532,8 +1554,8 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                     const gptr<const S> eb = uniform_gptr<const S>(a.in.e[d] + off);
 #pragma unroll
                     for (int r = 0; r < R; ++r) {
-                        rawB[d][r] = load_raw<S, V>(generic(at(bb, row_off[r])));
-                        rawE[d][r] = load_raw<S, V>(generic(at(eb, row_off[r])));
+                        rawB[d][r] = load_raw<S, V>(generic(at(bb, lo[r])));
+                        rawE[d][r] = load_raw<S, V>(generic(at(eb, lo[r])));
                     }
                 }
             }
@@ -1639,9 +1661,9 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                         buf_store_nt(static_cast<S>(bnew[2][r][v] + dbz), plane_rsrc(a.ahead[2] + carry_off, nb), st_off[r][v]);
                     } else if (!kBuf && carry_plane && row_out[r] && col_out[v]) {
                         const size_t off = carry_off + v;
-                        __builtin_nontemporal_store(static_cast<S>(bnew[0][r][v] + dbx), generic(at(uniform_gptr(a.ahead[0] + off), row_off[r])));
-                        __builtin_nontemporal_store(static_cast<S>(bnew[1][r][v] + dby), generic(at(uniform_gptr(a.ahead[1] + off), row_off[r])));
-                        __builtin_nontemporal_store(static_cast<S>(bnew[2][r][v] + dbz), generic(at(uniform_gptr(a.ahead[2] + off), row_off[r])));
+                        __builtin_nontemporal_store(static_cast<S>(bnew[0][r][v] + dbx), generic(at(uniform_gptr(a.ahead[0] + off), lo[r])));
+                        __builtin_nontemporal_store(static_cast<S>(bnew[1][r][v] + dby), generic(at(uniform_gptr(a.ahead[1] + off), lo[r])));
+                        __builtin_nontemporal_store(static_cast<S>(bnew[2][r][v] + dbz), generic(at(uniform_gptr(a.ahead[2] + off), lo[r])));
                     }
 #pragma unroll
                     for (int d = 0; d < 3; ++d) bnew[d][r][v] = as_stored<S>(bnew[d][r][v]);
@@ -1670,6 +1692,11 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
         // stores of the last level: E_{L-1} at plane kk-L+1, B_{L-1} at kk-L
         // (buffer path: size-0 resources for planes outside the chunk)
         if constexpr (!kBuf) {
+            // the prefetch of plane kk+1 has arrived before this plane's
+            // stores are issued: the wait at the end of the iteration then
+            // leaves the stores in flight (their count is not known to the
+            // compiler, so its own wait would include them)
+            wait_vmem_all();
             const int pe = kk - L + 1;
             if (pe >= k0 && pe < k1) {
                 const size_t off = static_cast<size_t>(pe) * plane;
@@ -1678,7 +1705,9 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                     if (row_out[r])
 #pragma unroll
                         for (int d = 0; d < 3; ++d) {
-                            S* dst = generic(at(uniform_gptr(a.out.e[d] + off), row_off[r]));
+                            unsigned so = lo[r];
+                            asm volatile("" : "+v"(so));
+                            S* dst = generic(at(uniform_gptr(a.out.e[d] + off), so));
 #pragma unroll
                             for (int v = 0; v < V; ++v)
                                 if (col_out[v])
@@ -1692,7 +1721,9 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                     if (row_out[r])
 #pragma unroll
                         for (int d = 0; d < 3; ++d) {
-                            S* dst = generic(at(uniform_gptr(a.out.b[d] + off), row_off[r]));
+                            unsigned so = lo[r];
+                            asm volatile("" : "+v"(so));
+                            S* dst = generic(at(uniform_gptr(a.out.b[d] + off), so));
 #pragma unroll
                             for (int v = 0; v < V; ++v)
                                 if (col_out[v])

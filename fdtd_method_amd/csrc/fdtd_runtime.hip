@@ -293,8 +293,7 @@ fdtd::StripGeom make_strip_geom(const fdtd_grid* g, const BlockTuning& t) {
     G.Nj = g->Nj;
     G.Nk = g->Nk;
     G.W = t.TYg / t.R;
-    G.Gx = (t.L + t.V - 1) / t.V * t.V;
-    G.TX = 64 * t.V - 2 * G.Gx;
+    strip_x_geometry(t.L, t.V, g->esize, &G.Gx, &G.TX);
     G.TY = t.TYg - 2 * t.L;
     G.kchunk = t.kchunk;
     G.ntx = (g->Ni + G.TX - 1) / G.TX;
@@ -318,8 +317,9 @@ bool strip_valid(const fdtd_grid* g, const BlockTuning& t) {
     if ((unsigned long long)g->Ni * g->Nj * g->esize >= (1ull << 31)) return false;  // 32-bit plane offsets, buffer range checks
     const int W = t.TYg / t.R;
     if (W < 1 || W > 16 || 64 * W > fdtd::strip_max_threads(t.V, t.R, t.L, cs)) return false;
-    const int Gx = (t.L + t.V - 1) / t.V * t.V;
-    if (64 * t.V - 2 * Gx < 1 || t.TYg - 2 * t.L < 1) return false;
+    int Gx = 0, TX = 0;
+    strip_x_geometry(t.L, t.V, g->esize, &Gx, &TX);
+    if (TX < 1 || t.TYg - 2 * t.L < 1) return false;
     return fdtd::strip_lds_bytes(t.L, W, t.V, cs) <= kLdsLimit;
 }
 
