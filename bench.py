@@ -74,9 +74,8 @@ C = 3e10
 PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s + %HBM roofline, FP32/FP16 vs FP64 GPU at 512³, L2 err vs FP64"
 WORD = {Precision.FP64: 8, Precision.FP32: 4, Precision.FP16: 2}
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_sweep_fp32_512_traffic.json")
-STRIP_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "strip_sweep_fp32_512_traffic.json")
 PML_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fused_pml_sweep_fp32_512_pml0.2_traffic.json")
+PROFILE_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 
 def log(*a):
@@ -93,6 +92,28 @@ def committed_traffic(path, **match):
     for rec in t.get("configs", [t]):
         if all(rec.get(k) == v for k, v in match.items()):
             return rec.get("hbm_bytes_per_launch")
+    return None
+
+
+def committed_profile(prec, n, blocking, tuning):
+    """The committed counter record (profiles/hbm_traffic.json) of the steady
+    kernel of a 512^3-workload path: matched by precision, grid, and the
+    blocking (L, V, R, W, kchunk and stored tile width) or the fused launch
+    shape (V, TY, kchunk) the run used; None when no counter run of that
+    exact configuration is committed."""
+    if not os.path.exists(PROFILE_FILE):
+        return None
+    want = {"precision": prec.name.lower(), "n": n}
+    if blocking:
+        want["kernel"] = "strip" if blocking.get("R", 0) > 0 else "patch"
+        want["blocking"] = [blocking["L"], blocking["V"], blocking.get("R", 0),
+                            blocking.get("W", 0), blocking["kchunk"]]
+    else:
+        want["kernel"] = "fused"
+        want["tuning"] = list(tuning)
+    for rec in json.load(open(PROFILE_FILE))["configs"]:
+        if all(rec.get(k) == v for k, v in want.items()):
+            return dict(rec, file=os.path.relpath(PROFILE_FILE, ROOT))
     return None
 
 
@@ -114,22 +135,47 @@ def make_grid(n, precision, autotune=True):
     return g
 
 
+def reset(g):
+    """Zero fields, step 0; FP16 scales sized for one step's injection."""
+    g.zero_fields()
+    if g.precision == Precision.FP16:
+        g.set_scales([1.0] * 6)
+        g.autoscale(target_log2=10, magnitude_floor=4.0 * np.pi * g.dt * 1e-2)
+
+
 def run_path(g, steps, warmup, path=_lib.PATH_FUSED):
-    """Warm up, reset, then time `steps` steps: wall clock and HIP events."""
+    """From zero fields: `warmup` untimed steps, then `steps` timed steps that
+    continue the same run (so the timed call starts from the half step the
+    warm-up's last launch carried, as a driver's next call would).  Wall
+    clock around the timed call, and HIP events around each of its
+    launches on the grid's stream (fused path) or around the whole call
+    (three-sweep path)."""
+    reset(g)
     if warmup:
         g.time_update_fields(warmup, path)
-        g.zero_fields()
-        if g.precision == Precision.FP16:
-            g.set_scales([1.0] * 6)
-            g.autoscale(target_log2=10, magnitude_floor=4.0 * np.pi * g.dt * 1e-2)
     g.synchronize()
-    t0 = time.perf_counter()
-    ms_total, ms_main = g.time_update_fields(steps, path)
-    g.synchronize()
-    wall = time.perf_counter() - t0
     cells = float(g.cells)
-    return dict(ms_per_step=1e3 * wall / steps, gcells=cells * steps / wall / 1e9,
-                event_ms_total=ms_total, event_main_ms=ms_main, launch_ms=ms_main / steps)
+    t0 = time.perf_counter()
+    if path == _lib.PATH_FUSED:
+        kern = g.time_update_fields_kernels(steps)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        main = sum(v[0] for v in kern.values())
+        launches = sum(v[1] for v in kern.values())
+    else:
+        _, main = g.time_update_fields(steps, path)
+        g.synchronize()
+        wall = time.perf_counter() - t0
+        kern, launches = None, steps
+    rec = dict(ms_per_step=1e3 * wall / steps, gcells=cells * steps / wall / 1e9,
+               event_main_ms=main, launch_ms=main / max(1, launches), kernels=kern)
+    if kern:
+        # the steady launch: the kernel a long run is made of
+        k = kern["steady"] if kern["steady"][1] else kern["last"]
+        rec["steady_launch_ms"] = k[0] / k[1]
+        if kern["last"][1]:
+            rec["last_launch_ms"] = kern["last"][0] / kern["last"][1]
+    return rec
 
 
 def errors(test, ref):
@@ -153,11 +199,53 @@ def energy(g):
     return sum(s) / (8.0 * np.pi)
 
 
-def cpu_baseline(n, pml=None):
+# Host threads for the CPU baseline: one per physical core available to this
+# process, at most 16 -- the host share of one GPU on the machines this runs
+# on (a box of one GPU grants its process 16 hardware threads).
+CPU_SHARE = 16
+
+
+def host_cpu():
+    """Model, sockets, physical cores and hardware threads of the host
+    (/proc/cpuinfo), and how many of them this process may use."""
+    info = dict(model=None, sockets=None, physical_cores=None, hw_threads=os.cpu_count())
+    cores, sockets, model = {}, set(), None
     try:
-        threads = min(16, len(os.sched_getaffinity(0)))
+        proc = phys = core = None
+        for line in open("/proc/cpuinfo"):
+            key, _, val = line.partition(":")
+            key, val = key.strip(), val.strip()
+            if key == "processor":
+                proc = int(val)
+            elif key == "physical id":
+                phys = val
+            elif key == "core id":
+                core = val
+            elif key == "model name" and model is None:
+                model = val
+            elif not key and proc is not None:
+                cores[proc] = (phys, core)
+                sockets.add(phys)
+                proc = phys = core = None
+        if proc is not None:
+            cores[proc] = (phys, core)
+            sockets.add(phys)
+    except OSError:
+        pass
+    try:
+        allowed = sorted(os.sched_getaffinity(0))
     except AttributeError:
-        threads = min(16, os.cpu_count() or 1)
+        allowed = list(range(os.cpu_count() or 1))
+    info.update(model=model, sockets=len(sockets) or None,
+                physical_cores=len(set(cores.values())) or None,
+                allowed_hw_threads=len(allowed),
+                allowed_physical_cores=len({cores[p] for p in allowed if p in cores}) or None)
+    return info
+
+
+def cpu_baseline(n, pml=None):
+    host = host_cpu()
+    threads = min(CPU_SHARE, host.get("allowed_physical_cores") or host["allowed_hw_threads"])
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
                OMP_PLACES="cores")
     cmd = [sys.executable, "-m", "oracle.cpu_bench", "--n", str(n), "--seconds", "12"]
@@ -169,7 +257,11 @@ def cpu_baseline(n, pml=None):
         log(res.stderr[-2000:])
         return None
     rec = json.loads(res.stdout.strip().splitlines()[-1])
-    return {k: rec[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    out = {k: rec[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    out["host"] = host
+    out["threads_rule"] = (f"one OpenMP thread per physical core available, at most {CPU_SHARE} "
+                           "(the host share of one GPU), OMP_PROC_BIND=close, OMP_PLACES=cores")
+    return out
 
 
 def bench_batch(a):
@@ -542,8 +634,6 @@ def main():
             g.set_time_blocking_strips(*[int(x) for x in a.blocking.split(",")])
             blk = g.get_time_blocking_strips()
         r = run_path(g, K, W)
-        if blk:
-            r["pass_ms"] = r["event_main_ms"] / (K // blk["L"]) if K % blk["L"] == 0 else None
         nbytes = 1 << 30
         ms_copy = calibration_copy(nbytes, 5)
         print(json.dumps(dict(profile=a.profile, n=n, steps=K, tuning=g.get_tuning(),
@@ -586,42 +676,50 @@ def main():
     g64.close()
 
 
-    def fused_roofline(prec):
-        r = fused[prec]
-        word = WORD[prec]
-        gbs = 12 * word * cells / (r["launch_ms"] * 1e-3) / 1e9
+    def kernel_names(prec, cfg=None):
         sname = {Precision.FP32: "float", Precision.FP64: "double", Precision.FP16: "_Float16"}[prec]
         cname = "double" if prec == Precision.FP64 else "float"
-        return {"bound": "hbm", "achieved": gbs, "peak": PEAK_GBS, "unit": "GB/s",
-                "frac": gbs / PEAK_GBS,
-                "traffic": committed_traffic(TRAFFIC_FILE, n=n) if prec == Precision.FP32 else None,
-                "kernel": f"fdtd::fused_sweep<{sname},{cname},{sname},V,true>",
-                "alg_bytes_per_launch": 12.0 * word * cells, "launch_ms": r["launch_ms"]}
-
-    def blocked_roofline(prec):
-        """Temporally blocked passes: one launch advances L steps; the
-        per-step algorithmic figure (12 words per cell-update) is reported as
-        an effective bandwidth, which may exceed the HBM peak."""
-        cfg, r = blocking[prec], best[prec]
-        L = cfg["L"]
-        npass, rem = K // L, K % L
-        pass_ms = (r["event_main_ms"] - rem * fused[prec]["launch_ms"]) / npass
-        word = WORD[prec]
-        gbs = 12 * word * cells * L / (pass_ms * 1e-3) / 1e9
-        sname = {Precision.FP32: "float", Precision.FP64: "double", Precision.FP16: "_Float16"}[prec]
-        cname = "double" if prec == Precision.FP64 else "float"
+        if cfg is None:
+            return f"fdtd::fused_sweep<{sname},{cname},{sname},V,true>"
         if cfg.get("R", 0) > 0:
-            kern = f"fdtd::strip_sweep<{sname},{cname},{cfg['V']},{cfg['R']},{L},false>"
-        else:
-            kern = f"fdtd::blocked_sweep<{sname},{cname},{cfg['V']},{L},false>"
-        traffic = None
-        if prec == Precision.FP32:
-            traffic = committed_traffic(STRIP_TRAFFIC_FILE, n=n, blocking=[L, cfg["V"], cfg.get("R", 0),
-                                                                    cfg.get("W", 0)])
-        return {"bound": "hbm", "effective": True, "achieved": gbs, "peak": PEAK_GBS,
-                "unit": "GB/s", "frac": gbs / PEAK_GBS, "traffic": traffic, "kernel": kern,
-                "steps_per_launch": L, "alg_bytes_per_launch": 12.0 * word * cells * L,
-                "launch_ms": pass_ms}
+            return f"fdtd::strip_sweep<{sname},{cname},{cfg['V']},{cfg['R']},{cfg['L']},false>"
+        return f"fdtd::blocked_sweep<{sname},{cname},{cfg['V']},{cfg['L']},false>"
+
+    def roofline(prec, blocked):
+        """Roofline of the steady kernel of a path, from its launches inside
+        the timed region (HIP events on the grid's stream).  Algorithmic
+        bytes per launch: E and B read once and written once, 12 words per
+        cell (a temporally blocked pass advances L steps over that one
+        read and write).  `traffic`: HBM bytes per launch measured with
+        rocprofv3 counters for the same kernel and geometry (committed under
+        profiles/), `traffic_frac` the same over the launch time.  For
+        blocked passes `effective_frac` restates the launch as L unblocked
+        steps (12 words per cell-update each), a figure that can exceed 1."""
+        cfg = blocking[prec] if blocked else None
+        r = best[prec] if blocked else fused[prec]
+        word = WORD[prec]
+        alg = 12.0 * word * cells
+        ms = r["steady_launch_ms"]
+        gbs = alg / (ms * 1e-3) / 1e9
+        prof = committed_profile(prec, n, cfg, tunes[prec])
+        out = {"bound": "hbm", "achieved": gbs, "peak": PEAK_GBS, "unit": "GB/s",
+               "frac": gbs / PEAK_GBS,
+               "traffic": prof.get("hbm_bytes_per_launch") if prof else None,
+               "kernel": kernel_names(prec, cfg), "alg_bytes_per_launch": alg, "launch_ms": ms,
+               "launches_timed": (r["kernels"]["steady"][1] if r.get("kernels") else None)}
+        if prof:
+            out["traffic_frac"] = prof["hbm_bytes_per_launch"] / (ms * 1e-3) / 1e9 / PEAK_GBS
+            out["traffic_profile"] = prof.get("file")
+            if prof.get("valu_busy") is not None:
+                out["valu_busy"] = prof["valu_busy"]
+        if blocked:
+            L = cfg["L"]
+            out["steps_per_launch"] = L
+            out["effective_achieved"] = gbs * L
+            out["effective_frac"] = gbs * L / PEAK_GBS
+            if r.get("last_launch_ms"):
+                out["last_launch_ms"] = r["last_launch_ms"]
+        return out
 
     def path_rec(r, prec, tune=None, bytes_per_cell=None, cfg=None):
         bpc = bytes_per_cell or 12 * WORD[prec]
@@ -655,7 +753,7 @@ def main():
             "workload": (f"{n}^3 periodic vacuum Yee grid, dx=1, dt=0.5dx/(c*sqrt3), "
                          "Gaussian-modulated sinusoid Jz point source at the centre "
                          "(period 20dt, sigma 40dt, peak 120dt), zero initial fields, "
-                         f"{K} steps"),
+                         f"{W} untimed steps, then {K} timed steps continuing the run"),
             "grid": [n, n, n],
             "path": ("fp32 storage and compute, temporally blocked passes of "
                      f"{cfg32['L']} fused steps (autotuned; bit-identical to single sweeps)"
@@ -664,8 +762,12 @@ def main():
             "fused_launch_shape": dict(V=tunes[Precision.FP32][0], TY=tunes[Precision.FP32][1],
                                        kchunk=tunes[Precision.FP32][2]),
         },
-        "roofline": blocked_roofline(Precision.FP32) if blocked32 else fused_roofline(Precision.FP32),
-        "fused_sweep_roofline": fused_roofline(Precision.FP32),
+        "roofline": roofline(Precision.FP32, blocked32),
+        "fused_sweep_roofline": roofline(Precision.FP32, False),
+        "rooflines": {"fp64": roofline(Precision.FP64, blocking[Precision.FP64]["L"] > 0),
+                      "fp16": roofline(Precision.FP16, blocking[Precision.FP16]["L"] > 0),
+                      "fp64_fused": roofline(Precision.FP64, False),
+                      "fp16_fused": roofline(Precision.FP16, False)},
         "cpu_baseline": cpu,
         "paths": {
             "fp32": path_rec(best[Precision.FP32], Precision.FP32, cfg=blocking[Precision.FP32]),

@@ -34,7 +34,8 @@ EXPORTED = [
     "fdtd_zero_fields", "fdtd_set_step_carry", "fdtd_set_time_blocking", "fdtd_get_time_blocking",
     "fdtd_autotune_blocking", "fdtd_set_time_blocking_strips", "fdtd_get_time_blocking_strips",
     "fdtd_set_tuning",
-    "fdtd_get_tuning", "fdtd_autotune", "fdtd_time_update_fields", "fdtd_calibration_copy",
+    "fdtd_get_tuning", "fdtd_autotune", "fdtd_time_update_fields", "fdtd_time_update_fields_kernels",
+    "fdtd_calibration_copy",
     "fdtd_batch_create", "fdtd_batch_destroy", "fdtd_batch_update_fields",
     "fdtd_batch_update_fields_graph", "fdtd_batch_synchronize", "fdtd_batch_time",
     "fdtd_batch_set_shape", "fdtd_batch_get_shape", "fdtd_batch_autotune",
@@ -127,6 +128,7 @@ def load():
         "fdtd_autotune": ([_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)], _i),
         "fdtd_time_update_fields": ([_vp, _i, _i, ctypes.POINTER(ctypes.c_float),
                                      ctypes.POINTER(ctypes.c_float)], _i),
+        "fdtd_time_update_fields_kernels": ([_vp, _i, _dp, ctypes.POINTER(_i)], _i),
         "fdtd_calibration_copy": ([ctypes.c_size_t, _i, ctypes.POINTER(ctypes.c_float)], _i),
         "fdtd_batch_create": ([_i, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _i),
         "fdtd_batch_destroy": ([_vp], _i),

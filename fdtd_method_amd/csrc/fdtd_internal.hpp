@@ -77,6 +77,10 @@ inline size_t current_size(int precision) { return precision == FDTD_FP64 ? 8 : 
 
 }  // namespace fdtd::rt
 
+namespace fdtd::rt {
+struct LaunchTrace;
+}
+
 struct fdtd_grid {
     fdtd_parameters params{};
     int Ni = 0, Nj = 0, Nk = 0;
@@ -138,6 +142,9 @@ struct fdtd_grid {
     void* ES[2][6] = {};        // E splits Exy Exz Eyx Eyz Ezx Ezy, ping-pong with E
     void* BS[6] = {};           // B splits Bxy Bxz Byx Byz Bzx Bzy, in place
     void* pml_tab = nullptr;    // per-axis decay/coefficient tables (compute type)
+    // set by fdtd_time_update_fields_kernels: the stepping launches are
+    // bracketed by events on the grid's stream
+    fdtd::rt::LaunchTrace* trace = nullptr;
 };
 
 namespace fdtd::rt {

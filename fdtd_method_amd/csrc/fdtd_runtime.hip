@@ -820,6 +820,46 @@ int launch_blocked(fdtd_grid* g, bool final, int slot, int in, int out,
     }
 }
 
+// Per-launch timing (fdtd_time_update_fields_kernels): events before and
+// after each stepping launch, with its kind (0 B/2 prologue, 1 steady pass
+// or sweep, 2 a call's last pass or sweep, 3 the single sweeps that follow
+// the passes of a blocked call).
+struct LaunchTrace {
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> pool;
+    std::vector<Rec> recs;
+    size_t next = 0;
+    hipEvent_t take() {
+        if (next == pool.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[next++];
+    }
+    ~LaunchTrace() {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+hipEvent_t trace_begin(fdtd_grid* g) {
+    if (!g->trace) return nullptr;
+    hipEvent_t e = g->trace->take();
+    if (e) (void)hipEventRecord(e, g->stream);
+    return e;
+}
+
+void trace_end(fdtd_grid* g, hipEvent_t a, int kind) {
+    if (!g->trace || !a) return;
+    hipEvent_t e = g->trace->take();
+    if (!e) return;
+    (void)hipEventRecord(e, g->stream);
+    g->trace->recs.push_back({kind, a, e});
+}
+
 // n steps with temporal blocking: B/2 (or, for strip passes, the carried
 // half step), then n / L blocked passes, then the remaining n % L steps as
 // single fused sweeps.  Source tables are filled in whole passes.  Strip
@@ -830,7 +870,11 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
     const int carried = strips ? g->ba_cur : -1;
     g->ba_cur = -1;
     int rc = FDTD_OK;
-    if (carried < 0) rc = plain_B(g);
+    if (carried < 0) {
+        hipEvent_t t0 = trace_begin(g);
+        rc = plain_B(g);
+        trace_end(g, t0, 0);
+    }
     if (rc) return rc;
     const bool carry_out = strips && g->carry && ensure_carry(g);
     const int ahead_buf = carried < 0 ? 0 : 1 - carried;
@@ -863,7 +907,9 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
         const bool last = rem == 0 && p + 1 == npass;
         void* const* in_b = p == 0 && carried >= 0 ? g->BA[carried] : nullptr;
         void* const* ahead = last && carry_out ? g->BA[ahead_buf] : nullptr;
+        hipEvent_t t0 = trace_begin(g);
         rc = launch_blocked(g, last, s % chunk, in, out, in_b, ahead);
+        trace_end(g, t0, last ? 2 : 1);
         g->pass_rescale = false;
         for (int c = 0; c < 6; ++c) g->pass_iscale[c] = 1.0f;
         if (rc) return rc;
@@ -874,7 +920,9 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
         const int in = g->cur, out = 1 - g->cur;
         SweepExtra x;
         if (q + 1 == rem && carry_out) x.ahead = g->BA[ahead_buf];
+        hipEvent_t t0 = trace_begin(g);
         rc = launch_fused(g, g->tune, q + 1 < rem, 0, in, out, x);
+        trace_end(g, t0, q + 1 < rem ? 3 : 2);
         if (rc) return rc;
         g->cur = out;
     }
@@ -903,7 +951,11 @@ int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
     // E_n); otherwise B/2 in place.
     const int carried = g->ba_cur;
     int rc = FDTD_OK;
-    if (carried < 0) rc = plain_B(g);
+    if (carried < 0) {
+        hipEvent_t t0 = trace_begin(g);
+        rc = plain_B(g);
+        trace_end(g, t0, 0);
+    }
     if (rc) return rc;
     const bool carry_out = g->carry && !g->pml && ensure_carry(g);
     const int ahead_buf = carried < 0 ? 0 : 1 - carried;
@@ -937,7 +989,9 @@ int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
         x.maxbits = track ? g->d_max : nullptr;
         if (s == 0 && carried >= 0) x.in_b = g->BA[carried];
         if (s + 1 == nsteps && carry_out) x.ahead = g->BA[ahead_buf];
+        hipEvent_t t0 = trace_begin(g);
         rc = launch_fused(g, g->tune, s + 1 < nsteps, s % kValTable, in, out, x);
+        trace_end(g, t0, s + 1 < nsteps ? 1 : 2);
         if (rc) return rc;
         g->cur = out;
         if (s + 1 == nsteps && carry_out) g->ba_cur = ahead_buf;
@@ -2008,6 +2062,27 @@ int fdtd_time_update_fields(fdtd_grid* g, int nsteps, int path, float* ms_total,
     HIP_TRY(hipEventElapsedTime(&b, g->ev[1], g->ev[2]));
     if (ms_total) *ms_total = a;
     if (ms_main) *ms_main = b;
+    return FDTD_OK;
+}
+
+int fdtd_time_update_fields_kernels(fdtd_grid* g, int nsteps, double ms[4], int count[4]) {
+    if (!g || nsteps <= 0 || !ms || !count) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    LaunchTrace trace;
+    g->trace = &trace;
+    int rc = fdtd_update_fields(g, nsteps);
+    g->trace = nullptr;
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (rc) return rc;
+    for (int k = 0; k < 4; ++k) {
+        ms[k] = 0.0;
+        count[k] = 0;
+    }
+    for (const auto& r : trace.recs) {
+        float t = 0;
+        HIP_TRY(hipEventElapsedTime(&t, r.a, r.b));
+        ms[r.kind] += t;
+        count[r.kind] += 1;
+    }
     return FDTD_OK;
 }
 

@@ -346,6 +346,19 @@ class FDTD:
         _lib.check(self._L.fdtd_autotune(self._h, int(reps), out, ctypes.byref(ms)))
         return tuple(out), ms.value
 
+    def time_update_fields_kernels(self, nsteps: int):
+        """update_fields(nsteps) timed launch by launch: {kind: (ms, launches)} for
+        'prologue' (B/2 without a carried step), 'steady' (passes or sweeps),
+        'last' (the call's last pass or sweep) and 'remainder' (single sweeps
+        after a blocked call's passes)."""
+        self._flush()
+        ms = (ctypes.c_double * 4)()
+        cnt = (ctypes.c_int * 4)()
+        _lib.check(self._L.fdtd_time_update_fields_kernels(self._h, int(nsteps), ms, cnt))
+        self._invalidate()
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(("prologue", "steady", "last",
+                                                          "remainder"))}
+
     def time_update_fields(self, nsteps: int, path: int = _lib.PATH_FUSED):
         """(ms_total, ms_main) of nsteps steps, HIP events on the grid's stream."""
         self._flush()

@@ -251,6 +251,14 @@ int fdtd_get_time_blocking_strips(const fdtd_grid* grid, int out[5]);
  * call (including the B half step of the fused path), ms_main only the
  * nsteps sweep launches (fused) or step triples (3-sweep). */
 int fdtd_time_update_fields(fdtd_grid* grid, int nsteps, int path, float* ms_total, float* ms_main);
+/* fdtd_update_fields(grid, nsteps) timed launch by launch (HIP events before
+ * and after each stepping kernel on the grid's stream): ms[k] and count[k]
+ * are the total duration and number of launches of kind k -- 0 the B half
+ * step that starts a call without a carried step, 1 steady passes or sweeps,
+ * 2 the call's last pass or sweep (which also stores the carried half step),
+ * 3 the single sweeps that follow the passes of a blocked call.  No
+ * counterpart in the reference. */
+int fdtd_time_update_fields_kernels(fdtd_grid* grid, int nsteps, double ms[4], int count[4]);
 
 /* Copy `bytes` (multiple of 16) device-to-device `iters` times with a
  * streaming kernel; ms = average per copy.  Calibrates byte counters. */
