@@ -120,6 +120,7 @@ def committed_profile(prec, n, blocking, tuning):
 def make_grid(n, precision, autotune=True):
     dt = 0.5 / (C * np.sqrt(3.0))
     g = FDTD(Parameters(n, n, n), dt, precision)
+    g.set_time_blocking_strips(0)  # single sweeps until the blocking autotuner runs
     src = dict(i=n // 2, j=n // 2, k=n // 2, comp=2, kind=1, amp=1.0, period=20 * dt,
                t0=120 * dt, sigma=40 * dt, phase=0.0)
     g.set_point_sources([src])
@@ -421,6 +422,7 @@ def bench_blocked(a):
 
     def make(prec):
         g = FDTD(Parameters(Ni, Nj, Nk), dt, prec)
+        g.set_time_blocking_strips(0)  # the full-row sweep unless set below
         g.set_point_sources(srcs)
         return g
 

@@ -26,6 +26,7 @@ EXPORTED = [
     "fdtd_pml_download_split", "fdtd_pml_profile", "fdtd_upload", "fdtd_download",
     "fdtd_download_plane",
     "fdtd_zero_currents", "fdtd_set_current_values", "fdtd_set_point_sources",
+    "fdtd_update_fields_with_currents", "fdtd_get_stats",
     "fdtd_update_fields", "fdtd_update_fields_3sweep", "fdtd_get_step", "fdtd_set_step",
     "fdtd_synchronize", "fdtd_get_info", "fdtd_sum_squares", "fdtd_sum_squares_noalloc",
     "fdtd_max_abs", "fdtd_compare",
@@ -98,6 +99,10 @@ def load():
         "fdtd_set_current_values": ([_vp, _i, ctypes.POINTER(ctypes.c_longlong),
                                      ctypes.POINTER(_i), _dp], _i),
         "fdtd_set_point_sources": ([_vp, _i, ctypes.POINTER(PointSource)], _i),
+        "fdtd_update_fields_with_currents": ([_vp, _i, ctypes.POINTER(_i),
+                                              ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_i),
+                                              _dp], _i),
+        "fdtd_get_stats": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _i),
         "fdtd_update_fields": ([_vp, _i], _i),
         "fdtd_update_fields_3sweep": ([_vp, _i], _i),
         "fdtd_get_step": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _i),

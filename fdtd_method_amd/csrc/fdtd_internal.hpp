@@ -99,6 +99,16 @@ struct fdtd_grid {
     int device = 0;  // the HIP device current when the grid was created
     long long step = 0;
     std::map<long long, double> sparse_j;  // key = index*3 + comp
+    // Per-step current values of one fdtd_update_fields_with_currents call:
+    // the sparse J cells of the call (keys as sparse_j) become kind-2
+    // sources reading column c of row (step - jtab_step0) of d_jtab.
+    std::vector<long long> jtab_keys;
+    double* d_jtab = nullptr;
+    size_t d_jtab_cap = 0;       // doubles
+    long long jtab_step0 = 0;
+    int jtab_rows = 0;
+    // launches so far: {steps, blocked passes, single-step sweeps, B/2 prologues}
+    long long stats[4] = {0, 0, 0, 0};
     std::vector<fdtd::PointSource> waveforms;
     fdtd::PointSource* d_src = nullptr;  // the list the next launches read (one of d_src_buf)
     double* d_vals = nullptr;  // source values, kValTable steps x nsrc

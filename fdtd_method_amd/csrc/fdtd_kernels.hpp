@@ -54,9 +54,11 @@ template <typename C> struct Coefs {
 //   kind 0: amp
 //   kind 1: amp * sin(2*pi*(t - t0)/period + phase) * exp(-(t - t0)^2/(2 sigma^2))
 //           (envelope omitted when sigma <= 0)
+//   kind 2: column `col` of a table of values given per step by the host
+//           (the values a caller set before each step of a call)
 struct PointSource {
     int i, j, k, comp;
-    int kind, pad0;
+    int kind, col;
     double amp, period, t0, sigma, phase;
 };
 
@@ -232,10 +234,15 @@ plain_update_B(Fields<S> f, Coefs<C> c, int Ni, int Nj, int Nk) {
 static __global__ void __launch_bounds__(256)
 eval_sources(const PointSource* __restrict__ src, int nsrc, int nsteps,
              const long long* __restrict__ step_dev, long long step, double dt,
-             double* __restrict__ vals) {
+             double* __restrict__ vals, const double* __restrict__ tab = nullptr,
+             int tab_stride = 0) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsrc * nsteps) return;
     const int s = t / nsrc, q = t - s * nsrc;
+    if (src[q].kind == 2) {
+        vals[t] = tab ? tab[static_cast<size_t>(s) * tab_stride + src[q].col] : 0.0;
+        return;
+    }
     const long long n = (step_dev ? *step_dev : step) + s;
     vals[t] = source_value(src[q], static_cast<double>(n + 1) * dt);
 }

@@ -165,14 +165,31 @@ fdtd::SweepGeom make_geom(const fdtd_grid* g, const Tuning& t) {
 
 std::vector<fdtd::PointSource> host_sources(const fdtd_grid* g) {
     std::vector<fdtd::PointSource> all;
-    all.reserve(g->sparse_j.size() + g->waveforms.size());
-    for (const auto& kv : g->sparse_j) {
-        const long long idx = kv.first / 3;
+    auto cell_source = [g](long long key) {
+        const long long idx = key / 3;
         fdtd::PointSource s{};
-        s.comp = (int)(kv.first % 3);
+        s.comp = (int)(key % 3);
         s.i = (int)(idx % g->Ni);
         s.j = (int)((idx / g->Ni) % g->Nj);
         s.k = (int)(idx / ((long long)g->Ni * g->Nj));
+        return s;
+    };
+    if (!g->jtab_keys.empty()) {
+        // a call with per-step values: every sparse cell of the call, in key
+        // order (the order sparse_j gives), reads its column of the table
+        all.reserve(g->jtab_keys.size() + g->waveforms.size());
+        for (size_t c = 0; c < g->jtab_keys.size(); ++c) {
+            fdtd::PointSource s = cell_source(g->jtab_keys[c]);
+            s.kind = 2;
+            s.col = (int)c;
+            all.push_back(s);
+        }
+        all.insert(all.end(), g->waveforms.begin(), g->waveforms.end());
+        return all;
+    }
+    all.reserve(g->sparse_j.size() + g->waveforms.size());
+    for (const auto& kv : g->sparse_j) {
+        fdtd::PointSource s = cell_source(kv.first);
         s.kind = 0;
         s.amp = kv.second;
         all.push_back(s);
@@ -248,9 +265,16 @@ int flush_sources(fdtd_grid* g) {
 int eval_sources(fdtd_grid* g, long long step, int nsteps) {
     if (g->nsrc == 0) return FDTD_OK;
     const int n = g->nsrc * nsteps;
+    const double* tab = nullptr;
+    if (g->jtab_rows > 0) {
+        const long long row = step - g->jtab_step0;
+        if (row < 0 || row + nsteps > g->jtab_rows)
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "steps outside the per-step current table");
+        tab = g->d_jtab + row * (long long)g->jtab_keys.size();
+    }
     hipLaunchKernelGGL(fdtd::eval_sources, dim3((n + 255) / 256), dim3(256), 0, g->stream,
                        g->d_src, g->nsrc, nsteps, (const long long*)nullptr, step, g->dt,
-                       g->d_vals);
+                       g->d_vals, tab, (int)g->jtab_keys.size());
     HIP_TRY(hipGetLastError());
     return FDTD_OK;
 }
@@ -852,7 +876,9 @@ hipEvent_t trace_begin(fdtd_grid* g) {
     return e;
 }
 
-void trace_end(fdtd_grid* g, hipEvent_t a, int kind) {
+// Launch counters (fdtd_get_stats) and the per-launch timing.
+void trace_end(fdtd_grid* g, hipEvent_t a, int kind, bool pass) {
+    g->stats[kind == 0 ? 3 : (pass ? 1 : 2)] += 1;
     if (!g->trace || !a) return;
     hipEvent_t e = g->trace->take();
     if (!e) return;
@@ -873,7 +899,7 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
     if (carried < 0) {
         hipEvent_t t0 = trace_begin(g);
         rc = plain_B(g);
-        trace_end(g, t0, 0);
+        trace_end(g, t0, 0, false);
     }
     if (rc) return rc;
     const bool carry_out = strips && g->carry && ensure_carry(g);
@@ -909,7 +935,7 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
         void* const* ahead = last && carry_out ? g->BA[ahead_buf] : nullptr;
         hipEvent_t t0 = trace_begin(g);
         rc = launch_blocked(g, last, s % chunk, in, out, in_b, ahead);
-        trace_end(g, t0, last ? 2 : 1);
+        trace_end(g, t0, last ? 2 : 1, true);
         g->pass_rescale = false;
         for (int c = 0; c < 6; ++c) g->pass_iscale[c] = 1.0f;
         if (rc) return rc;
@@ -922,7 +948,7 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
         if (q + 1 == rem && carry_out) x.ahead = g->BA[ahead_buf];
         hipEvent_t t0 = trace_begin(g);
         rc = launch_fused(g, g->tune, q + 1 < rem, 0, in, out, x);
-        trace_end(g, t0, q + 1 < rem ? 3 : 2);
+        trace_end(g, t0, q + 1 < rem ? 3 : 2, false);
         if (rc) return rc;
         g->cur = out;
     }
@@ -954,7 +980,7 @@ int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
     if (carried < 0) {
         hipEvent_t t0 = trace_begin(g);
         rc = plain_B(g);
-        trace_end(g, t0, 0);
+        trace_end(g, t0, 0, false);
     }
     if (rc) return rc;
     const bool carry_out = g->carry && !g->pml && ensure_carry(g);
@@ -991,7 +1017,7 @@ int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
         if (s + 1 == nsteps && carry_out) x.ahead = g->BA[ahead_buf];
         hipEvent_t t0 = trace_begin(g);
         rc = launch_fused(g, g->tune, s + 1 < nsteps, s % kValTable, in, out, x);
-        trace_end(g, t0, s + 1 < nsteps ? 1 : 2);
+        trace_end(g, t0, s + 1 < nsteps ? 1 : 2, false);
         if (rc) return rc;
         g->cur = out;
         if (s + 1 == nsteps && carry_out) g->ba_cur = ahead_buf;
@@ -1238,6 +1264,24 @@ int create_grid(const fdtd_parameters* p, double dt, double pml_percent, int pre
         if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
             return cleanup(fail(FDTD_ERR_HIP, "hipEventCreate"));
     g->fused_ok = default_tuning(g, &g->tune);
+    // Periodic grids large enough to fill the device with strip tiles step
+    // in temporally blocked passes of 4 steps by default (bit-identical to
+    // single sweeps; FP16 with in-run rescaling to FP16 rounding), with the
+    // shape the autotuner picks at 512^3; fdtd_set_time_blocking_strips(grid,
+    // 0, ...) returns to single sweeps.
+    if (!g->pml && g->fused_ok && g->Ni >= 128 && g->Nj >= 64 && g->Nk >= 32) {
+        BlockTuning t;
+        t.L = 4;
+        t.V = 1;
+        t.R = 2;
+        t.lanes = 64;
+        t.TYg = (precision == FDTD_FP64 ? 8 : 16) * t.R;
+        t.kchunk = 1;
+        if (strip_valid(g, t)) {
+            t.kchunk = strip_default_kchunk(g, t);
+            g->btune = t;
+        }
+    }
     if (hipStreamSynchronize(g->stream) != hipSuccess)
         return cleanup(fail(FDTD_ERR_HIP, "initialisation failed"));
     *out = g;
@@ -1282,6 +1326,7 @@ int fdtd_destroy(fdtd_grid* g) {
     for (int d = 0; d < 3; ++d)
         if (g->J[d]) (void)hipFree(g->J[d]);
     if (g->staging) (void)hipFree(g->staging);
+    if (g->d_jtab) (void)hipFree(g->d_jtab);
     for (int b = 0; b < 2; ++b) {
         if (g->d_src_buf[b]) (void)hipFree(g->d_src_buf[b]);
         if (g->h_src[b]) (void)hipHostFree(g->h_src[b]);
@@ -1554,8 +1599,90 @@ int fdtd_update_fields(fdtd_grid* g, int nsteps) {
     if (!g || nsteps < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
     if (nsteps == 0) return FDTD_OK;
     if (int rc = flush_sources(g)) return rc;
+    g->stats[0] += nsteps;
     if (!g->fused_ok) return run_3sweep(g, nsteps);
     return run_fused(g, nsteps);
+}
+
+int fdtd_update_fields_with_currents(fdtd_grid* g, int nsteps, const int* counts,
+                                     const long long* index, const int* comp, const double* value) {
+    if (!g || nsteps < 0 || (nsteps > 0 && !counts))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    long long total = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        if (counts[s] < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "negative count at step %d", s);
+        total += counts[s];
+    }
+    if (total == 0) return fdtd_update_fields(g, nsteps);
+    if (!index || !comp || !value) return fail(FDTD_ERR_INVALID_ARGUMENT, "null current values");
+    for (long long q = 0; q < total; ++q) {
+        if (comp[q] < FDTD_JX || comp[q] > FDTD_JZ)
+            return fail(FDTD_ERR_INVALID_COMPONENT, "ERROR: Invalid field component");
+        if (index[q] < 0 || (size_t)index[q] >= g->N)
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "index %lld out of range", index[q]);
+    }
+    // The cells the call touches (the current sparse cells and every cell
+    // written), and each step's values: a write persists until the next one.
+    std::map<long long, double> cur = g->sparse_j;
+    for (long long q = 0; q < total; ++q) cur.emplace(index[q] * 3 + (comp[q] - FDTD_JX), 0.0);
+    const bool tabulate = !g->j_dense_active && (int)cur.size() <= kMaxSparseCurrents;
+    if (!tabulate) {
+        // dense J (or too many cells for the per-plane source scan): the
+        // steps one at a time, each after its writes
+        long long q = 0;
+        for (int s = 0; s < nsteps; ++s) {
+            if (counts[s] > 0) {
+                if (int rc = fdtd_set_current_values(g, counts[s], index + q, comp + q, value + q))
+                    return rc;
+                q += counts[s];
+            }
+            if (int rc = fdtd_update_fields(g, 1)) return rc;
+        }
+        return FDTD_OK;
+    }
+    std::vector<long long> keys;
+    std::map<long long, int> col;
+    for (const auto& kv : cur) {
+        col[kv.first] = (int)keys.size();
+        keys.push_back(kv.first);
+    }
+    std::map<long long, double> now = g->sparse_j;  // values in effect
+    std::vector<double> tab((size_t)nsteps * keys.size(), 0.0);
+    long long q = 0;
+    for (int s = 0; s < nsteps; ++s) {
+        for (int e = 0; e < counts[s]; ++e, ++q) now[index[q] * 3 + (comp[q] - FDTD_JX)] = value[q];
+        double* row = tab.data() + (size_t)s * keys.size();
+        for (const auto& kv : now) row[col[kv.first]] = kv.second;
+    }
+    if (tab.size() > g->d_jtab_cap) {
+        HIP_TRY(hipStreamSynchronize(g->stream));
+        if (g->d_jtab) HIP_TRY(hipFree(g->d_jtab));
+        g->d_jtab = nullptr;
+        g->d_jtab_cap = 0;
+        HIP_TRY(hipMalloc(&g->d_jtab, sizeof(double) * tab.size()));
+        g->d_jtab_cap = tab.size();
+    }
+    // the table is read by this call's source evaluations, queued on the
+    // grid's stream: a synchronous copy keeps the host buffer's lifetime simple
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    HIP_TRY(hipMemcpy(g->d_jtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+    g->jtab_keys = keys;
+    g->jtab_step0 = g->step;
+    g->jtab_rows = nsteps;
+    g->src_dirty = true;
+    int rc = fdtd_update_fields(g, nsteps);
+    // the values in effect after the call, as single sparse values
+    g->jtab_keys.clear();
+    g->jtab_rows = 0;
+    g->sparse_j = now;
+    g->src_dirty = true;
+    return rc;
+}
+
+int fdtd_get_stats(const fdtd_grid* g, long long out[4]) {
+    if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    for (int k = 0; k < 4; ++k) out[k] = g->stats[k];
+    return FDTD_OK;
 }
 
 int fdtd_update_fields_3sweep(fdtd_grid* g, int nsteps) {

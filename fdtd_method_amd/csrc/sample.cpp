@@ -90,15 +90,11 @@ static double run_wave(FDTD& method, const Parameters& params, int it,
         if (after_step) after_step(t);
     }
     method.zeroed_currents();
-    if (after_step) {
-        for (int t = cur_time; t < it; ++t) {
-            method.update_fields();
-            after_step(t);
-        }
-    } else if (it > cur_time) {
-        method.update_fields(it - cur_time);
+    for (int t = cur_time; t < it; ++t) {
+        method.update_fields();
+        if (after_step) after_step(t);
     }
-    method.synchronize();
+    method.synchronize();  // runs the deferred steps
     auto end = std::chrono::high_resolution_clock::now();
     std::chrono::duration<double> elapsed = end - start;
     return elapsed.count();

@@ -210,6 +210,36 @@ class FDTD:
             self._h, idx.size, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
             cc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dptr(vv)))
 
+    def update_fields_with_currents(self, writes) -> None:
+        """len(writes) steps; before step s the sparse currents writes[s] =
+        (index, comp, value) (sequences of equal length, comp 6..8) are set as
+        set_current_values would: the reference drivers' loop of J writes and
+        update_fields() as one call (fdtd_update_fields_with_currents)."""
+        self._flush()
+        counts = np.array([len(w[0]) for w in writes], dtype=np.int32)
+        idx = np.concatenate([np.asarray(w[0], dtype=np.int64) for w in writes] or
+                             [np.zeros(0, np.int64)])
+        cc = np.concatenate([np.asarray(w[1], dtype=np.int32) for w in writes] or
+                            [np.zeros(0, np.int32)])
+        vv = np.concatenate([np.asarray(w[2], dtype=np.float64) for w in writes] or
+                            [np.zeros(0)])
+        idx, cc, vv = (np.ascontiguousarray(a) for a in (idx, cc, vv))
+        _lib.check(self._L.fdtd_update_fields_with_currents(
+            self._h, len(writes), counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            idx.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+            cc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dptr(vv)))
+        self._invalidate()
+        for c in (Component.JX, Component.JY, Component.JZ):
+            f = self._fields.get(c)
+            if f is not None and not f._dirty:
+                f._host = None
+
+    def launch_stats(self):
+        """{steps, passes, sweeps, prologues} launched since creation."""
+        out = (ctypes.c_longlong * 4)()
+        _lib.check(self._L.fdtd_get_stats(self._h, out))
+        return dict(steps=out[0], passes=out[1], sweeps=out[2], prologues=out[3])
+
     def set_point_sources(self, sources) -> None:
         """sources: iterable of dicts {i,j,k,comp(0..2),kind,amp,period,t0,sigma,phase}."""
         arr = (_lib.PointSource * max(1, len(sources)))()

@@ -13,8 +13,14 @@
 //    writes are uploaded before the next step, single-cell J writes are
 //    sent to the device as sparse currents (the drivers' box source,
 //    perf-tests/sample/sample.cpp:67-81).
+//  * update_fields() defers: steps accumulate, with the single-cell J
+//    writes made between them, and run as one library call when a field is
+//    read or written (other than a single J cell), at zeroed_currents(),
+//    synchronize(), update_fields(n), destruction, or after 256 steps.  A
+//    one-step loop therefore runs as temporally blocked passes, with the
+//    results of stepping one call at a time, bit for bit.
 //  * The constructor takes an optional storage precision (FP64 default).
-//  * update_fields(int n) advances n steps in one call (fused sweeps).
+//  * update_fields(int n) advances n steps in one call.
 //  * Only single-threaded use per object, like the reference.
 //
 // Header-only: it needs the C-ABI (fdtd_mi355x.h) and links against
@@ -159,7 +165,13 @@ public:
         : FDTD(parameters, dt, precision, -1.0) {}
     FDTD(const FDTD&) = delete;
     FDTD& operator=(const FDTD&) = delete;
-    virtual ~FDTD() { fdtd_destroy(grid_); }
+    virtual ~FDTD() {
+        try {
+            run_pending();
+        } catch (...) {
+        }
+        fdtd_destroy(grid_);
+    }
 
     // Field& get_field(Component) -- src/FDTD/FDTD.cpp:138-151.
     Field& get_field(Component c) {
@@ -168,10 +180,11 @@ public:
         return fields_[id];
     }
 
-    // virtual void update_fields() -- src/FDTD/FDTD.cpp:153-157.
-    virtual void update_fields() { update_fields(1); }
+    // virtual void update_fields() -- src/FDTD/FDTD.cpp:153-157: one step,
+    // deferred (see the header comment).
+    virtual void update_fields() { defer_step(); }
 
-    // n steps in one call.
+    // n steps in one call (after any deferred ones).
     void update_fields(int nsteps) {
         flush();
         check_status(fdtd_update_fields(grid_, nsteps));
@@ -180,8 +193,45 @@ public:
 
     // void zeroed_currents() -- src/FDTD/FDTD.cpp:132-136.
     void zeroed_currents() {
+        run_pending();
         for (int c = 6; c < 9; ++c) fields_[c].drop();
         check_status(fdtd_zero_currents(grid_));
+    }
+
+    // Runs the deferred steps now.
+    void run_pending() {
+        if (pending_ == 0) return;
+        const int steps = pending_;
+        pending_ = 0;
+        // groups 0..steps-1 precede the steps; the last group follows them
+        std::size_t before = 0;
+        for (int s = 0; s < steps; ++s) before += static_cast<std::size_t>(w_counts_[s]);
+        const int rc = fdtd_update_fields_with_currents(
+            grid_, steps, w_counts_.data(), w_index_.data(), w_comp_.data(), w_value_.data());
+        const int trailing = w_counts_[steps];
+        int rc2 = FDTD_OK;
+        if (rc == FDTD_OK && trailing > 0)
+            rc2 = fdtd_set_current_values(grid_, trailing, w_index_.data() + before,
+                                          w_comp_.data() + before, w_value_.data() + before);
+        w_counts_.clear();
+        w_index_.clear();
+        w_comp_.clear();
+        w_value_.clear();
+        for (int c = 0; c < 6; ++c) fields_[c].invalidate();
+        check_status(rc);
+        check_status(rc2);
+    }
+
+    // Deferred steps not yet run.
+    int pending_steps() const { return pending_; }
+
+    // {steps, temporally blocked passes, single-step sweeps, B/2 prologues}
+    // launched so far (after any deferred steps run).
+    std::array<long long, 4> launch_stats() {
+        run_pending();
+        std::array<long long, 4> s{};
+        check_status(fdtd_get_stats(grid_, s.data()));
+        return s;
     }
 
     // Temporal blocking (no counterpart in the reference): L steps per pass
@@ -204,7 +254,10 @@ public:
     const Parameters& parameters() const { return parameters_; }
     FP dt() const { return dt_; }
 
-    void synchronize() { check_status(fdtd_synchronize(grid_)); }
+    void synchronize() {
+        run_pending();
+        check_status(fdtd_synchronize(grid_));
+    }
 
     // FP64 copy of plane k of a component (Ni*Nj values, index i + j*Ni),
     // after pending host writes are uploaded.
@@ -235,9 +288,39 @@ protected:
 private:
     friend class Field;
 
+    static constexpr int kMaxPending = 256;
+
     void flush() {
+        run_pending();
         for (auto& f : fields_) f.push();
     }
+
+    void defer_step() {
+        for (int c = 0; c < 6; ++c) fields_[c].push();  // E/B writes precede the step
+        if (pending_ == 0) w_counts_.assign(1, 0);
+        ++pending_;
+        w_counts_.push_back(0);
+        if (pending_ >= kMaxPending) run_pending();
+    }
+
+    // A single J cell write: before the next deferred step when steps are
+    // pending, else at once.
+    void write_current(long long index, int comp, double v) {
+        if (pending_ == 0) {
+            check_status(fdtd_set_current_values(grid_, 1, &index, &comp, &v));
+            return;
+        }
+        w_index_.push_back(index);
+        w_comp_.push_back(comp);
+        w_value_.push_back(v);
+        w_counts_.back() += 1;
+    }
+
+    int pending_ = 0;
+    std::vector<int> w_counts_;  // writes before deferred step s; the last: after them
+    std::vector<long long> w_index_;
+    std::vector<int> w_comp_;
+    std::vector<double> w_value_;
 
     Parameters parameters_;
     FP dt_;
@@ -257,8 +340,9 @@ public:
         : FDTD(parameters, dt, precision, pml_percent) {}
 
     using FDTD::update_fields;
-    // void update_fields() override -- src/FDTD/FDTD_PML.cpp:343-365.
-    void update_fields() override { FDTD::update_fields(1); }
+    // void update_fields() override -- src/FDTD/FDTD_PML.cpp:343-365
+    // (deferred like FDTD::update_fields()).
+    void update_fields() override { FDTD::update_fields(); }
 
     // Layer widths {p_i, p_j, p_k} (FDTD_PML.cpp:254-256).
     std::array<int, 3> pml_widths() const {
@@ -299,6 +383,7 @@ inline void write_csv_slice(FDTD& m, Component c, int k, const std::string& path
 inline std::size_t Field::size() const { return owner_->cells(); }
 
 inline void Field::pull() {
+    owner_->run_pending();
     if (valid_) return;
     host_.resize(owner_->cells());
     check_status(fdtd_download(owner_->handle(), comp_, host_.data(), host_.size()));
@@ -319,9 +404,8 @@ inline double Field::get(std::size_t i) {
 inline void Field::set(std::size_t i, double v) {
     if (comp_ >= FDTD_JX && !dirty_) {
         if (i >= owner_->cells()) throw std::out_of_range("field index out of range");
-        const long long idx = static_cast<long long>(i);
-        check_status(fdtd_set_current_values(owner_->handle(), 1, &idx, &comp_, &v));
-        if (valid_) host_[i] = v;
+        owner_->write_current(static_cast<long long>(i), comp_, v);
+        valid_ = false;  // the mirror no longer reflects J (refreshed on read)
         return;
     }
     pull();

@@ -137,6 +137,23 @@ int fdtd_zero_currents(fdtd_grid* grid);
 int fdtd_set_current_values(fdtd_grid* grid, int n, const long long* index,
                             const int* component, const double* value);
 
+/* nsteps steps, each after a batch of sparse current writes: before step s
+ * the counts[s] next entries of (index, component, value) are set as by
+ * fdtd_set_current_values (they persist until changed or zeroed).  Equal,
+ * bit for bit, to that loop of fdtd_set_current_values and
+ * fdtd_update_fields(grid, 1) -- the reference drivers' loop
+ * (perf-tests/sample/sample.cpp:66-87) -- but the steps run as one call, so
+ * temporally blocked passes apply: the cells the call writes (at most 64)
+ * become sources whose per-step values come from a device table.  Dense J
+ * or more cells fall back to the step-by-step loop. */
+int fdtd_update_fields_with_currents(fdtd_grid* grid, int nsteps, const int* counts,
+                                     const long long* index, const int* component,
+                                     const double* value);
+
+/* Launch counters since creation: {steps, temporally blocked passes,
+ * single-step sweeps, B half-step prologues}. */
+int fdtd_get_stats(const fdtd_grid* grid, long long out[4]);
+
 /* Replace the list of waveform point sources (evaluated on the device). */
 int fdtd_set_point_sources(fdtd_grid* grid, int n, const fdtd_point_source* sources);
 

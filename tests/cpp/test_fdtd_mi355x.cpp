@@ -239,6 +239,49 @@ void test_time_blocking() {
     EXPECT_TRUE(same);
 }
 
+void test_deferred_steps() {
+    // The reference drivers' loop (perf-tests/sample/sample.cpp:66-87): J
+    // box writes, then update_fields(), one step at a time.  The class
+    // defers the steps and runs them as one call (strip passes of 4 steps on
+    // a grid this size); the fields equal stepping one call at a time.
+    Parameters p{128, 64, 32, 0, 1, 0, 1, 0, 1, 1.0, 1.0, 1.0};
+    const FP dt = 0.5 / (FDTD_const::C * std::sqrt(3.0));
+    FDTD a(p, dt, g_precision), b(p, dt, g_precision);
+    const int centre = 64 + 32 * 128 + 16 * 128 * 64;
+    for (FDTD* g : {&a, &b}) {
+        Field& ex = g->get_field(Component::EX);
+        for (int i = 0; i < (int)g->cells(); i += 97) ex[i] = 1e-3 * std::cos(0.3 * i);
+    }
+    for (int t = 0; t < 14; ++t) {
+        const double v = std::sin(2.0 * FDTD_const::PI * (t + 1) / 20.0);
+        for (FDTD* g : {&a, &b})
+            for (int d = 0; d < 2; ++d) {
+                g->get_field(Component::JX)[centre + d] = v;
+                g->get_field(Component::JY)[centre + d * 128] = 0.5 * v;
+                g->get_field(Component::JZ)[centre + d * 128 * 64] = -v;
+            }
+        a.update_fields();     // deferred
+        b.update_fields(1);    // one call per step
+    }
+    EXPECT_TRUE(a.pending_steps() == 14);
+    a.zeroed_currents();
+    b.zeroed_currents();
+    for (int t = 0; t < 6; ++t) {
+        a.update_fields();
+        b.update_fields(1);
+    }
+    const auto sa = a.launch_stats(), sb = b.launch_stats();
+    EXPECT_TRUE(sa[0] == 20 && sb[0] == 20);
+    EXPECT_TRUE(sa[1] == 14 / 4 + 6 / 4);  // strip passes
+    EXPECT_TRUE(sb[1] == 0);
+    bool same = true;
+    for (int c = 0; c < 6; ++c)
+        for (int i = 0; i < (int)a.cells(); ++i)
+            same = same && (double)a.get_field(static_cast<Component>(c))[i] ==
+                               (double)b.get_field(static_cast<Component>(c))[i];
+    EXPECT_TRUE(same);
+}
+
 void test_pml_class() {
     // FDTD_PML (src/FDTD/FDTD_PML.cpp:205-365): layer widths, J ignored in
     // the layer, a pulse absorbed, pml_percent validated.
@@ -301,7 +344,8 @@ int main(int argc, char** argv) {
     test_multi_step_equivalence();
     test_pml_class();
     test_time_blocking();
-    ran += 5;
+    test_deferred_steps();
+    ran += 6;
     std::printf("%d tests, %d failures\n", ran, g_failures);
     return g_failures == 0 ? 0 : 1;
 }

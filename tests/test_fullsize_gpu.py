@@ -60,8 +60,8 @@ def make(shape, src, prec=Precision.FP32, blocking=None, waves=False):
         # one step's injection, as the benchmark does, so that FP16 holds the
         # fields
         g.autoscale(target_log2=10, magnitude_floor=4.0 * np.pi * DT * 1e-2)
-    if blocking:
-        g.set_time_blocking_strips(*blocking)
+    # blocking None: single sweeps (grids this large default to strip passes)
+    g.set_time_blocking_strips(*(blocking or (0,)))
     return g
 
 
@@ -182,8 +182,7 @@ def test_plane_wave_stays_transversely_uniform_512(prec, blocking):
     n = 512
     g = FDTD(Parameters(n, n, n), DT, prec)
     g.add_plane_wave((3, 0, 0), 1.0, 0.3, (0.0, 1.0, 0.0))
-    if blocking:
-        g.set_time_blocking_strips(*blocking)
+    g.set_time_blocking_strips(*(blocking or (0,)))
     g.update_fields(37)  # strip passes and a remainder sweep
     for c in (Component.EY, Component.BZ):
         first = None

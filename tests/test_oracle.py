@@ -272,3 +272,20 @@ def test_point_source_runner_matches_dense_J():
     g2.update_fields(7)
     for c in range(6):
         assert np.array_equal(g1.fields()[c], g2.fields()[c])
+
+
+def test_sum_squares_against_numpy_and_exact_cases():
+    """oracle_sum_squares (the diagnostics checker) against numpy's float64
+    dot product, and exact on values whose squares sum without rounding."""
+    rng = np.random.default_rng(77)
+    for n in (1, 7, 1000, 123457):
+        a = rng.standard_normal(n) * 10.0 ** rng.integers(-3, 4)
+        got = O.lib().oracle_sum_squares(O._ptr(a), a.size)
+        assert abs(got - float(np.dot(a, a))) <= 1e-13 * float(np.dot(a, a))
+    ints = np.arange(-50, 51, dtype=np.float64)            # squares sum exactly
+    assert O.lib().oracle_sum_squares(O._ptr(ints), ints.size) == 2 * sum(k * k for k in range(51))
+    assert O.lib().oracle_sum_squares(O._ptr(np.zeros(5)), 5) == 0.0
+    g = O.OracleGrid(4, 3, 2, 1.0, 1.0, 1.0, 1e-12)
+    for c in range(6):
+        g.fields()[c][:] = c + 1.0
+    assert g.sum_squares() == [24.0 * (c + 1.0) ** 2 for c in range(6)]
