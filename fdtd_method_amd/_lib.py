@@ -27,6 +27,8 @@ EXPORTED = [
     "fdtd_download_plane",
     "fdtd_zero_currents", "fdtd_set_current_values", "fdtd_set_point_sources",
     "fdtd_update_fields_with_currents", "fdtd_get_stats",
+    "fdtd_set_probes", "fdtd_probe_rows", "fdtd_read_probes", "fdtd_set_energy_interval",
+    "fdtd_energy_rows", "fdtd_read_energy", "fdtd_rewind_records",
     "fdtd_update_fields", "fdtd_update_fields_3sweep", "fdtd_get_step", "fdtd_set_step",
     "fdtd_synchronize", "fdtd_get_info", "fdtd_sum_squares", "fdtd_sum_squares_noalloc",
     "fdtd_max_abs", "fdtd_compare",
@@ -103,6 +105,15 @@ def load():
                                               ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_i),
                                               _dp], _i),
         "fdtd_get_stats": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _i),
+        "fdtd_set_probes": ([_vp, _i, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_i),
+                             ctypes.c_longlong], _i),
+        "fdtd_probe_rows": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _i),
+        "fdtd_read_probes": ([_vp, ctypes.c_longlong, ctypes.c_longlong, _dp], _i),
+        "fdtd_set_energy_interval": ([_vp, _i, ctypes.c_longlong], _i),
+        "fdtd_energy_rows": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _i),
+        "fdtd_read_energy": ([_vp, ctypes.c_longlong, ctypes.c_longlong,
+                              ctypes.POINTER(ctypes.c_longlong), _dp], _i),
+        "fdtd_rewind_records": ([_vp], _i),
         "fdtd_update_fields": ([_vp, _i], _i),
         "fdtd_update_fields_3sweep": ([_vp, _i], _i),
         "fdtd_get_step": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _i),

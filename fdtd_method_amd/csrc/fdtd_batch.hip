@@ -693,6 +693,10 @@ int check_steps(fdtd_batch* b) {
     for (fdtd_grid* g : b->grids)
         if (!g->waveforms.empty() && g->dt != b->grids[0]->dt)
             return fail(FDTD_ERR_UNSUPPORTED, "grids with waveform sources must share one dt");
+    for (fdtd_grid* g : b->grids)
+        if (g->nprobe > 0 || g->energy_every > 0)
+            return fail(FDTD_ERR_UNSUPPORTED,
+                        "batches record no probes or energy: step such grids on their own");
     return FDTD_OK;
 }
 

@@ -155,6 +155,25 @@ struct fdtd_grid {
     // set by fdtd_time_update_fields_kernels: the stepping launches are
     // bracketed by events on the grid's stream
     fdtd::rt::LaunchTrace* trace = nullptr;
+    // Probes (fdtd_set_probes): FP64 values of single cells after every
+    // step, one row of nprobe values per step in d_probe_out (probe_cap
+    // rows).  Rows are written by the stepping kernels themselves (fused
+    // sweeps, strip passes) or by a gather after each step (three-sweep
+    // and PML steps); probe_rows counts the rows queued so far.
+    fdtd::Probe* d_probe = nullptr;  // sorted by plane key (E at k: k, B at k: k+1 mod Nk)
+    int* d_pstart = nullptr;         // Nk+1 ranges of d_probe
+    int nprobe = 0;
+    double* d_probe_out = nullptr;
+    long long probe_cap = 0, probe_rows = 0;
+    // Energy records (fdtd_set_energy_interval): after every step that is a
+    // multiple of energy_every, the six sums of squares (physical units) in
+    // d_energy, reduced on the device; energy_steps holds each row's step.
+    int energy_every = 0;
+    double* d_energy = nullptr;
+    long long energy_cap = 0;
+    std::vector<long long> energy_steps;
+    // true while fdtd_update_fields runs its steps: only those record
+    bool recording = false;
 };
 
 namespace fdtd::rt {

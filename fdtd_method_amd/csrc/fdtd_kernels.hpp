@@ -281,6 +281,63 @@ __device__ __forceinline__ void add_point_sources(C (&t)[3][V], const Coefs<C>& 
     }
 }
 
+// Probes: single field values recorded in FP64 after every step.  A probe
+// of E at plane k is listed under plane key k, one of B at plane k under key
+// k+1 (mod Nk): the sweep iteration (or strip level) that computes E at
+// plane k also computes B at plane k-1.
+struct Probe {
+    int i, j, k, comp;  // comp 0..5: Ex..Bz
+    int col, pad;       // output column
+};
+
+struct ProbeArgs {
+    const Probe* p;      // sorted by plane key
+    const int* pstart;   // probes of key k: [pstart[k], pstart[k+1])
+    int n;               // 0: none
+    int stride;          // output row length
+    double* out;         // the launch's first step's row; step s at out + s*stride
+    double inv[6];       // 1 / storage scale of Ex..Bz (values as computed in the launch)
+};
+
+// Records the probes of plane key `key` that lie in columns i0..i0+V-1 of
+// row j: E components from e (when e_ok), B components from b (when b_ok),
+// cell v only when vok[v]; values are as stored (rounded to S), in physical
+// units.
+template <typename S, typename C, int V>
+__device__ __forceinline__ void record_probes(const ProbeArgs& pa, double* out, int key, int i0,
+                                              int j, const C (&e)[3][V], const C (&b)[3][V],
+                                              bool e_ok, bool b_ok, const bool (&vok)[V]) {
+    const cptr<int> ps = as_const(pa.pstart);
+    const int q0 = ps[key], q1 = ps[key + 1];
+    if (q1 <= q0) return;
+    const cptr<Probe> pr = as_const(pa.p);
+    for (int q = q0; q < q1; ++q) {
+        const int pi = pr[q].i;
+        if (pr[q].j != j || pi < i0 || pi >= i0 + V) continue;
+        const int comp = pr[q].comp;
+        if (comp < 3 ? !e_ok : !b_ok) continue;
+        C x = C(0);
+        bool ok = false;
+#pragma unroll
+        for (int v = 0; v < V; ++v)
+            if (pi == i0 + v) {
+                ok = vok[v];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if (comp == d) x = e[d][v];
+                    if (comp == 3 + d) x = b[d][v];
+                }
+            }
+        // the scale by selects: an index into the argument array would
+        // copy it to scratch
+        double inv = pa.inv[0];
+#pragma unroll
+        for (int c = 1; c < 6; ++c)
+            if (comp == c) inv = pa.inv[c];
+        if (ok) out[pr[q].col] = static_cast<double>(static_cast<S>(x)) * inv;
+    }
+}
+
 // One cell of the E update: dense J (type JT, may be null) plus point
 // sources with values `vals`.
 template <typename S, typename C, typename JT>
@@ -531,6 +588,7 @@ struct SweepArgs {
     // first B half step, (B_n + d) + d, so that run needs no B/2 prologue.
     S* ahead[3];
     Pml<S, C> pml;  // used by PML sweeps only
+    ProbeArgs probe;  // values after this sweep's step (n = 0: none)
 };
 
 // Lane -> pack (V cells) order within a row of a PML sweep: the packs that
@@ -551,9 +609,10 @@ __device__ __forceinline__ int pml_pack(int lane, int npacks, int pi, int V) {
 }
 
 template <typename S, typename C, typename JT, int V, bool STEADY, bool PML = false,
-          bool AUTO = false>
+          bool AUTO = false, bool PROBE = false>
 __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int tile_y, int chunk,
                                            C* __restrict__ lds) {
+    static_assert(!(PML && PROBE), "PML sweeps record no probes");
     const int Ni = a.g.Ni, Nj = a.g.Nj, Nk = a.g.Nk, TY = a.g.TY;
     const int lanes = Ni / V;
     const int j0 = tile_y * TY;
@@ -722,6 +781,7 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
         }
 
         C e1[3][V];
+        C b_after[3][V];  // PROBE: B of plane kk-1 after this sweep's step
         if (active) {
             // First curl term of each E component, then the J term in front
             // of it as in the reference: (cur*J + first) - second.
@@ -814,6 +874,7 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                 const int offb = (kk - 1) * plane + row_off;
                 C nb[3][V];
                 C na[3][V];  // the carried half step (final sweeps with a.ahead)
+                C b_half[3][V];  // PROBE, steady sweeps: B after this sweep's step
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     const C ey_ip = (v < V - 1) ? eP[1][v + 1] : sEy[r * Ni + ipl];
@@ -826,6 +887,11 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                     nb[0][v] = bP[0][v] + dbx;
                     nb[1][v] = bP[1][v] + dby;
                     nb[2][v] = bP[2][v] + dbz;
+                    if constexpr (PROBE && STEADY) {
+                        b_half[0][v] = nb[0][v];
+                        b_half[1][v] = nb[1][v];
+                        b_half[2][v] = nb[2][v];
+                    }
                     if (STEADY) {
                         nb[0][v] = nb[0][v] + dbx;
                         nb[1][v] = nb[1][v] + dby;
@@ -839,6 +905,12 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                 if (!STEADY && !PML && a.ahead[0] != nullptr) {
 #pragma unroll
                     for (int d = 0; d < 3; ++d) put(a.ahead[d] + offb, na[d], 3 + d);
+                }
+                if constexpr (PROBE) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+#pragma unroll
+                        for (int v = 0; v < V; ++v) b_after[d][v] = STEADY ? b_half[d][v] : nb[d][v];
                 }
                 if constexpr (PML) {
                     // Layer cells of plane kk-1: the next step's full-step B
@@ -896,6 +968,16 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
 #pragma unroll
                 for (int d = 0; d < 3; ++d) put(a.out.b[d] + offb, nb[d], 3 + d);
             }
+            if constexpr (PROBE) {
+                // probes: E of plane kw (this chunk's planes), B of plane kk-1
+                if (own) {
+                    bool all[V];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) all[v] = true;
+                    record_probes<S, C, V>(a.probe, a.probe.out, kw, ib, j, e1, b_after, kk < k1,
+                                           kk > k0, all);
+                }
+            }
         }
         __syncthreads();
 
@@ -937,7 +1019,10 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
 // boundary planes then share one L2.
 // AUTO (FP16 storage only): the variant that applies SweepArgs::oscale to
 // what it stores and tracks its max; run only around rescaling points.
-template <typename S, typename C, typename JT, int V, bool STEADY, bool AUTO = false>
+// PROBE: the variant that records SweepArgs::probe (the step's B is held in
+// registers until then, which the plain variant has no room for).
+template <typename S, typename C, typename JT, int V, bool STEADY, bool AUTO = false,
+          bool PROBE = false>
 __global__ void __launch_bounds__(fused_max_threads(V, sizeof(C)))
 fused_sweep(SweepArgs<S, C, JT> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -946,7 +1031,8 @@ fused_sweep(SweepArgs<S, C, JT> a) {
     if (t >= a.g.ntiles) return;
     const int tile_y = t % a.g.ntile_y;
     const int chunk = t / a.g.ntile_y;
-    fused_tile<S, C, JT, V, STEADY, false, AUTO>(a, tile_y, chunk, reinterpret_cast<C*>(lds_raw));
+    fused_tile<S, C, JT, V, STEADY, false, AUTO, PROBE>(a, tile_y, chunk,
+                                                       reinterpret_cast<C*>(lds_raw));
 }
 
 // The fused sweep of a grid with a PML layer (same tiles and LDS staging;
@@ -1360,6 +1446,7 @@ struct StripArgs {
     // B: iscale[3..5], exact powers of two) is scaled and rounded to FP16
     // before use -- the same values as rescaling the state in place first
     float iscale[6];
+    ProbeArgs probe;  // values after each of the pass's L steps (n = 0: none)
 };
 
 // Value of the same register in lane-1 (lane_prev) or lane+1 (lane_next) of
@@ -1392,7 +1479,8 @@ __device__ __forceinline__ double lane_next(double x) {
 // One tile (index < a.g.ntiles) of a strip pass; every thread of the
 // workgroup calls it with the same tile.  RESCALE (FP16 storage only): the
 // pass after an in-run rescaling decision, which scales what it reads.
-template <typename S, typename C, int V, int R, int L, bool FINAL, bool RESCALE = false>
+template <typename S, typename C, int V, int R, int L, bool FINAL, bool RESCALE = false,
+          bool PROBE = false>
 __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int tile,
                                            C* __restrict__ lds) {
     const StripGeom& G = a.g;
@@ -1653,6 +1741,7 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
             }
             // B_l at plane pe-1 from E_l(pe-1) (eP) and E_l(pe) (enew)
             C bnew[3][R][V];
+            C bh[3][R][V];  // B after step l of the pass (the first half step's sum)
 #pragma unroll
             for (int r = 0; r < R; ++r)
 #pragma unroll
@@ -1667,6 +1756,9 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                     bnew[0][r][v] = bPi[l][0][r][v] + dbx;
                     bnew[1][r][v] = bPi[l][1][r][v] + dby;
                     bnew[2][r][v] = bPi[l][2][r][v] + dbz;
+                    bh[0][r][v] = bnew[0][r][v];
+                    bh[1][r][v] = bnew[1][r][v];
+                    bh[2][r][v] = bnew[2][r][v];
                     if (!(FINAL && l == L - 1)) {
                         bnew[0][r][v] = bnew[0][r][v] + dbx;
                         bnew[1][r][v] = bnew[1][r][v] + dby;
@@ -1686,6 +1778,28 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
 #pragma unroll
                     for (int d = 0; d < 3; ++d) bnew[d][r][v] = as_stored<S>(bnew[d][r][v]);
                 }
+            if constexpr (PROBE) {
+                // probes of step l: E of plane pkl, B of plane pkl-1, cells
+                // this tile stores
+                const int pe = kk - l;
+                const bool e_ok = pe >= k0 && pe < k1, b_ok = pe - 1 >= k0 && pe - 1 < k1;
+                double* const pout = a.probe.out + static_cast<size_t>(l) * a.probe.stride;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    C er[3][V], br[3][V];
+                    bool vok[V];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        vok[v] = row_out[r] && col_out[v];
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            er[d][v] = enew[d][r][v];
+                            br[d][v] = bh[d][r][v];
+                        }
+                    }
+                    record_probes<S, C, V>(a.probe, pout, pkl, gi0, gj[r], er, br, e_ok, b_ok, vok);
+                }
+            }
             // level l+1 inputs: E_l(pe-1) (previous iteration) and B_l(pe-1)
 #pragma unroll
             for (int d = 0; d < 3; ++d)
@@ -1812,27 +1926,30 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
     for (int kk = kfirst; kk <= klast; ++kk) plane_step(kk, eP, bP, eP, bP);
 }
 
-template <typename S, typename C, int V, int R, int L, bool FINAL>
+// PROBE: the variant that records StripArgs::probe after each level (the
+// level's B is held in registers until then: up to 35 more VGPRs, which the
+// plain variant does without).
+template <typename S, typename C, int V, int R, int L, bool FINAL, bool PROBE = false>
 __global__ void __launch_bounds__(strip_max_threads(V, R, L, sizeof(C)))
 strip_sweep(StripArgs<S, C> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int blk = blockIdx.x;
     const int tile = (blk & 7) * a.g.per_xcd + (blk >> 3);
     if (tile >= a.g.ntiles) return;  // whole workgroup
-    strip_tile<S, C, V, R, L, FINAL>(a, tile, reinterpret_cast<C*>(lds_raw));
+    strip_tile<S, C, V, R, L, FINAL, false, PROBE>(a, tile, reinterpret_cast<C*>(lds_raw));
 }
 
 // FP16 storage: the strip pass after an in-run rescaling decision, reading
 // the state scaled by StripArgs::iscale (a kernel of its own, so that the
 // other passes carry none of it).
-template <typename S, typename C, int V, int R, int L, bool FINAL>
+template <typename S, typename C, int V, int R, int L, bool FINAL, bool PROBE = false>
 __global__ void __launch_bounds__(strip_max_threads(V, R, L, sizeof(C)))
 strip_rescale_sweep(StripArgs<S, C> a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int blk = blockIdx.x;
     const int tile = (blk & 7) * a.g.per_xcd + (blk >> 3);
     if (tile >= a.g.ntiles) return;  // whole workgroup
-    strip_tile<S, C, V, R, L, FINAL, true>(a, tile, reinterpret_cast<C*>(lds_raw));
+    strip_tile<S, C, V, R, L, FINAL, true, PROBE>(a, tile, reinterpret_cast<C*>(lds_raw));
 }
 
 // ---------------------------------------------------------------------------
@@ -1920,6 +2037,7 @@ __device__ __forceinline__ SweepArgs<S, C, C> load_batch_grid(const BatchGrid<S,
     a.g.Ni = uni(q.Ni); a.g.Nj = uni(q.Nj); a.g.Nk = uni(q.Nk); a.g.TY = uni(q.TY);
     a.g.kchunk = uni(q.kchunk); a.g.ntile_y = uni(q.ntile_y); a.g.ntiles = uni(q.ntiles);
     a.g.per_xcd = uni(q.per_xcd);
+    a.probe = ProbeArgs{};  // batches record no probes
     return a;
 }
 
@@ -1993,6 +2111,7 @@ __device__ __forceinline__ StripArgs<S, C> load_batch_strip_grid(const BatchStri
     a.vstride = vstride;
 #pragma unroll
     for (int c6 = 0; c6 < 6; ++c6) a.iscale[c6] = 1.0f;
+    a.probe = ProbeArgs{};  // batches record no probes
     const Coefs<C>& c = p->c;
     a.c.ex_bz = uni(c.ex_bz); a.c.ex_by = uni(c.ex_by); a.c.ey_bx = uni(c.ey_bx);
     a.c.ey_bz = uni(c.ey_bz); a.c.ez_by = uni(c.ez_by); a.c.ez_bx = uni(c.ez_bx);
@@ -2169,6 +2288,38 @@ reduce_partials(const double* __restrict__ part, int nblk, int w, unsigned max_m
         const double r = is_max ? block_max(acc, sh) : block_sum(acc, sh);
         if (threadIdx.x == 0) out[col] = r;
     }
+}
+
+// out[0] = the sum of column 0 of `nblk` partial records of width w, in
+// reduce_partials' order (energy records equal fdtd_sum_squares bitwise).
+static __global__ void __launch_bounds__(kReduceThreads)
+sum_partials_col0(const double* __restrict__ part, int nblk, int w, double* __restrict__ out) {
+    __shared__ double sh[kReduceThreads];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kReduceThreads) acc = acc + part[b * w];
+    const double r = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[0] = r;
+}
+
+// Probe values read from the fields in memory (the state after a step of
+// the three-sweep or PML path): probe q's value goes to pa.out[pa.p[q].col].
+template <typename S>
+__global__ void __launch_bounds__(256)
+gather_probes(Fields<S> f, ProbeArgs pa, int Ni, int Nj) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= pa.n) return;
+    const Probe r = pa.p[q];
+    const size_t idx = static_cast<size_t>(r.i) + static_cast<size_t>(r.j) * Ni +
+                       static_cast<size_t>(r.k) * Ni * Nj;
+    const S* src = f.e[0];
+    double inv = pa.inv[0];
+#pragma unroll
+    for (int c = 1; c < 6; ++c)
+        if (r.comp == c) {
+            src = c < 3 ? f.e[c] : f.b[c - 3];
+            inv = pa.inv[c];
+        }
+    pa.out[r.col] = static_cast<double>(src[idx]) * inv;
 }
 
 // Element-wise conversions between the FP64 host layout and storage types.

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -544,13 +545,49 @@ struct SweepExtra {
     void* const* ahead = nullptr;
 };
 
-template <typename S, int V, bool STEADY, bool AUTO>
+// Probes of the steps a run queues from here: the next row.
+bool probing(const fdtd_grid* g) { return g->recording && g->nprobe > 0; }
+
+fdtd::ProbeArgs probe_args(const fdtd_grid* g) {
+    fdtd::ProbeArgs p{};
+    if (!probing(g)) return p;
+    p.p = g->d_probe;
+    p.pstart = g->d_pstart;
+    p.n = g->nprobe;
+    p.stride = g->nprobe;
+    p.out = g->d_probe_out + static_cast<size_t>(g->probe_rows) * g->nprobe;
+    for (int c = 0; c < 6; ++c) p.inv[c] = 1.0 / g->scales[c];
+    return p;
+}
+
+// Probe row of the step just completed, read from the fields in memory.
+template <typename S>
+int gather_probes_s(fdtd_grid* g) {
+    const fdtd::ProbeArgs p = probe_args(g);
+    hipLaunchKernelGGL(fdtd::gather_probes<S>, dim3((p.n + 255) / 256), dim3(256), 0, g->stream,
+                       fields_of<S>(g, g->cur), p, g->Ni, g->Nj);
+    HIP_TRY(hipGetLastError());
+    g->probe_rows += 1;
+    return FDTD_OK;
+}
+
+int gather_probes(fdtd_grid* g) {
+    if (!probing(g)) return FDTD_OK;
+    switch (g->precision) {
+        case FDTD_FP64: return gather_probes_s<double>(g);
+        case FDTD_FP32: return gather_probes_s<float>(g);
+        default: return gather_probes_s<_Float16>(g);
+    }
+}
+
+template <typename S, int V, bool STEADY, bool AUTO, bool PROBE>
 int launch_fused_a(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
                    const SweepExtra& x) {
     using C = typename fdtd::ComputeOf<S>::type;
     using JT = C;
-    auto kern = g->pml ? fdtd::fused_pml_sweep<S, C, JT, V, STEADY, AUTO>
-                       : fdtd::fused_sweep<S, C, JT, V, STEADY, AUTO>;
+    auto kern = fdtd::fused_sweep<S, C, JT, V, STEADY, AUTO, PROBE>;
+    if constexpr (!PROBE)
+        if (g->pml) kern = fdtd::fused_pml_sweep<S, C, JT, V, STEADY, AUTO>;
     fdtd::SweepArgs<S, C, JT> a;
     a.in = fields_of<S>(g, in);
     a.out = fields_of<S>(g, out);
@@ -569,22 +606,29 @@ int launch_fused_a(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
     a.pml = make_pml<S, C>(g, in, out);
     for (int c = 0; c < 6; ++c) a.oscale[c] = C(x.oscale ? x.oscale[c] : 1.0);
     a.maxbits = x.maxbits;
+    a.probe = PROBE ? probe_args(g) : fdtd::ProbeArgs{};
     const size_t lds = sweep_lds_bytes(g, t.TY);
     if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
     const int threads = (t.TY + 1) * (g->Ni / V);
     hipLaunchKernelGGL(kern, dim3(8 * a.g.per_xcd), dim3(threads), lds, g->stream, a);
     HIP_TRY(hipGetLastError());
+    if (PROBE) g->probe_rows += 1;
     return FDTD_OK;
 }
 
-// FP16 sweeps that rescale or track the max take the AUTO variant.
+// FP16 sweeps that rescale or track the max take the AUTO variant; sweeps
+// of a run with probes (not on PML grids, which gather them after whole
+// steps), the PROBE variant.
 template <typename S, int V, bool STEADY>
 int launch_fused_v(fdtd_grid* g, const Tuning& t, int slot, int in, int out,
                    const SweepExtra& x) {
+    const bool probe = probing(g) && !g->pml;
     if constexpr (sizeof(S) == 2)
         if (x.oscale || x.maxbits)
-            return launch_fused_a<S, V, STEADY, true>(g, t, slot, in, out, x);
-    return launch_fused_a<S, V, STEADY, false>(g, t, slot, in, out, x);
+            return probe ? launch_fused_a<S, V, STEADY, true, true>(g, t, slot, in, out, x)
+                         : launch_fused_a<S, V, STEADY, true, false>(g, t, slot, in, out, x);
+    return probe ? launch_fused_a<S, V, STEADY, false, true>(g, t, slot, in, out, x)
+                 : launch_fused_a<S, V, STEADY, false, false>(g, t, slot, in, out, x);
 }
 
 template <typename S, bool STEADY>
@@ -671,10 +715,15 @@ int launch_strip_v(fdtd_grid* g, int slot, int in, int out, void* const* in_b, v
     if constexpr (!strip_built(sizeof(C), V, R)) {
         return fail(FDTD_ERR_UNSUPPORTED, "strip variant V=%d R=%d not built", V, R);
     } else {
-        // FP16: the pass after an in-run rescaling decision scales its reads
-        auto kern = fdtd::strip_sweep<S, C, V, R, L, FINAL>;
+        // FP16: the pass after an in-run rescaling decision scales its
+        // reads; passes of a run with probes record them
+        const bool probe = probing(g);
+        auto kern = probe ? fdtd::strip_sweep<S, C, V, R, L, FINAL, true>
+                          : fdtd::strip_sweep<S, C, V, R, L, FINAL, false>;
         if constexpr (sizeof(S) == 2)
-            if (g->pass_rescale) kern = fdtd::strip_rescale_sweep<S, C, V, R, L, FINAL>;
+            if (g->pass_rescale)
+                kern = probe ? fdtd::strip_rescale_sweep<S, C, V, R, L, FINAL, true>
+                             : fdtd::strip_rescale_sweep<S, C, V, R, L, FINAL, false>;
         const BlockTuning& t = g->btune;
         fdtd::StripArgs<S, C> a;
         a.in = fields_of<S>(g, in);
@@ -690,10 +739,12 @@ int launch_strip_v(fdtd_grid* g, int slot, int in, int out, void* const* in_b, v
         a.c = make_coefs<C>(g);
         a.g = make_strip_geom(g, t);
         for (int c = 0; c < 6; ++c) a.iscale[c] = g->pass_iscale[c];
+        a.probe = probe_args(g);
         const size_t lds = fdtd::strip_lds_bytes(L, a.g.W, V, sizeof(C));
         if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kern), lds)) return rc;
         hipLaunchKernelGGL(kern, dim3(8 * a.g.per_xcd), dim3(64 * a.g.W), lds, g->stream, a);
         HIP_TRY(hipGetLastError());
+        if (a.probe.n > 0) g->probe_rows += L;
         return FDTD_OK;
     }
 }
@@ -970,7 +1021,9 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
 int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
     // Blocked passes take point sources and sparse J values; a dense J array
     // (uploaded J) runs single sweeps, which read it.
-    if (g->btune.L >= 1 && nsteps >= g->btune.L && !g->pml && !g->j_dense_active)
+    // Probes are recorded by strip passes, not by patch passes.
+    if (g->btune.L >= 1 && nsteps >= g->btune.L && !g->pml && !g->j_dense_active &&
+        (g->btune.R > 0 || !probing(g)))
         return run_blocked(g, nsteps, mid);
     // A carried half step replaces the B/2 prologue (bit-identical: the
     // final sweep of the previous run computed (B_n + d) + d from the same
@@ -1043,8 +1096,98 @@ int run_3sweep(fdtd_grid* g, int nsteps) {
         if (!rc) rc = plain_B(g, false);
         if (rc) return rc;
         g->step += 1;
+        if ((rc = gather_probes(g))) return rc;
     }
     return FDTD_OK;
+}
+
+// Energy record of the current state: the six sums of squares (physical
+// units) reduced into the next row of d_energy, queued without a wait, in
+// fdtd_sum_squares' summation order.
+int record_energy(fdtd_grid* g) {
+    const long long row = static_cast<long long>(g->energy_steps.size());
+    if (row >= g->energy_cap) return fail(FDTD_ERR_INVALID_ARGUMENT, "energy records full");
+    const int nb = std::min<int>(kReduceBlocks, grid_blocks(g->N));
+    for (int c = 0; c < 6; ++c) {
+        const void* p = field_ptr(g, c);
+        const double inv = 1.0 / g->scales[c];
+        double* part = g->d_part + (size_t)2 * kReduceBlocks * c;
+        switch (g->precision) {
+            case FDTD_FP64:
+                hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<double>, dim3(nb), dim3(256), 0,
+                                   g->stream, static_cast<const double*>(p), g->N, inv, part);
+                break;
+            case FDTD_FP32:
+                hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<float>, dim3(nb), dim3(256), 0,
+                                   g->stream, static_cast<const float*>(p), g->N, inv, part);
+                break;
+            default:
+                hipLaunchKernelGGL(fdtd::sumsq_maxabs_partial<_Float16>, dim3(nb), dim3(256), 0,
+                                   g->stream, static_cast<const _Float16*>(p), g->N, inv, part);
+        }
+        hipLaunchKernelGGL(fdtd::sum_partials_col0, dim3(1), dim3(256), 0, g->stream, part, nb, 2,
+                           g->d_energy + 6 * row + c);
+    }
+    HIP_TRY(hipGetLastError());
+    g->energy_steps.push_back(g->step);
+    return FDTD_OK;
+}
+
+// Energy records a call of nsteps steps adds: the steps s in
+// (step, step + nsteps] with s a multiple of energy_every.
+long long energy_records_in(const fdtd_grid* g, long long nsteps) {
+    if (g->energy_every <= 0 || nsteps <= 0) return 0;
+    const long long e = g->energy_every;
+    auto floor_div = [e](long long a) { return a >= 0 ? a / e : -((-a + e - 1) / e); };
+    return floor_div(g->step + nsteps) - floor_div(g->step);
+}
+
+int check_record_capacity(const fdtd_grid* g, long long nsteps) {
+    if (g->nprobe > 0 && g->probe_rows + nsteps > g->probe_cap)
+        return fail(FDTD_ERR_INVALID_ARGUMENT,
+                    "probe rows full: %lld of %lld used, %lld more steps asked",
+                    g->probe_rows, g->probe_cap, nsteps);
+    if ((long long)g->energy_steps.size() + energy_records_in(g, nsteps) > g->energy_cap &&
+        g->energy_every > 0)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "energy records full: %zu of %lld used",
+                    g->energy_steps.size(), g->energy_cap);
+    return FDTD_OK;
+}
+
+// The steps of the stepping entry points, recording probes and energy.
+// Energy records split the run at the recorded steps: every run leaves the
+// state (E_n, B_n) in memory, and the carried half step makes the split
+// bit-identical.  Probes of PML grids are gathered after single-step runs
+// (their sweeps keep B half a step ahead in memory).
+int run_steps(fdtd_grid* g, int nsteps, bool three_sweep) {
+    auto segment = [&](int n) -> int {
+        if (three_sweep || !g->fused_ok) return run_3sweep(g, n);
+        if (probing(g) && g->pml) {
+            for (int s = 0; s < n; ++s) {
+                if (int rc = run_fused(g, 1)) return rc;
+                if (int rc = gather_probes(g)) return rc;
+            }
+            return FDTD_OK;
+        }
+        return run_fused(g, n);
+    };
+    if (int rc = check_record_capacity(g, nsteps)) return rc;
+    g->recording = true;
+    int rc = FDTD_OK;
+    if (g->energy_every <= 0) {
+        rc = segment(nsteps);
+    } else {
+        const long long e = g->energy_every;
+        for (int done = 0; done < nsteps && rc == FDTD_OK;) {
+            const long long ph = ((g->step % e) + e) % e;
+            const int m = static_cast<int>(std::min<long long>(nsteps - done, e - ph));
+            rc = segment(m);
+            done += m;
+            if (rc == FDTD_OK && ((g->step % e) + e) % e == 0) rc = record_energy(g);
+        }
+    }
+    g->recording = false;
+    return rc;
 }
 
 // ---- conversions and reductions ------------------------------------------------
@@ -1338,6 +1481,10 @@ int fdtd_destroy(fdtd_grid* g) {
     if (g->d_part) (void)hipFree(g->d_part);
     if (g->d_out) (void)hipFree(g->d_out);
     if (g->d_max) (void)hipFree(g->d_max);
+    if (g->d_probe) (void)hipFree(g->d_probe);
+    if (g->d_pstart) (void)hipFree(g->d_pstart);
+    if (g->d_probe_out) (void)hipFree(g->d_probe_out);
+    if (g->d_energy) (void)hipFree(g->d_energy);
     for (auto& e : g->ev)
         if (e) (void)hipEventDestroy(e);
     if (g->stream) (void)hipStreamDestroy(g->stream);
@@ -1600,8 +1747,7 @@ int fdtd_update_fields(fdtd_grid* g, int nsteps) {
     if (nsteps == 0) return FDTD_OK;
     if (int rc = flush_sources(g)) return rc;
     g->stats[0] += nsteps;
-    if (!g->fused_ok) return run_3sweep(g, nsteps);
-    return run_fused(g, nsteps);
+    return run_steps(g, nsteps, false);
 }
 
 int fdtd_update_fields_with_currents(fdtd_grid* g, int nsteps, const int* counts,
@@ -1615,6 +1761,7 @@ int fdtd_update_fields_with_currents(fdtd_grid* g, int nsteps, const int* counts
     }
     if (total == 0) return fdtd_update_fields(g, nsteps);
     if (!index || !comp || !value) return fail(FDTD_ERR_INVALID_ARGUMENT, "null current values");
+    if (int rc = check_record_capacity(g, nsteps)) return rc;
     for (long long q = 0; q < total; ++q) {
         if (comp[q] < FDTD_JX || comp[q] > FDTD_JZ)
             return fail(FDTD_ERR_INVALID_COMPONENT, "ERROR: Invalid field component");
@@ -1679,6 +1826,119 @@ int fdtd_update_fields_with_currents(fdtd_grid* g, int nsteps, const int* counts
     return rc;
 }
 
+int fdtd_set_probes(fdtd_grid* g, int n, const long long* index, const int* comp,
+                    long long capacity) {
+    if (!g || n < 0 || (n > 0 && (!index || !comp || capacity <= 0)))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (n > 0 && capacity > (long long)(SIZE_MAX / sizeof(double)) / n)
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "probe capacity too large");
+    std::vector<fdtd::Probe> list(n);
+    const size_t plane = (size_t)g->Ni * g->Nj;
+    for (int q = 0; q < n; ++q) {
+        if (comp[q] < FDTD_EX || comp[q] > FDTD_BZ)
+            return fail(FDTD_ERR_INVALID_COMPONENT, "ERROR: Invalid field component");
+        if (index[q] < 0 || (size_t)index[q] >= g->N)
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "index %lld out of range", index[q]);
+        fdtd::Probe& p = list[q];
+        p.i = (int)(index[q] % g->Ni);
+        p.j = (int)((index[q] / g->Ni) % g->Nj);
+        p.k = (int)(index[q] / (long long)plane);
+        p.comp = comp[q];
+        p.col = q;
+        p.pad = 0;
+    }
+    // plane key: E at plane k under k, B at plane k under k+1 (mod Nk)
+    auto key = [g](const fdtd::Probe& p) { return p.comp < 3 ? p.k : (p.k + 1) % g->Nk; };
+    std::stable_sort(list.begin(), list.end(),
+                     [&](const fdtd::Probe& a, const fdtd::Probe& b) { return key(a) < key(b); });
+    std::vector<int> start(g->Nk + 1, 0);
+    for (const auto& p : list) start[key(p) + 1] += 1;
+    for (int k = 0; k < g->Nk; ++k) start[k + 1] += start[k];
+    // queued work may read the old tables
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (g->d_probe) HIP_TRY(hipFree(g->d_probe));
+    if (g->d_pstart) HIP_TRY(hipFree(g->d_pstart));
+    if (g->d_probe_out) HIP_TRY(hipFree(g->d_probe_out));
+    g->d_probe = nullptr;
+    g->d_pstart = nullptr;
+    g->d_probe_out = nullptr;
+    g->nprobe = 0;
+    g->probe_cap = 0;
+    g->probe_rows = 0;
+    if (n == 0) return FDTD_OK;
+    HIP_TRY(hipMalloc(&g->d_probe, sizeof(fdtd::Probe) * n));
+    HIP_TRY(hipMalloc(&g->d_pstart, sizeof(int) * (g->Nk + 1)));
+    HIP_TRY(hipMalloc(&g->d_probe_out, sizeof(double) * n * capacity));
+    HIP_TRY(hipMemcpy(g->d_probe, list.data(), sizeof(fdtd::Probe) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->d_pstart, start.data(), sizeof(int) * (g->Nk + 1), hipMemcpyHostToDevice));
+    // rows not yet written read as NaN (all bits set)
+    HIP_TRY(hipMemsetAsync(g->d_probe_out, 0xFF, sizeof(double) * n * capacity, g->stream));
+    g->nprobe = n;
+    g->probe_cap = capacity;
+    return FDTD_OK;
+}
+
+int fdtd_probe_rows(const fdtd_grid* g, long long* rows) {
+    if (!g || !rows) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    *rows = g->probe_rows;
+    return FDTD_OK;
+}
+
+int fdtd_read_probes(fdtd_grid* g, long long row0, long long nrows, double* host) {
+    if (!g || row0 < 0 || nrows < 0 || row0 + nrows > g->probe_rows || (nrows > 0 && !host))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid probe rows");
+    if (nrows == 0) return FDTD_OK;
+    HIP_TRY(hipMemcpyAsync(host, g->d_probe_out + (size_t)row0 * g->nprobe,
+                           sizeof(double) * (size_t)nrows * g->nprobe, hipMemcpyDeviceToHost,
+                           g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    return FDTD_OK;
+}
+
+int fdtd_set_energy_interval(fdtd_grid* g, int every, long long capacity) {
+    if (!g || every < 0 || (every > 0 && capacity <= 0) ||
+        capacity > (long long)(SIZE_MAX / (6 * sizeof(double))))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (g->d_energy) HIP_TRY(hipFree(g->d_energy));
+    g->d_energy = nullptr;
+    g->energy_every = 0;
+    g->energy_cap = 0;
+    g->energy_steps.clear();
+    if (every == 0) return FDTD_OK;
+    HIP_TRY(hipMalloc(&g->d_energy, sizeof(double) * 6 * capacity));
+    g->energy_every = every;
+    g->energy_cap = capacity;
+    return FDTD_OK;
+}
+
+int fdtd_energy_rows(const fdtd_grid* g, long long* rows) {
+    if (!g || !rows) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    *rows = static_cast<long long>(g->energy_steps.size());
+    return FDTD_OK;
+}
+
+int fdtd_read_energy(fdtd_grid* g, long long row0, long long nrows, long long* steps,
+                     double* sums) {
+    const long long have = static_cast<long long>(g ? g->energy_steps.size() : 0);
+    if (!g || row0 < 0 || nrows < 0 || row0 + nrows > have || (nrows > 0 && !sums))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid energy rows");
+    if (nrows == 0) return FDTD_OK;
+    HIP_TRY(hipMemcpyAsync(sums, g->d_energy + (size_t)6 * row0, sizeof(double) * 6 * nrows,
+                           hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    if (steps)
+        for (long long r = 0; r < nrows; ++r) steps[r] = g->energy_steps[row0 + r];
+    return FDTD_OK;
+}
+
+int fdtd_rewind_records(fdtd_grid* g) {
+    if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    g->probe_rows = 0;
+    g->energy_steps.clear();
+    return FDTD_OK;
+}
+
 int fdtd_get_stats(const fdtd_grid* g, long long out[4]) {
     if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
     for (int k = 0; k < 4; ++k) out[k] = g->stats[k];
@@ -1687,8 +1947,9 @@ int fdtd_get_stats(const fdtd_grid* g, long long out[4]) {
 
 int fdtd_update_fields_3sweep(fdtd_grid* g, int nsteps) {
     if (!g || nsteps < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (nsteps == 0) return FDTD_OK;
     if (int rc = flush_sources(g)) return rc;
-    return run_3sweep(g, nsteps);
+    return run_steps(g, nsteps, true);
 }
 
 int fdtd_get_step(const fdtd_grid* g, long long* step) {

@@ -240,6 +240,54 @@ class FDTD:
         _lib.check(self._L.fdtd_get_stats(self._h, out))
         return dict(steps=out[0], passes=out[1], sweeps=out[2], prologues=out[3])
 
+    def set_probes(self, cells, capacity: int) -> None:
+        """Record FP64 values of single cells after every step on the device.
+
+        cells: iterable of (i, j, k, component) with component 0..5 (Ex..Bz);
+        capacity: rows (steps) the device buffer holds.  An empty list removes
+        the probes."""
+        cells = list(cells)
+        n = len(cells)
+        idx = (ctypes.c_longlong * max(1, n))()
+        comp = (ctypes.c_int * max(1, n))()
+        for q, (i, j, k, c) in enumerate(cells):
+            Ni, Nj, _ = self.shape
+            idx[q] = int(i) + int(j) * Ni + int(k) * Ni * Nj
+            comp[q] = int(c)
+        _lib.check(self._L.fdtd_set_probes(self._h, n, idx, comp, int(capacity)))
+        self._nprobe = n
+
+    def probe_rows(self) -> int:
+        out = ctypes.c_longlong()
+        _lib.check(self._L.fdtd_probe_rows(self._h, ctypes.byref(out)))
+        return out.value
+
+    def read_probes(self, row0: int = 0, nrows=None) -> np.ndarray:
+        """Probe rows [row0, row0 + nrows) as an (nrows, nprobes) FP64 array."""
+        if nrows is None:
+            nrows = self.probe_rows() - row0
+        out = np.empty((nrows, getattr(self, "_nprobe", 0)), dtype=np.float64)
+        _lib.check(self._L.fdtd_read_probes(self._h, int(row0), int(nrows), _dptr(out)))
+        return out
+
+    def set_energy_interval(self, every: int, capacity: int = 0) -> None:
+        """Sums of squares of Ex..Bz after every step that is a multiple of
+        `every`, reduced on the device into a buffer of `capacity` rows."""
+        _lib.check(self._L.fdtd_set_energy_interval(self._h, int(every), int(capacity)))
+
+    def energy_records(self):
+        """(steps, sums): the recorded steps and an (n, 6) array of sums."""
+        n = ctypes.c_longlong()
+        _lib.check(self._L.fdtd_energy_rows(self._h, ctypes.byref(n)))
+        steps = (ctypes.c_longlong * max(1, n.value))()
+        sums = np.empty((n.value, 6), dtype=np.float64)
+        _lib.check(self._L.fdtd_read_energy(self._h, 0, n.value, steps, _dptr(sums)))
+        return [steps[r] for r in range(n.value)], sums
+
+    def rewind_records(self) -> None:
+        """Probe rows and energy records restart at row 0."""
+        _lib.check(self._L.fdtd_rewind_records(self._h))
+
     def set_point_sources(self, sources) -> None:
         """sources: iterable of dicts {i,j,k,comp(0..2),kind,amp,period,t0,sigma,phase}."""
         arr = (_lib.PointSource * max(1, len(sources)))()

@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "fdtd_mi355x.h"
@@ -234,6 +235,62 @@ public:
         return s;
     }
 
+    // Probes (no counterpart in the reference, whose drivers read single
+    // values of get_field() after each update_fields()): the FP64 values of
+    // the given cells after every later step, recorded on the device into
+    // `capacity` rows, read back with read_probes().  Deferred steps run
+    // first, so they are not recorded.
+    struct ProbeCell {
+        int i, j, k;
+        Component c;
+    };
+    void set_probes(const std::vector<ProbeCell>& cells, long long capacity) {
+        run_pending();
+        std::vector<long long> index;
+        std::vector<int> comp;
+        for (const auto& p : cells) {
+            index.push_back(p.i + static_cast<long long>(parameters_.Ni) *
+                                      (p.j + static_cast<long long>(parameters_.Nj) * p.k));
+            comp.push_back(static_cast<int>(p.c));
+        }
+        check_status(fdtd_set_probes(grid_, static_cast<int>(cells.size()), index.data(),
+                                     comp.data(), capacity));
+        nprobes_ = cells.size();
+    }
+    long long probe_rows() {
+        run_pending();
+        long long rows = 0;
+        check_status(fdtd_probe_rows(grid_, &rows));
+        return rows;
+    }
+    // All rows recorded so far, row-major (probe_rows() x cells).
+    std::vector<double> read_probes() {
+        const long long rows = probe_rows();
+        std::vector<double> out(static_cast<std::size_t>(rows) * nprobes_);
+        check_status(fdtd_read_probes(grid_, 0, rows, out.data()));
+        return out;
+    }
+    // Energy records: the six sums of squares (Ex..Bz, physical units)
+    // after every step that is a multiple of `every`, reduced on the device.
+    void set_energy_interval(int every, long long capacity) {
+        run_pending();
+        check_status(fdtd_set_energy_interval(grid_, every, capacity));
+    }
+    std::vector<std::pair<long long, std::array<double, 6>>> energy_records() {
+        run_pending();
+        long long rows = 0;
+        check_status(fdtd_energy_rows(grid_, &rows));
+        std::vector<long long> steps(static_cast<std::size_t>(rows));
+        std::vector<double> sums(static_cast<std::size_t>(rows) * 6);
+        check_status(fdtd_read_energy(grid_, 0, rows, steps.data(), sums.data()));
+        std::vector<std::pair<long long, std::array<double, 6>>> out(steps.size());
+        for (std::size_t r = 0; r < steps.size(); ++r) {
+            out[r].first = steps[r];
+            for (int c = 0; c < 6; ++c) out[r].second[c] = sums[6 * r + c];
+        }
+        return out;
+    }
+
     // Temporal blocking (no counterpart in the reference): L steps per pass
     // over HBM with the wavefront-strip kernel (0 = defaults; L = 0 turns it
     // off), or the fastest of single sweeps and blocked passes timed on
@@ -317,6 +374,7 @@ private:
     }
 
     int pending_ = 0;
+    std::size_t nprobes_ = 0;
     std::vector<int> w_counts_;  // writes before deferred step s; the last: after them
     std::vector<long long> w_index_;
     std::vector<int> w_comp_;

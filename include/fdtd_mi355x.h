@@ -154,6 +154,46 @@ int fdtd_update_fields_with_currents(fdtd_grid* grid, int nsteps, const int* cou
  * single-step sweeps, B half-step prologues}. */
 int fdtd_get_stats(const fdtd_grid* grid, long long out[4]);
 
+/* Probes: the FP64 values (physical units, as stored) of n single field
+ * values after every step of fdtd_update_fields, fdtd_update_fields_3sweep
+ * and fdtd_update_fields_with_currents, one row of n values per step in a
+ * device buffer of `capacity` rows, written by the stepping kernels
+ * themselves (fused sweeps, strip passes) or gathered after each step
+ * (three-sweep steps, PML grids), with no host wait.  Replaces reading
+ * single values of get_field() (include/FDTD/FDTD.h:38) after each
+ * update_fields() call of the reference's drivers.  Probe q is cell
+ * index[q] (i + j*Ni + k*Ni*Nj) of component[q] (FDTD_EX..FDTD_BZ).  A
+ * stepping call that would need more rows than remain fails with
+ * FDTD_ERR_INVALID_ARGUMENT before any step runs.  n = 0 removes the
+ * probes; a new list restarts at row 0.  Patch-blocked passes record no
+ * probes: with probes set, runs take strip passes or single sweeps. */
+int fdtd_set_probes(fdtd_grid* grid, int n, const long long* index, const int* component,
+                    long long capacity);
+/* Rows written so far (queued steps included). */
+int fdtd_probe_rows(const fdtd_grid* grid, long long* rows);
+/* Rows row0 .. row0+nrows-1 into host (nrows * n doubles, row-major);
+ * waits for the grid's stream. */
+int fdtd_read_probes(fdtd_grid* grid, long long row0, long long nrows, double* host);
+
+/* Energy records: after every step whose number is a multiple of `every`,
+ * the six sums of squares of fdtd_sum_squares (bitwise the same values),
+ * reduced on the device into one row of a buffer of `capacity` rows, with
+ * no host wait.  No counterpart in the reference, whose drivers would call
+ * a full reduction between update_fields() calls.  every = 0 turns them
+ * off; a new interval restarts at row 0.  A stepping call that would need
+ * more rows than remain fails before any step runs.  Runs are split at the
+ * recorded steps, bit-identically (FP16 grids with in-run rescaling take
+ * their rescaling decisions within each part). */
+int fdtd_set_energy_interval(fdtd_grid* grid, int every, long long capacity);
+int fdtd_energy_rows(const fdtd_grid* grid, long long* rows);
+/* Rows row0 .. row0+nrows-1: the step of each row into steps (may be
+ * null), the six sums of each into sums (6 * nrows doubles); waits for the
+ * grid's stream. */
+int fdtd_read_energy(fdtd_grid* grid, long long row0, long long nrows, long long* steps,
+                     double* sums);
+/* Probe rows and energy records restart at row 0 (tables kept). */
+int fdtd_rewind_records(fdtd_grid* grid);
+
 /* Replace the list of waveform point sources (evaluated on the device). */
 int fdtd_set_point_sources(fdtd_grid* grid, int n, const fdtd_point_source* sources);
 

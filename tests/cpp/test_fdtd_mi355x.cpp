@@ -282,6 +282,59 @@ void test_deferred_steps() {
     EXPECT_TRUE(same);
 }
 
+void test_probes_and_energy() {
+    // Probes record every deferred step of the drivers' loop on the device;
+    // each row equals the fields read after stepping one call at a time,
+    // and the energy records equal the sums of squares of the read fields.
+    Parameters p{128, 64, 32, 0, 1, 0, 1, 0, 1, 1.0, 1.0, 1.0};
+    const FP dt = 0.5 / (FDTD_const::C * std::sqrt(3.0));
+    FDTD a(p, dt, g_precision), b(p, dt, g_precision);
+    const int centre = 64 + 32 * 128 + 16 * 128 * 64;
+    const std::vector<FDTD::ProbeCell> cells = {
+        {64, 32, 16, Component::EZ}, {65, 32, 16, Component::BY}, {0, 0, 0, Component::EX},
+        {127, 63, 31, Component::BZ}, {66, 30, 17, Component::EY}, {64, 33, 15, Component::BX}};
+    const int steps = 10;
+    a.set_probes(cells, steps);
+    a.set_energy_interval(5, 2);
+    std::vector<double> expect;
+    std::vector<double> energy;
+    for (int t = 0; t < steps; ++t) {
+        const double v = std::sin(2.0 * FDTD_const::PI * (t + 1) / 20.0);
+        for (FDTD* g : {&a, &b}) g->get_field(Component::JZ)[centre] = v;
+        a.update_fields();   // deferred
+        b.update_fields(1);
+        for (const auto& c : cells)
+            expect.push_back(b.get_field(c.c)[c.i + 128 * (c.j + 64 * c.k)]);
+        if ((t + 1) % 5 == 0) {
+            for (int c = 0; c < 6; ++c) {
+                double e = 0;
+                for (int i = 0; i < (int)b.cells(); ++i) {
+                    const double x = b.get_field(static_cast<Component>(c))[i];
+                    e += x * x;
+                }
+                energy.push_back(e);
+            }
+        }
+    }
+    EXPECT_TRUE(a.pending_steps() == steps);
+    const std::vector<double> rows = a.read_probes();
+    EXPECT_TRUE(rows.size() == expect.size());
+    EXPECT_TRUE(rows == expect);
+    const auto rec = a.energy_records();
+    EXPECT_TRUE(rec.size() == 2 && rec[0].first == 5 && rec[1].first == 10);
+    for (std::size_t r = 0; r < rec.size() && r < 2; ++r)
+        for (int c = 0; c < 6; ++c)
+            EXPECT_NEAR(rec[r].second[c], energy[6 * r + c], 1e-12 * energy[6 * r + c] + 1e-300);
+    // a call beyond the rows left fails before any step runs
+    bool thrown = false;
+    try {
+        a.update_fields(1);
+    } catch (const std::runtime_error&) {
+        thrown = true;
+    }
+    EXPECT_TRUE(thrown);
+}
+
 void test_pml_class() {
     // FDTD_PML (src/FDTD/FDTD_PML.cpp:205-365): layer widths, J ignored in
     // the layer, a pulse absorbed, pml_percent validated.
@@ -345,7 +398,8 @@ int main(int argc, char** argv) {
     test_pml_class();
     test_time_blocking();
     test_deferred_steps();
-    ran += 6;
+    test_probes_and_energy();
+    ran += 7;
     std::printf("%d tests, %d failures\n", ran, g_failures);
     return g_failures == 0 ? 0 : 1;
 }
