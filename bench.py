@@ -16,11 +16,15 @@ bit-identical to single sweeps).
 Reported:
   value         FP32, autotuned path, Gcell-updates/s (wall clock over K
                 steps, synchronised on both sides)
-  roofline      the kernel of that path.  Blocked passes: 48 B per
-                cell-update x L steps per launch / the average launch time
-                from HIP events on the grid's stream, an effective bandwidth
-                that can exceed 8 TB/s; traffic = measured HBM bytes per
-                launch (rocprofv3 counter runs committed under profiles/)
+  roofline      the steady kernel of that path: achieved = one read and
+                one write of E and B (48 B per cell in FP32) per launch /
+                the average launch time (HIP events around each launch on
+                the grid's stream, inside the timed region), frac = that /
+                8 TB/s; traffic = measured HBM bytes per launch (rocprofv3
+                FETCH_SIZE/WRITE_SIZE runs committed in
+                profiles/hbm_traffic.json) and traffic_frac its rate;
+                valu_busy from the same counter runs; for blocked passes
+                effective_frac restates the pass as L single steps
   fused_sweep_roofline  the same for the single-step fused sweep
   paths         FP32/FP64/FP16 autotuned and fused, and the
                 reference-structured FP64 three-sweep path
@@ -46,8 +50,10 @@ python bench.py --workload blocked [--dims 1024,1024,256] [--steps 4000]
 python bench.py --workload pml [--n 512] [--pml 0.2] [--steps 1000]
     the centre-source workload on the PML class (absorbing layer of
     int(N*pml) cells per side, as the reference driver's PML block): FP32 and
-    FP64 fused, roofline over 12 words per main-box cell and 36 per layer
-    cell, error vs FP64, CPU baseline of the reference PML class (256^3 sample)
+    FP64, each with the two in-place phase sweeps per step and with the fused
+    PML sweep; roofline over 12 words per main-box cell and 36 per layer cell
+    per step, error vs FP64, CPU baseline of the reference PML class (256^3
+    sample)
 python bench.py --workload precision [--n 768] [--steps 2000]
     768^3, superposition of 8 seeded plane waves plus the centre Jz source:
     FP64, FP32 and FP16 (per-component power-of-two scales) runs, per
@@ -509,6 +515,9 @@ def pml_bytes_per_step(shape, widths, word):
 
 
 def bench_pml(a):
+    """The PML workload on both PML step forms: the two in-place phase sweeps
+    per step (pml_e_phase, pml_b_phase) and the fused PML sweep.  `value` is
+    the grid's default form in FP32."""
     from fdtd_method_amd import FDTD_PML
     n, K, pml = a.n, a.steps, a.pml
     dt = 0.5 / (C * np.sqrt(3.0))
@@ -516,25 +525,32 @@ def bench_pml(a):
                t0=120 * dt, sigma=40 * dt, phase=0.0)
     cells = float(n) ** 3
 
-    def make(prec):
+    def make(prec, form):
         g = FDTD_PML(Parameters(n, n, n), dt, pml, prec)
         g.set_point_sources([src])
-        if a.tuning:
-            g.set_tuning(*[int(x) for x in a.tuning.split(",")])
-            return g
-        shape, ms = g.autotune(reps=3)
-        log(f"pml autotune {prec.name}: V={shape[0]} TY={shape[1]} kchunk={shape[2]} "
-            f"({ms:.3f} ms/sweep)")
+        if form == "phases":
+            g.set_pml_path(1)
+        if form == "fused":
+            g.set_pml_path(0)
+            if a.tuning:
+                g.set_tuning(*[int(x) for x in a.tuning.split(",")])
+            else:
+                shape, ms = g.autotune(reps=3)
+                log(f"pml autotune {prec.name}: V={shape[0]} TY={shape[1]} kchunk={shape[2]} "
+                    f"({ms:.3f} ms/sweep)")
         return g
 
+    default_form = "phases" if make(Precision.FP32, "default").get_pml_path()["path"] == 1 \
+        else "fused"
     if a.profile:
         # one precision only, for rocprofv3 runs
         prec = {"fp32": Precision.FP32, "fp64": Precision.FP64, "fp16": Precision.FP16}[a.profile]
-        g = make(prec)
+        g = make(prec, a.pml_form or default_form)
         r = run_path(g, K, a.warmup)
         nbytes, main, layer = pml_bytes_per_step((n, n, n), g.pml_widths, WORD[prec])
         ms_copy = calibration_copy(1 << 30, 5)
         print(json.dumps(dict(profile=a.profile, workload="pml", n=n, pml=pml, steps=K,
+                              form=a.pml_form or default_form, pml_path=g.get_pml_path(),
                               tuning=g.get_tuning(), widths=g.pml_widths,
                               alg_bytes_per_step=nbytes, main_cells=main, layer_cells=layer,
                               **r, calibration_copy=dict(bytes=1 << 30, ms=ms_copy))),
@@ -544,20 +560,35 @@ def bench_pml(a):
     res = {}
     grids = {}
     for prec in (Precision.FP32, Precision.FP64):
-        g = grids[prec] = make(prec)
-        r = run_path(g, K, a.warmup)
-        nbytes, main, layer = pml_bytes_per_step((n, n, n), g.pml_widths, WORD[prec])
-        gbs = nbytes / (r["launch_ms"] * 1e-3) / 1e9
-        res[prec] = dict(gcells=r["gcells"], ms_per_step=r["ms_per_step"],
-                         launch_ms=r["launch_ms"], alg_bytes_per_step=nbytes, alg_gbs=gbs,
-                         frac_of_peak=gbs / PEAK_GBS, launch_shape=g.get_tuning())
-        log(f"pml {prec.name}: {r['gcells']:.1f} Gcell/s, launch {r['launch_ms']:.3f} ms, "
-            f"{gbs:.0f} GB/s algorithmic")
+        for form in ("phases", "fused"):
+            g = make(prec, form)
+            r = run_path(g, K, a.warmup)
+            nbytes, main, layer = pml_bytes_per_step((n, n, n), g.pml_widths, WORD[prec])
+            gbs = nbytes / (r["steady_launch_ms"] * 1e-3) / 1e9
+            rec = dict(gcells=r["gcells"], ms_per_step=r["ms_per_step"],
+                       step_kernel_ms=r["steady_launch_ms"], alg_bytes_per_step=nbytes,
+                       alg_gbs=gbs, frac_of_peak=gbs / PEAK_GBS)
+            if form == "fused":
+                rec["launch_shape"] = g.get_tuning()
+            else:
+                rec["phase_shape"] = g.get_pml_path()
+                # the phases read E and B of main cells once more per step
+                wd = WORD[prec]
+                rec["phase_bytes_per_step"] = wd * (18 * main + 36 * layer)
+            res[f"{prec.name.lower()}_{form}"] = rec
+            log(f"pml {prec.name} {form}: {r['gcells']:.1f} Gcell/s, step kernels "
+                f"{r['steady_launch_ms']:.3f} ms, {gbs:.0f} GB/s algorithmic")
+            if form == default_form:
+                grids[prec] = g
+            else:
+                g.close()
     err = errors(grids[Precision.FP32], grids[Precision.FP64])
     widths = grids[Precision.FP32].pml_widths
     en = energy(grids[Precision.FP64])
     nbytes, main, layer = pml_bytes_per_step((n, n, n), widths, 4)
-    r32 = res[Precision.FP32]
+    r32 = res[f"fp32_{default_form}"]
+    kern = ("fdtd::pml_e_phase<float,float,float,V> + fdtd::pml_b_phase<float,float,float,V,true>"
+            if default_form == "phases" else "fdtd::fused_pml_sweep<float,float,float,V,true>")
     cpu = None if a.no_cpu_baseline else cpu_baseline(min(n, 256), pml)
     out = {
         "metric": "Gcell-updates/s and %HBM roofline of the PML (absorbing layer) step",
@@ -568,15 +599,16 @@ def bench_pml(a):
                                 f"{list(widths)}, {layer / cells:.1%} of the cells), dx=1, "
                                 "dt=0.5dx/(c*sqrt3), Gaussian-modulated Jz point source at the "
                                 f"centre, zero initial fields, {K} steps"),
-                   "main_box_cells": main, "layer_cells": layer},
+                   "main_box_cells": main, "layer_cells": layer, "pml_step_form": default_form},
         "roofline": {"bound": "hbm", "achieved": r32["alg_gbs"], "peak": PEAK_GBS, "unit": "GB/s",
                      "frac": r32["alg_gbs"] / PEAK_GBS,
                      "traffic": committed_traffic(PML_TRAFFIC_FILE, n=n, pml=pml,
-                                                  tuning=list(r32["launch_shape"])),
-                     "kernel": "fdtd::fused_pml_sweep<float,float,float,V,true>",
-                     "alg_bytes_per_launch": nbytes, "launch_ms": r32["launch_ms"]},
+                                                  form=default_form,
+                                                  tuning=list(grids[Precision.FP32].get_tuning())),
+                     "kernel": kern, "alg_bytes_per_step": nbytes,
+                     "step_kernel_ms": r32["step_kernel_ms"]},
         "cpu_baseline": cpu,
-        "paths": {"fp32_fused": res[Precision.FP32], "fp64_fused": res[Precision.FP64]},
+        "paths": res,
         "errors_vs_fp64": {"fp32": err},
         "energy_fp64": en,
     }
@@ -603,6 +635,8 @@ def main():
     ap.add_argument("--workload", choices=["point", "batch", "precision", "blocked", "pml"],
                     default="point")
     ap.add_argument("--pml", type=float, default=0.2, help="pml_percent of --workload pml")
+    ap.add_argument("--pml-form", choices=["phases", "fused"], default=None,
+                    help="PML step form for --workload pml --profile (default: the grid's)")
     ap.add_argument("--dims", default="1024,1024,256", help="grid of --workload blocked")
     ap.add_argument("--grids", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)

@@ -6,8 +6,9 @@ host-side mirror of the reference interface.
 """
 from .structures import (C, PI, Axis, Component, CurrentParameters, Parameters, Precision,
                          SelectedFields)
-from .fdtd import FDTD, FDTD_PML, Field, calibration_copy, pml_profile
+from .fdtd import FDTD, FDTD_PML, Field, calibration_copy, pml_profile, set_tune_cache
 from .batch import Batch, plane_wave_batch
 
 __all__ = ["C", "PI", "Axis", "Component", "CurrentParameters", "Parameters", "Precision",
-           "SelectedFields", "FDTD", "FDTD_PML", "Field", "calibration_copy", "pml_profile", "Batch", "plane_wave_batch"]
+           "SelectedFields", "FDTD", "FDTD_PML", "Field", "calibration_copy", "pml_profile",
+           "set_tune_cache", "Batch", "plane_wave_batch"]

@@ -23,6 +23,7 @@ PATH_FUSED, PATH_3SWEEP = 0, 1
 # Every symbol include/fdtd_mi355x.h declares (checked by the CPU test suite).
 EXPORTED = [
     "fdtd_last_error", "fdtd_create", "fdtd_destroy", "fdtd_create_pml", "fdtd_get_pml",
+    "fdtd_set_pml_path", "fdtd_get_pml_path",
     "fdtd_pml_download_split", "fdtd_pml_profile", "fdtd_upload", "fdtd_download",
     "fdtd_download_plane",
     "fdtd_zero_currents", "fdtd_set_current_values", "fdtd_set_point_sources",
@@ -31,11 +32,11 @@ EXPORTED = [
     "fdtd_energy_rows", "fdtd_read_energy", "fdtd_rewind_records",
     "fdtd_update_fields", "fdtd_update_fields_3sweep", "fdtd_get_step", "fdtd_set_step",
     "fdtd_synchronize", "fdtd_get_info", "fdtd_sum_squares", "fdtd_sum_squares_noalloc",
-    "fdtd_max_abs", "fdtd_compare",
+    "fdtd_max_abs", "fdtd_compare", "fdtd_count_nonfinite",
     "fdtd_checksum", "fdtd_add_plane_wave",
     "fdtd_set_scales", "fdtd_get_scales", "fdtd_autoscale", "fdtd_set_autoscale_interval",
     "fdtd_zero_fields", "fdtd_set_step_carry", "fdtd_set_time_blocking", "fdtd_get_time_blocking",
-    "fdtd_autotune_blocking", "fdtd_set_time_blocking_strips", "fdtd_get_time_blocking_strips",
+    "fdtd_autotune_blocking", "fdtd_set_tune_cache", "fdtd_set_time_blocking_strips", "fdtd_get_time_blocking_strips",
     "fdtd_set_tuning",
     "fdtd_get_tuning", "fdtd_autotune", "fdtd_time_update_fields", "fdtd_time_update_fields_kernels",
     "fdtd_calibration_copy",
@@ -91,6 +92,8 @@ def load():
         "fdtd_create_pml": ([ctypes.POINTER(Parameters), ctypes.c_double, ctypes.c_double, _i,
                              ctypes.POINTER(_vp)], _i),
         "fdtd_get_pml": ([_vp, ctypes.POINTER(_i), _dp], _i),
+        "fdtd_set_pml_path": ([_vp, _i, _i, _i, _i], _i),
+        "fdtd_get_pml_path": ([_vp, ctypes.POINTER(_i)], _i),
         "fdtd_pml_download_split": ([_vp, _i, _dp, ctypes.c_size_t], _i),
         "fdtd_pml_profile": ([_i, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp,
                               _dp], _i),
@@ -124,6 +127,7 @@ def load():
         "fdtd_sum_squares": ([_vp, _dp], _i),
         "fdtd_sum_squares_noalloc": ([_vp, _dp], _i),
         "fdtd_max_abs": ([_vp, _dp], _i),
+        "fdtd_count_nonfinite": ([_vp, ctypes.POINTER(ctypes.c_longlong)], _i),
         "fdtd_compare": ([_vp, _vp, _dp], _i),
         "fdtd_checksum": ([_vp, ctypes.POINTER(ctypes.c_ulonglong)], _i),
         "fdtd_add_plane_wave": ([_vp, ctypes.POINTER(_i), ctypes.c_double, ctypes.c_double,
@@ -137,6 +141,7 @@ def load():
         "fdtd_set_time_blocking": ([_vp, _i, _i, _i, _i, _i], _i),
         "fdtd_get_time_blocking": ([_vp, ctypes.POINTER(_i)], _i),
         "fdtd_autotune_blocking": ([_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_float)], _i),
+        "fdtd_set_tune_cache": ([ctypes.c_char_p], _i),
         "fdtd_set_time_blocking_strips": ([_vp, _i, _i, _i, _i, _i], _i),
         "fdtd_get_time_blocking_strips": ([_vp, ctypes.POINTER(_i)], _i),
         "fdtd_set_tuning": ([_vp, _i, _i, _i], _i),

@@ -23,10 +23,13 @@ HIP_SOURCES = ["fdtd_runtime.hip", "fdtd_batch.hip"]
 # build (no FMA): the FP64 path is then bit-identical to the reference.
 # -packed-fp32-ops keeps the compiler from emitting packed FP32 math.
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wno-unused-command-line-argument", "-Wl,--no-undefined",
+    "-Wno-unused-command-line-argument",
 ]
+# roctx ranges (rocprofv3 --marker-trace) come from the ROCm roctx library.
+LINK_FLAGS = ["--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,--no-undefined",
+              "-lrocprofiler-sdk-roctx"]
 
 
 def hipcc() -> str:
@@ -37,17 +40,38 @@ def hipcc() -> str:
 
 
 def build_library(verbose: bool = False, extra_flags=None) -> str:
-    """Compile the HIP library for gfx950; returns the .so path."""
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    tmp = LIB_PATH + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, *(extra_flags or []), f"-I{INCLUDE}", f"-I{CSRC}", *srcs,
-           "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stderr[-8000:]}")
-    os.replace(tmp, LIB_PATH)
+    """Compile the HIP library for gfx950 (one hipcc per source, run
+    concurrently, then one link); returns the .so path."""
+    objs, procs = [], []
+    for src in HIP_SOURCES:
+        obj = os.path.join(PKG, src.replace(".hip", ".o.tmp"))
+        cmd = [hipcc(), *HIPCC_FLAGS, *(extra_flags or []), f"-I{INCLUDE}", f"-I{CSRC}", "-c",
+               os.path.join(CSRC, src), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                            text=True)))
+        objs.append(obj)
+    failed = []
+    for src, proc in procs:
+        _, err = proc.communicate()
+        if proc.returncode != 0:
+            failed.append(f"hipcc failed for {src} ({proc.returncode}):\n{err[-8000:]}")
+    try:
+        if failed:
+            raise RuntimeError("\n".join(failed))
+        tmp = LIB_PATH + ".tmp"
+        cmd = [hipcc(), *LINK_FLAGS, *objs, "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"link failed ({res.returncode}):\n{res.stderr[-8000:]}")
+        os.replace(tmp, LIB_PATH)
+    finally:
+        for obj in objs:
+            if os.path.exists(obj):
+                os.remove(obj)
     return LIB_PATH
 
 

@@ -767,6 +767,7 @@ int fdtd_batch_destroy(fdtd_batch* b) {
 int fdtd_batch_update_fields(fdtd_batch* b, int nsteps) {
     if (!b || nsteps < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
     if (nsteps == 0) return FDTD_OK;
+    TraceRange tr("batch_update_fields");
     if (int rc = check_steps(b)) return rc;
     if (int rc = prepare(b)) return rc;
     int swaps = 0;
@@ -780,6 +781,7 @@ int fdtd_batch_update_fields_graph(fdtd_batch* b, int nsteps, int replays) {
     if (!b || nsteps <= 0 || replays < 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
     if (int rc = check_steps(b)) return rc;
     if (int rc = prepare(b)) return rc;
+    TraceRange tr("batch_update_fields_graph");
     if (!b->graph || graph_key_of(b, nsteps) != b->graph_key) {
         drop_graph(b);
         hipGraph_t graph = nullptr;

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <map>
 #include <string>
@@ -34,6 +35,15 @@ int fail(int code, const char* fmt, ...);
         if (e_ != hipSuccess)                                                          \
             return fail(FDTD_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
     } while (0)
+
+// A roctx range over a scope (a run, a pass, a sweep): shown by
+// rocprofv3 --marker-trace, free when no tool is attached.
+struct TraceRange {
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
 
 struct Tuning {
     int V = 1, TY = 1, kchunk = 1;
@@ -152,6 +162,11 @@ struct fdtd_grid {
     void* ES[2][6] = {};        // E splits Exy Exz Eyx Eyz Ezx Ezy, ping-pong with E
     void* BS[6] = {};           // B splits Bxy Bxz Byx Byz Bzx Bzy, in place
     void* pml_tab = nullptr;    // per-axis decay/coefficient tables (compute type)
+    // PML steps: 0 = the fused PML sweep (shape tune); 1 = the two in-place
+    // phase sweeps (pml_e_phase, pml_b_phase) with launch shape ptune.  FP16
+    // sweeps that rescale or track the max take the fused sweep either way.
+    int pml_path = 0;
+    fdtd::rt::Tuning ptune;
     // set by fdtd_time_update_fields_kernels: the stepping launches are
     // bracketed by events on the grid's stream
     fdtd::rt::LaunchTrace* trace = nullptr;

@@ -179,6 +179,29 @@ __device__ __forceinline__ void buf_store_nt(S x, __amdgpu_buffer_rsrc_t r, unsi
     }
 }
 
+// V values stored as one pack through a buffer resource (non-temporal);
+// an offset at or beyond the resource's size stores nothing.
+template <typename S, typename C, int V>
+__device__ __forceinline__ void buf_store_pack(const C (&x)[V], __amdgpu_buffer_rsrc_t r,
+                                               unsigned off) {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    Pack<S, V> pk;
+#pragma unroll
+    for (int v = 0; v < V; ++v) pk.v[v] = static_cast<S>(x[v]);
+    constexpr int bytes = static_cast<int>(sizeof(S)) * V;
+    if constexpr (bytes == 2)
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, pk), r, off, 0, kBufNT);
+    else if constexpr (bytes == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, pk), r, off, 0, kBufNT);
+    else if constexpr (bytes == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, pk), r, off, 0, kBufNT);
+    else {
+        static_assert(bytes == 16, "pack of 2, 4, 8 or 16 bytes");
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, pk), r, off, 0, kBufNT);
+    }
+}
+
 // A field value as the storage type holds it.  Blocked levels keep their
 // intermediate fields in registers; with FP16 storage a level's output is
 // rounded where the next unblocked sweep would read it back from memory
@@ -1054,6 +1077,444 @@ fused_pml_sweep(SweepArgs<S, C, JT> a) {
     const int tile_y = t % a.g.ntile_y;
     const int chunk = t / a.g.ntile_y;
     fused_tile<S, C, JT, V, STEADY, true, AUTO>(a, tile_y, chunk, reinterpret_cast<C*>(lds_raw));
+}
+
+// ---------------------------------------------------------------------------
+// Two-phase PML step: a PML grid's step as two in-place z-marching sweeps
+// over full-row tiles, the E phase (E and the E splits from B) and the B
+// phase (B and the B splits from E).  A phase reads only the field it does
+// not write, so it needs no halo rows, recomputes nothing and updates in
+// place; each keeps the per-lane state of one field only, which the fused
+// PML sweep (both fields, their splits and a halo row) has no room for.
+// The state between steps is the fused sweep's: (E_n, B after the next
+// step's first B update), so both forms give the same values (with FP16
+// storage the B phase reads E_{n+1} as stored, rounded, where the fused
+// sweep uses it before rounding).
+//   E phase, plane k: main cells E += curl(B) + J (kokkos_functors.h:64-90);
+//     layer cells E splits decay and take one curl term each, E = their sum
+//     (FDTD_PML.cpp:113-130).  B neighbours at i-1, j-1 from LDS rows, at
+//     k-1 from the previous plane's registers.
+//   B phase, plane k (marching down): main cells (B + d) + d, d the half-step
+//     increment from E_{n+1} (the last half step of step n and the first of
+//     step n+1), or B + d in a run's final phase; layer cells the full-step
+//     split update (FDTD_PML.cpp:182-199), untouched in the final phase.  E
+//     neighbours at i+1, j+1 from LDS rows, at k+1 from the previous plane.
+// Per cell: main 18 words (E phase 9, B phase 9), layer 36 (18 + 18): the
+// fused sweep's 12 and 36, with 6 more for main cells.
+
+template <typename S, typename C, typename JT>
+struct PmlPhaseArgs {
+    Fields<S> f;                 // updated in place
+    const JT* j[3];              // dense J or null (E phase)
+    const PointSource* src;      // point sources sorted by plane (E phase)
+    const double* vals;
+    const int* kstart;           // sources of plane k: [kstart[k], kstart[k+1])
+    int nsrc;
+    int pad;
+    Coefs<C> c;
+    SweepGeom g;                 // TY rows per tile (no halo row)
+    Pml<S, C> pml;               // es_in == es_out
+};
+
+__host__ __device__ constexpr int pml_phase_max_threads(int V, int csize) {
+    return V * csize >= 16 ? 512 : 1024;
+}
+
+// LDS of a phase: three neighbour components of TY + 1 rows and three
+// x-table rows.
+__host__ __device__ inline size_t pml_phase_lds_bytes(int TY, int Ni, int csize) {
+    return static_cast<size_t>(3 * TY + 5) * Ni * csize;
+}
+
+template <typename S, typename C, typename JT, int V>
+__global__ void __launch_bounds__(pml_phase_max_threads(V, sizeof(C)))
+pml_e_phase(PmlPhaseArgs<S, C, JT> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int blk = blockIdx.x;
+    const int t = (blk & 7) * a.g.per_xcd + (blk >> 3);
+    if (t >= a.g.ntiles) return;  // whole workgroup
+    const int Ni = a.g.Ni, Nj = a.g.Nj, Nk = a.g.Nk, TY = a.g.TY;
+    const int tile_y = t % a.g.ntile_y;
+    const int chunk = t / a.g.ntile_y;
+    const int lanes = Ni / V;
+    const int j0 = tile_y * TY;
+    const int ty = min(TY, Nj - j0);
+    const int k0 = chunk * a.g.kchunk;
+    const int k1 = min(k0 + a.g.kchunk, Nk);
+    const int tid = threadIdx.x;
+    const int r = tid / lanes;
+    const Pml<S, C>& P = a.pml;
+    const int ib = pml_pack(tid - r * lanes, lanes, P.pi, V) * V;
+    const bool own = r < ty;
+    const bool top = r == 0;                       // also loads row j0-1
+    const int j = own ? j0 + r : j0;
+    const int jt = (j0 == 0) ? Nj - 1 : j0 - 1;
+    const int plane = Ni * Nj;
+    const int im0 = (ib == 0) ? Ni - 1 : ib - 1;
+    const int row_off = j * Ni + ib;
+    const int top_off = jt * Ni + ib;
+    const bool jmain = in_main(j, Nj, P.pj);
+    const bool xmain = ib >= P.pi && ib + V <= Ni - P.pi;   // every cell main in x
+    const bool xlayer = ib + V <= P.pi || ib >= Ni - P.pi;  // every cell layer in x
+
+    C* const lds = reinterpret_cast<C*>(lds_raw);
+    C* const sBz = lds;                  // rows q = 0..TY: y = j0 - 1 + q
+    C* const sBx = sBz + (TY + 1) * Ni;  // rows q = 0..TY: y = j0 - 1 + q
+    C* const sBy = sBx + (TY + 1) * Ni;  // rows q = 0..TY-1: y = j0 + q
+    C* const sTx = sBy + TY * Ni;        // x tables: a, Eyx<-Bz, Ezx<-By
+    for (int q = tid; q < 3 * Ni; q += blockDim.x) sTx[q] = P.tx[q];
+    const C ay = P.ty[j], cy_xy = P.ty[Nj + j], cy_zy = P.ty[2 * Nj + j];
+    const Coefs<C> c = a.c;
+
+    using PK = RawPack<S, V>;
+    PK bq[3], tq[2], eq[3], sq[6];
+    // Plane k's loads, issued on every lane through plane buffer resources
+    // (a lane that needs a load has its byte offset, the others kBufSkip,
+    // which reads 0 without a memory access): B (own rows), Bz and Bx of
+    // row j0-1 (top row), E (lanes holding main cells), E splits (lanes
+    // holding layer cells).  With no branch around them the compiler counts
+    // them, so the next plane's loads stay in flight across the plane's
+    // work (conditional loads made it wait for all of them).
+    const unsigned pbytes = static_cast<unsigned>(plane * sizeof(S));
+    const unsigned own_b = own ? static_cast<unsigned>(row_off * sizeof(S)) : kBufSkip;
+    const unsigned top_b = (own && top) ? static_cast<unsigned>(top_off * sizeof(S)) : kBufSkip;
+    auto rs = [&](const S* base, int k) { return plane_rsrc(base + static_cast<size_t>(k) * plane, pbytes); };
+    auto load = [&](int k) {
+        const bool kmain = jmain && in_main(k, Nk, P.pk);
+        const unsigned e_b = (kmain && !xlayer) ? own_b : kBufSkip;
+        const unsigned s_b = !(kmain && xmain) ? own_b : kBufSkip;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bq[d] = buf_load_raw<S, V>(rs(a.f.b[d], k), own_b);
+        tq[0] = buf_load_raw<S, V>(rs(a.f.b[2], k), top_b);
+        tq[1] = buf_load_raw<S, V>(rs(a.f.b[0], k), top_b);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) eq[d] = buf_load_raw<S, V>(rs(a.f.e[d], k), e_b);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) sq[q] = buf_load_raw<S, V>(rs(P.es_in[q], k), s_b);
+    };
+
+    C bP[2][V];  // Bx, By at plane k-1 (own cells)
+    if (own) {
+        const int off = ((k0 == 0) ? Nk - 1 : k0 - 1) * plane + row_off;
+        load_pack<S, C, V>(a.f.b[0] + off, bP[0]);
+        load_pack<S, C, V>(a.f.b[1] + off, bP[1]);
+    }
+    load(k0);
+    wait_vmem_all();  // the loop's waits then count its own stores only
+    // the z tables through the scalar cache: a vector load inside the layer
+    // branch would make the branch wait for the next plane's loads as well
+    const cptr<C> tz = as_const(P.tz);
+    for (int kk = k0; kk < k1; ++kk) {
+        const bool kmain = jmain && in_main(kk, Nk, P.pk);
+        const bool many = kmain && !xlayer, mall = kmain && xmain;
+        const int off = kk * plane + row_off;
+        const C az = tz[kk], cz_xz = tz[Nk + kk], cz_yz = tz[2 * Nk + kk];
+        C bn[3][V], en[3][V], hs[6][V];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            widen<S, C, V>(bq[d], bn[d]);
+            widen<S, C, V>(eq[d], en[d]);
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) widen<S, C, V>(sq[q], hs[q]);
+        if (own) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                sBz[(r + 1) * Ni + ib + v] = bn[2][v];
+                sBx[(r + 1) * Ni + ib + v] = bn[0][v];
+                sBy[r * Ni + ib + v] = bn[1][v];
+            }
+            if (top) {
+                C bt[2][V];
+                widen<S, C, V>(tq[0], bt[0]);
+                widen<S, C, V>(tq[1], bt[1]);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    sBz[ib + v] = bt[0][v];
+                    sBx[ib + v] = bt[1][v];
+                }
+            }
+        }
+        if (kk + 1 < k1) load(kk + 1);
+        __syncthreads();
+        C e1[3][V];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < V; ++v) e1[d][v] = en[d][v];
+        if (own) {
+            if (many) {
+                C t[3][V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const C by_im = (v > 0) ? bn[1][v - 1] : sBy[r * Ni + im0];
+                    t[0][v] = c.ex_bz * (bn[2][v] - sBz[r * Ni + ib + v]);
+                    t[1][v] = c.ey_bx * (bn[0][v] - bP[0][v]);
+                    t[2][v] = c.ez_by * (bn[1][v] - by_im);
+                }
+                if (a.j[0] != nullptr) {
+                    C jv[V];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        load_pack<JT, C, V>(a.j[d] + off, jv);
+                        const C cj = d == 0 ? c.jx : (d == 1 ? c.jy : c.jz);
+#pragma unroll
+                        for (int v = 0; v < V; ++v) t[d][v] = cj * jv[v] + t[d][v];
+                    }
+                }
+                if (a.nsrc > 0) {
+                    const cptr<int> ks = as_const(a.kstart);
+                    add_point_sources<C, V>(t, c, a.src, a.vals, ks[kk], ks[kk + 1], ib, j, kk);
+                }
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const C bz_im = (v > 0) ? bn[2][v - 1] : sBz[(r + 1) * Ni + im0];
+                    e1[0][v] = en[0][v] + (t[0][v] - c.ex_by * (bn[1][v] - bP[1][v]));
+                    e1[1][v] = en[1][v] + (t[1][v] - c.ey_bz * (bn[2][v] - bz_im));
+                    e1[2][v] = en[2][v] + (t[2][v] - c.ez_bx * (bn[0][v] - sBx[r * Ni + ib + v]));
+                }
+            }
+            if (!mall) {
+                C (&sxy)[V] = hs[kExy];
+                C (&sxz)[V] = hs[kExz];
+                C (&syx)[V] = hs[kEyx];
+                C (&syz)[V] = hs[kEyz];
+                C (&szx)[V] = hs[kEzx];
+                C (&szy)[V] = hs[kEzy];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const int i = ib + v;
+                    if (kmain && in_main(i, Ni, P.pi)) continue;
+                    const C ax = sTx[i], cx_yx = sTx[Ni + i], cx_zx = sTx[2 * Ni + i];
+                    const C by_im = (v > 0) ? bn[1][v - 1] : sBy[r * Ni + im0];
+                    const C bz_im = (v > 0) ? bn[2][v - 1] : sBz[(r + 1) * Ni + im0];
+                    sxy[v] = stored<S, C>(sxy[v] * ay + cy_xy * (bn[2][v] - sBz[r * Ni + ib + v]));
+                    szy[v] = stored<S, C>(szy[v] * ay - cy_zy * (bn[0][v] - sBx[r * Ni + ib + v]));
+                    sxz[v] = stored<S, C>(sxz[v] * az - cz_xz * (bn[1][v] - bP[1][v]));
+                    szx[v] = stored<S, C>(szx[v] * ax + cx_zx * (bn[1][v] - by_im));
+                    syx[v] = stored<S, C>(syx[v] * ax - cx_yx * (bn[2][v] - bz_im));
+                    syz[v] = stored<S, C>(syz[v] * az + cz_yz * (bn[0][v] - bP[0][v]));
+                    e1[0][v] = sxz[v] + sxy[v];
+                    e1[1][v] = syx[v] + syz[v];
+                    e1[2][v] = szy[v] + szx[v];
+                }
+            }
+        }
+        // Stores on every lane (skipped where the offset is kBufSkip), after
+        // a wait for the next plane's loads (issued before the barrier, so
+        // mostly complete): the wait at the top of the next iteration then
+        // does not also wait for these stores.
+        wait_vmem_all();
+        {
+            const unsigned s_b = !mall ? own_b : kBufSkip;
+#pragma unroll
+            for (int q = 0; q < 6; ++q) buf_store_pack<S, C, V>(hs[q], rs(P.es_out[q], kk), s_b);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) buf_store_pack<S, C, V>(e1[d], rs(a.f.e[d], kk), own_b);
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            bP[0][v] = bn[0][v];
+            bP[1][v] = bn[1][v];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename S, typename C, typename JT, int V, bool STEADY>
+__global__ void __launch_bounds__(pml_phase_max_threads(V, sizeof(C)))
+pml_b_phase(PmlPhaseArgs<S, C, JT> a) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const int blk = blockIdx.x;
+    const int t = (blk & 7) * a.g.per_xcd + (blk >> 3);
+    if (t >= a.g.ntiles) return;  // whole workgroup
+    const int Ni = a.g.Ni, Nj = a.g.Nj, Nk = a.g.Nk, TY = a.g.TY;
+    const int tile_y = t % a.g.ntile_y;
+    const int chunk = t / a.g.ntile_y;
+    const int lanes = Ni / V;
+    const int j0 = tile_y * TY;
+    const int ty = min(TY, Nj - j0);
+    const int k0 = chunk * a.g.kchunk;
+    const int k1 = min(k0 + a.g.kchunk, Nk);
+    const int tid = threadIdx.x;
+    const int r = tid / lanes;
+    const Pml<S, C>& P = a.pml;
+    const int ib = pml_pack(tid - r * lanes, lanes, P.pi, V) * V;
+    const bool own = r < ty;
+    const bool bottom = r == ty - 1;               // also loads row j0+ty
+    const int j = own ? j0 + r : j0;
+    const int jb = (j0 + ty == Nj) ? 0 : j0 + ty;
+    const int plane = Ni * Nj;
+    const int ipl = (ib + V == Ni) ? 0 : ib + V;
+    const int row_off = j * Ni + ib;
+    const int bot_off = jb * Ni + ib;
+    const bool jmain = in_main(j, Nj, P.pj);
+    const bool xmain = ib >= P.pi && ib + V <= Ni - P.pi;
+    const bool xlayer = ib + V <= P.pi || ib >= Ni - P.pi;
+
+    C* const lds = reinterpret_cast<C*>(lds_raw);
+    C* const sEz = lds;                  // rows q = 0..TY: y = j0 + q (row ty: j0+ty, wrapped)
+    C* const sEx = sEz + (TY + 1) * Ni;  // rows q = 0..TY
+    C* const sEy = sEx + (TY + 1) * Ni;  // rows q = 0..TY-1
+    C* const sTx = sEy + TY * Ni;        // x tables: a, Byx<-Ez, Bzx<-Ey
+    for (int q = tid; q < Ni; q += blockDim.x) {
+        sTx[q] = P.tx[q];
+        sTx[Ni + q] = P.tx[3 * Ni + q];
+        sTx[2 * Ni + q] = P.tx[4 * Ni + q];
+    }
+    const C ay = P.ty[j], cy_xy = P.ty[3 * Nj + j], cy_zy = P.ty[4 * Nj + j];
+    const Coefs<C> c = a.c;
+
+    using PK = RawPack<S, V>;
+    PK eq[3], uq[2], bq[3], sq[6];
+    // Plane k's loads, on every lane through plane buffer resources as in
+    // the E phase: E (own rows), Ex and Ez of row j0+ty (bottom row), B
+    // (lanes holding main cells), B splits (steady phases, lanes holding
+    // layer cells).
+    const unsigned pbytes = static_cast<unsigned>(plane * sizeof(S));
+    const unsigned own_b = own ? static_cast<unsigned>(row_off * sizeof(S)) : kBufSkip;
+    const unsigned bot_b = (own && bottom) ? static_cast<unsigned>(bot_off * sizeof(S)) : kBufSkip;
+    auto rs = [&](const S* base, int k) { return plane_rsrc(base + static_cast<size_t>(k) * plane, pbytes); };
+    auto load = [&](int k) {
+        const bool kmain = jmain && in_main(k, Nk, P.pk);
+        const unsigned b_b = (kmain && !xlayer) ? own_b : kBufSkip;
+        const unsigned s_b = (STEADY && !(kmain && xmain)) ? own_b : kBufSkip;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) eq[d] = buf_load_raw<S, V>(rs(a.f.e[d], k), own_b);
+        uq[0] = buf_load_raw<S, V>(rs(a.f.e[0], k), bot_b);
+        uq[1] = buf_load_raw<S, V>(rs(a.f.e[2], k), bot_b);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bq[d] = buf_load_raw<S, V>(rs(a.f.b[d], k), b_b);
+        if constexpr (STEADY) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) sq[q] = buf_load_raw<S, V>(rs(P.bs[q], k), s_b);
+        }
+    };
+
+    C eN[2][V];  // Ex, Ey at plane k+1 (own cells)
+    if (own) {
+        const int off = ((k1 == Nk) ? 0 : k1) * plane + row_off;
+        load_pack<S, C, V>(a.f.e[0] + off, eN[0]);
+        load_pack<S, C, V>(a.f.e[1] + off, eN[1]);
+    }
+    load(k1 - 1);
+    wait_vmem_all();  // the loop's waits then count its own stores only
+    const cptr<C> tz = as_const(P.tz);  // scalar loads, as in the E phase
+    for (int kk = k1 - 1; kk >= k0; --kk) {
+        const bool kmain = jmain && in_main(kk, Nk, P.pk);
+        const bool many = kmain && !xlayer, mall = kmain && xmain;
+        const C az = tz[kk], cz_xz = tz[3 * Nk + kk], cz_yz = tz[4 * Nk + kk];
+        C en[3][V], bn[3][V], hb[6][V];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            widen<S, C, V>(eq[d], en[d]);
+            widen<S, C, V>(bq[d], bn[d]);
+        }
+        if constexpr (STEADY) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) widen<S, C, V>(sq[q], hb[q]);
+        }
+        if (own) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                sEz[r * Ni + ib + v] = en[2][v];
+                sEx[r * Ni + ib + v] = en[0][v];
+                sEy[r * Ni + ib + v] = en[1][v];
+            }
+            if (bottom) {
+                C eu[2][V];
+                widen<S, C, V>(uq[0], eu[0]);
+                widen<S, C, V>(uq[1], eu[1]);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    sEx[ty * Ni + ib + v] = eu[0][v];
+                    sEz[ty * Ni + ib + v] = eu[1][v];
+                }
+            }
+        }
+        if (kk > k0) load(kk - 1);
+        __syncthreads();
+        // nb: the B to store (main cells: the half step(s); layer cells:
+        // the split sums, or unchanged in a final phase)
+        C nb[3][V];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int v = 0; v < V; ++v) nb[d][v] = bn[d][v];
+        if (own) {
+            if (many) {
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    if (!in_main(ib + v, Ni, P.pi)) continue;
+                    const C ey_ip = (v < V - 1) ? en[1][v + 1] : sEy[r * Ni + ipl];
+                    const C ez_ip = (v < V - 1) ? en[2][v + 1] : sEz[r * Ni + ipl];
+                    const C ex_jp = sEx[(r + 1) * Ni + ib + v];
+                    const C ez_jp = sEz[(r + 1) * Ni + ib + v];
+                    const C dbx = c.bx_ey * (eN[1][v] - en[1][v]) - c.bx_ez * (ez_jp - en[2][v]);
+                    const C dby = c.by_ez * (ez_ip - en[2][v]) - c.by_ex * (eN[0][v] - en[0][v]);
+                    const C dbz = c.bz_ex * (ex_jp - en[0][v]) - c.bz_ey * (ey_ip - en[1][v]);
+                    nb[0][v] = bn[0][v] + dbx;
+                    nb[1][v] = bn[1][v] + dby;
+                    nb[2][v] = bn[2][v] + dbz;
+                    if (STEADY) {
+                        nb[0][v] = nb[0][v] + dbx;
+                        nb[1][v] = nb[1][v] + dby;
+                        nb[2][v] = nb[2][v] + dbz;
+                    }
+                }
+            }
+            if constexpr (STEADY) {
+                if (!mall) {
+                    C (&sxy)[V] = hb[kBxy];
+                    C (&sxz)[V] = hb[kBxz];
+                    C (&syx)[V] = hb[kByx];
+                    C (&syz)[V] = hb[kByz];
+                    C (&szx)[V] = hb[kBzx];
+                    C (&szy)[V] = hb[kBzy];
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        const int i = ib + v;
+                        if (kmain && in_main(i, Ni, P.pi)) continue;
+                        const C ax = sTx[i], cx_yx = sTx[Ni + i], cx_zx = sTx[2 * Ni + i];
+                        const C ey_ip = (v < V - 1) ? en[1][v + 1] : sEy[r * Ni + ipl];
+                        const C ez_ip = (v < V - 1) ? en[2][v + 1] : sEz[r * Ni + ipl];
+                        const C ex_jp = sEx[(r + 1) * Ni + ib + v];
+                        const C ez_jp = sEz[(r + 1) * Ni + ib + v];
+                        syx[v] = stored<S, C>(syx[v] * ax + cx_yx * (ez_ip - en[2][v]));
+                        szx[v] = stored<S, C>(szx[v] * ax - cx_zx * (ey_ip - en[1][v]));
+                        sxy[v] = stored<S, C>(sxy[v] * ay - cy_xy * (ez_jp - en[2][v]));
+                        szy[v] = stored<S, C>(szy[v] * ay + cy_zy * (ex_jp - en[0][v]));
+                        sxz[v] = stored<S, C>(sxz[v] * az + cz_xz * (eN[1][v] - en[1][v]));
+                        syz[v] = stored<S, C>(syz[v] * az - cz_yz * (eN[0][v] - en[0][v]));
+                        nb[0][v] = sxy[v] + sxz[v];
+                        nb[1][v] = syz[v] + syx[v];
+                        nb[2][v] = szx[v] + szy[v];
+                    }
+                }
+            }
+        }
+        // Stores on every lane (skipped where the offset is kBufSkip): B of
+        // lanes with main cells, and in steady phases of lanes with layer
+        // cells too, with their splits; after a wait for the next plane's
+        // loads, as in the E phase.
+        wait_vmem_all();
+        {
+            const unsigned b_b = (many || (STEADY && !mall)) ? own_b : kBufSkip;
+            if constexpr (STEADY) {
+                const unsigned s_b = !mall ? own_b : kBufSkip;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) buf_store_pack<S, C, V>(hb[q], rs(P.bs[q], kk), s_b);
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) buf_store_pack<S, C, V>(nb[d], rs(a.f.b[d], kk), b_b);
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            eN[0][v] = en[0][v];
+            eN[1][v] = en[1][v];
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2426,6 +2887,26 @@ checksum_partial(const S* __restrict__ x, size_t n, unsigned long long* __restri
     for (size_t i = blockIdx.x * static_cast<size_t>(kReduceThreads) + threadIdx.x; i < n;
          i += static_cast<size_t>(gridDim.x) * kReduceThreads)
         acc += mix64(bits_of(x[i]) ^ (static_cast<unsigned long long>(i) * 0xD6E8FEB86659FD93ull));
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// Per-block counts of non-finite values (NaN, +-Inf) of one array.
+template <typename S>
+__global__ void __launch_bounds__(kReduceThreads)
+nonfinite_partial(const S* __restrict__ x, size_t n, unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long sh[kReduceThreads];
+    unsigned long long acc = 0;
+    for (size_t i = blockIdx.x * static_cast<size_t>(kReduceThreads) + threadIdx.x; i < n;
+         i += static_cast<size_t>(gridDim.x) * kReduceThreads)
+    {
+        bool finite;
+        if constexpr (sizeof(S) == 2)
+            finite = __builtin_isfinite(static_cast<float>(x[i]));  // exact widening
+        else
+            finite = __builtin_isfinite(x[i]);
+        acc += finite ? 0u : 1u;
+    }
     acc = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }

@@ -573,6 +573,7 @@ int gather_probes_s(fdtd_grid* g) {
 
 int gather_probes(fdtd_grid* g) {
     if (!probing(g)) return FDTD_OK;
+    TraceRange tr("probe_gather");
     switch (g->precision) {
         case FDTD_FP64: return gather_probes_s<double>(g);
         case FDTD_FP32: return gather_probes_s<float>(g);
@@ -658,6 +659,86 @@ int launch_fused(fdtd_grid* g, const Tuning& t, bool steady, int slot, int in, i
         default:
             return steady ? launch_fused_s<_Float16, true>(g, t, slot, in, out, x)
                           : launch_fused_s<_Float16, false>(g, t, slot, in, out, x);
+    }
+}
+
+// ---- two-phase PML steps --------------------------------------------------------
+
+// Default shape of the phase sweeps: 16-byte lanes (FP64 V=2, else V=4; a
+// smaller V when it does not divide Ni), as many full rows as the
+// workgroup bound allows, and z chunks giving about 1024 tiles.
+Tuning pml_phase_default(const fdtd_grid* g) {
+    const int cs = (int)compute_size(g->precision);
+    Tuning t;
+    t.V = cs == 8 ? 2 : 4;
+    while (t.V > 1 && g->Ni % t.V != 0) t.V /= 2;
+    const int lanes = g->Ni / t.V;
+    t.TY = std::max(1, std::min(g->Nj, fdtd::pml_phase_max_threads(t.V, cs) / std::max(1, lanes)));
+    const int ntile_y = (g->Nj + t.TY - 1) / t.TY;
+    const int nchunk = std::max(1, std::min(g->Nk / 8, (1024 + ntile_y - 1) / ntile_y));
+    t.kchunk = (g->Nk + nchunk - 1) / nchunk;
+    return t;
+}
+
+bool pml_phase_valid(const fdtd_grid* g, const Tuning& t) {
+    const int cs = (int)compute_size(g->precision);
+    if (t.V != 1 && t.V != 2 && t.V != 4) return false;
+    if (t.V * g->esize > 16) return false;  // packs of at most 16 bytes
+    if (g->Ni % t.V != 0 || t.TY < 1 || t.kchunk < 1) return false;
+    if ((long long)t.TY * (g->Ni / t.V) > fdtd::pml_phase_max_threads(t.V, cs)) return false;
+    return fdtd::pml_phase_lds_bytes(t.TY, g->Ni, cs) <= kLdsLimit;
+}
+
+// One step of a PML grid as the E phase and the B phase, in place on the
+// current buffers (steady: B takes both half steps and the layer its split
+// update; else the run's final B half step).
+template <typename S, int V>
+int launch_pml_phases_v(fdtd_grid* g, int slot, bool steady) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    using JT = C;
+    const Tuning& t = g->ptune;
+    fdtd::PmlPhaseArgs<S, C, JT> a;
+    a.f = fields_of<S>(g, g->cur);
+    for (int d = 0; d < 3; ++d)
+        a.j[d] = g->j_dense_active ? static_cast<const JT*>(g->J[d]) : nullptr;
+    a.src = g->d_src;
+    a.vals = g->d_vals ? g->d_vals + (size_t)slot * g->nsrc : nullptr;
+    a.kstart = g->d_kstart;
+    a.nsrc = g->nsrc;
+    a.pad = 0;
+    a.c = make_coefs<C>(g);
+    a.g = make_geom(g, t);
+    a.pml = make_pml<S, C>(g, g->cur, g->cur);
+    auto ke = fdtd::pml_e_phase<S, C, JT, V>;
+    auto kb = steady ? fdtd::pml_b_phase<S, C, JT, V, true> : fdtd::pml_b_phase<S, C, JT, V, false>;
+    const size_t lds = fdtd::pml_phase_lds_bytes(t.TY, g->Ni, sizeof(C));
+    if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(ke), lds)) return rc;
+    if (int rc = allow_dynamic_lds(reinterpret_cast<const void*>(kb), lds)) return rc;
+    const int threads = t.TY * (g->Ni / V);
+    hipLaunchKernelGGL(ke, dim3(8 * a.g.per_xcd), dim3(threads), lds, g->stream, a);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(kb, dim3(8 * a.g.per_xcd), dim3(threads), lds, g->stream, a);
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+template <typename S>
+int launch_pml_phases_s(fdtd_grid* g, int slot, bool steady) {
+    switch (g->ptune.V) {
+        case 1: return launch_pml_phases_v<S, 1>(g, slot, steady);
+        case 2: return launch_pml_phases_v<S, 2>(g, slot, steady);
+        case 4:
+            if constexpr (sizeof(S) <= 4) return launch_pml_phases_v<S, 4>(g, slot, steady);
+            [[fallthrough]];
+        default: return fail(FDTD_ERR_UNSUPPORTED, "unsupported V=%d", g->ptune.V);
+    }
+}
+
+int launch_pml_phases(fdtd_grid* g, int slot, bool steady) {
+    switch (g->precision) {
+        case FDTD_FP64: return launch_pml_phases_s<double>(g, slot, steady);
+        case FDTD_FP32: return launch_pml_phases_s<float>(g, slot, steady);
+        default: return launch_pml_phases_s<_Float16>(g, slot, steady);
     }
 }
 
@@ -793,6 +874,78 @@ int strip_occupancy(const fdtd_grid* g, const BlockTuning& t) {
         case FDTD_FP32: return strip_occupancy_s<float>(t);
         default: return strip_occupancy_s<_Float16>(t);
     }
+}
+
+// ---- autotuner cache ----------------------------------------------------------
+// fdtd_autotune_blocking's picks, kept across processes in a text file (the
+// path of fdtd_set_tune_cache, else $FDTD_MI355X_TUNE_CACHE; none: off).
+// One line per pick, "<key> | L V lanes TYg kchunk R ms", the key naming
+// the cache format, shape, precision, device architecture and CU count;
+// the last line of a key wins, and a pick that no longer validates is
+// timed again.
+constexpr int kTuneCacheVersion = 1;
+std::mutex g_tune_mu;
+std::string g_tune_path;
+bool g_tune_path_set = false;
+
+std::string tune_cache_path() {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    if (!g_tune_path_set) {
+        const char* e = std::getenv("FDTD_MI355X_TUNE_CACHE");
+        g_tune_path = e ? e : "";
+        g_tune_path_set = true;
+    }
+    return g_tune_path;
+}
+
+int device_cus();
+
+std::string tune_key(const fdtd_grid* g) {
+    hipDeviceProp_t prop;
+    std::string arch = "unknown";
+    if (hipGetDeviceProperties(&prop, g->device) == hipSuccess) arch = prop.gcnArchName;
+    for (char& c : arch)
+        if (c == ' ' || c == '|') c = '_';
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "v%d %d %d %d p%d %s cu%d", kTuneCacheVersion, g->Ni, g->Nj,
+                  g->Nk, g->precision, arch.c_str(), device_cus());
+    return buf;
+}
+
+bool tune_cache_lookup(const std::string& path, const std::string& key, BlockTuning* t,
+                       float* ms) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    FILE* f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    bool found = false;
+    char line[512];
+    while (std::fgets(line, sizeof line, f)) {
+        char* bar = std::strchr(line, '|');
+        if (!bar) continue;
+        std::string k(line, bar - line);
+        while (!k.empty() && k.back() == ' ') k.pop_back();
+        if (k != key) continue;
+        BlockTuning c;
+        float m = 0;
+        if (std::sscanf(bar + 1, "%d %d %d %d %d %d %f", &c.L, &c.V, &c.lanes, &c.TYg, &c.kchunk,
+                        &c.R, &m) == 7) {
+            *t = c;
+            *ms = m;
+            found = true;
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+void tune_cache_store(const std::string& path, const std::string& key, const BlockTuning& t,
+                      float ms) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    FILE* f = std::fopen(path.c_str(), "a");
+    if (!f) return;  // a cache that cannot be written only costs the next run a retune
+    std::fprintf(f, "%s | %d %d %d %d %d %d %.6f\n", key.c_str(), t.L, t.V, t.lanes, t.TYg,
+                 t.kchunk, t.R, ms);
+    std::fclose(f);
 }
 
 int device_cus() {
@@ -948,6 +1101,7 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
     g->ba_cur = -1;
     int rc = FDTD_OK;
     if (carried < 0) {
+        TraceRange tr("b_half_prologue");
         hipEvent_t t0 = trace_begin(g);
         rc = plain_B(g);
         trace_end(g, t0, 0, false);
@@ -984,6 +1138,7 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
         const bool last = rem == 0 && p + 1 == npass;
         void* const* in_b = p == 0 && carried >= 0 ? g->BA[carried] : nullptr;
         void* const* ahead = last && carry_out ? g->BA[ahead_buf] : nullptr;
+        TraceRange tr(strips ? "strip_pass" : "patch_pass");
         hipEvent_t t0 = trace_begin(g);
         rc = launch_blocked(g, last, s % chunk, in, out, in_b, ahead);
         trace_end(g, t0, last ? 2 : 1, true);
@@ -997,6 +1152,7 @@ int run_blocked(fdtd_grid* g, int nsteps, hipEvent_t mid) {
         const int in = g->cur, out = 1 - g->cur;
         SweepExtra x;
         if (q + 1 == rem && carry_out) x.ahead = g->BA[ahead_buf];
+        TraceRange tr("fused_sweep");
         hipEvent_t t0 = trace_begin(g);
         rc = launch_fused(g, g->tune, q + 1 < rem, 0, in, out, x);
         trace_end(g, t0, q + 1 < rem ? 3 : 2, false);
@@ -1031,6 +1187,7 @@ int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
     const int carried = g->ba_cur;
     int rc = FDTD_OK;
     if (carried < 0) {
+        TraceRange tr("b_half_prologue");
         hipEvent_t t0 = trace_begin(g);
         rc = plain_B(g);
         trace_end(g, t0, 0, false);
@@ -1062,12 +1219,22 @@ int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
             if ((rc = zero_fill(g->d_max, 32, g->stream))) return rc;
         if (s % kValTable == 0)
             if ((rc = eval_sources(g, g->step + s, std::min(kValTable, nsteps - s)))) return rc;
+        if (g->pml && g->pml_path == 1 && !decide && !track) {
+            // the step as two in-place phase sweeps
+            TraceRange tr("pml_phases");
+            hipEvent_t t0 = trace_begin(g);
+            rc = launch_pml_phases(g, s % kValTable, s + 1 < nsteps);
+            trace_end(g, t0, s + 1 < nsteps ? 1 : 2, false);
+            if (rc) return rc;
+            continue;
+        }
         const int in = g->cur, out = 1 - g->cur;
         SweepExtra x;
         x.oscale = decide ? f : nullptr;
         x.maxbits = track ? g->d_max : nullptr;
         if (s == 0 && carried >= 0) x.in_b = g->BA[carried];
         if (s + 1 == nsteps && carry_out) x.ahead = g->BA[ahead_buf];
+        TraceRange tr("fused_sweep");
         hipEvent_t t0 = trace_begin(g);
         rc = launch_fused(g, g->tune, s + 1 < nsteps, s % kValTable, in, out, x);
         trace_end(g, t0, s + 1 < nsteps ? 1 : 2, false);
@@ -1091,6 +1258,7 @@ int run_fused(fdtd_grid* g, int nsteps, hipEvent_t mid = nullptr) {
 int run_3sweep(fdtd_grid* g, int nsteps) {
     g->ba_cur = -1;
     for (int s = 0; s < nsteps; ++s) {
+        TraceRange tr("three_sweep_step");
         int rc = plain_B(g, true);
         if (!rc) rc = plain_E(g, g->step);
         if (!rc) rc = plain_B(g, false);
@@ -1107,6 +1275,7 @@ int run_3sweep(fdtd_grid* g, int nsteps) {
 int record_energy(fdtd_grid* g) {
     const long long row = static_cast<long long>(g->energy_steps.size());
     if (row >= g->energy_cap) return fail(FDTD_ERR_INVALID_ARGUMENT, "energy records full");
+    TraceRange tr("energy_record");
     const int nb = std::min<int>(kReduceBlocks, grid_blocks(g->N));
     for (int c = 0; c < 6; ++c) {
         const void* p = field_ptr(g, c);
@@ -1172,6 +1341,7 @@ int run_steps(fdtd_grid* g, int nsteps, bool three_sweep) {
         return run_fused(g, n);
     };
     if (int rc = check_record_capacity(g, nsteps)) return rc;
+    TraceRange tr(three_sweep ? "update_fields_3sweep" : "update_fields");
     g->recording = true;
     int rc = FDTD_OK;
     if (g->energy_every <= 0) {
@@ -1395,8 +1565,10 @@ int create_grid(const fdtd_parameters* p, double dt, double pml_percent, int pre
         if (!rc) rc = zero_fill(g->B[0][d], bytes, g->stream);
         if (rc) return cleanup(rc);
     }
-    if (g->pml)
+    if (g->pml) {
         if (int rc = alloc_pml(g)) return cleanup(rc);
+        g->ptune = pml_phase_default(g);
+    }
     if (hipMalloc(&g->d_part, sizeof(double) * 12 * kReduceBlocks) != hipSuccess ||
         hipMalloc(&g->d_out, sizeof(double) * 16) != hipSuccess ||
         hipMalloc(&g->d_max, 32) != hipSuccess)
@@ -2067,6 +2239,36 @@ int fdtd_checksum(fdtd_grid* g, unsigned long long out[6]) {
     return FDTD_OK;
 }
 
+int fdtd_count_nonfinite(fdtd_grid* g, long long out[6]) {
+    if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    const int nb = std::min<int>(kReduceBlocks, grid_blocks(g->N));
+    auto* part = reinterpret_cast<unsigned long long*>(g->d_part);
+    auto* res = reinterpret_cast<unsigned long long*>(g->d_out);
+    for (int c = 0; c < 6; ++c) {
+        const void* p = field_ptr(g, c);
+        switch (g->precision) {
+            case FDTD_FP64:
+                hipLaunchKernelGGL(fdtd::nonfinite_partial<double>, dim3(nb), dim3(256), 0,
+                                   g->stream, static_cast<const double*>(p), g->N, part);
+                break;
+            case FDTD_FP32:
+                hipLaunchKernelGGL(fdtd::nonfinite_partial<float>, dim3(nb), dim3(256), 0,
+                                   g->stream, static_cast<const float*>(p), g->N, part);
+                break;
+            default:
+                hipLaunchKernelGGL(fdtd::nonfinite_partial<_Float16>, dim3(nb), dim3(256), 0,
+                                   g->stream, static_cast<const _Float16*>(p), g->N, part);
+        }
+        hipLaunchKernelGGL(fdtd::checksum_final, dim3(1), dim3(256), 0, g->stream, part, nb, res + c);
+        HIP_TRY(hipGetLastError());
+    }
+    unsigned long long r[6];
+    HIP_TRY(hipMemcpyAsync(r, res, sizeof r, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    for (int c = 0; c < 6; ++c) out[c] = static_cast<long long>(r[c]);
+    return FDTD_OK;
+}
+
 int fdtd_add_plane_wave(fdtd_grid* g, const int m[3], double amplitude, double phase,
                         const double polarization[3]) {
     if (!g || !m || !polarization) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
@@ -2193,6 +2395,30 @@ int rescale_to_target(fdtd_grid* g, bool current_only) {
     if (int rc = fdtd_max_abs(g, mx)) return rc;
     target_scales(g, mx, g->autoscale_log2, 0.0, s);
     return apply_scales(g, s, current_only);
+}
+
+// Sets a blocking pick read from the autotuner cache; false (and the
+// grid's blocking unchanged) when it does not validate for this grid.
+bool apply_blocking(fdtd_grid* g, const BlockTuning& t) {
+    const BlockTuning keep = g->btune;
+    int rc;
+    if (t.L <= 0) {
+        g->btune = t;  // the full-row sweep, as the timing left it
+        g->btune.L = 0;
+        return true;
+    }
+    if (t.R > 0 && t.TYg % t.R == 0)
+        rc = fdtd_set_time_blocking_strips(g, t.L, t.V, t.R, t.TYg / t.R, t.kchunk);
+    else if (t.R == 0)
+        rc = fdtd_set_time_blocking(g, t.L, t.V, t.lanes, t.TYg, t.kchunk);
+    else
+        rc = FDTD_ERR_INVALID_ARGUMENT;
+    if (rc != FDTD_OK) {
+        g->btune = keep;
+        g_last_error.clear();
+        return false;
+    }
+    return true;
 }
 
 }  // namespace fdtd::rt
@@ -2331,9 +2557,60 @@ int fdtd_set_time_blocking_strips(fdtd_grid* g, int L, int V, int R, int W, int 
     return FDTD_OK;
 }
 
+int fdtd_set_pml_path(fdtd_grid* g, int path, int V, int TY, int kchunk) {
+    if (!g || (path != 0 && path != 1)) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
+    if (!g->pml) return fail(FDTD_ERR_UNSUPPORTED, "the grid has no PML layer");
+    if (path == 0) {
+        g->pml_path = 0;
+        return FDTD_OK;
+    }
+    Tuning t = pml_phase_default(g);
+    if (V > 0) {
+        t.V = V;
+        if (TY <= 0)
+            t.TY = std::max(1, std::min(g->Nj, fdtd::pml_phase_max_threads(V, (int)compute_size(g->precision)) /
+                                                   std::max(1, g->Ni / V)));
+    }
+    if (TY > 0) t.TY = TY;
+    if (kchunk > 0) t.kchunk = kchunk;
+    if (!pml_phase_valid(g, t))
+        return fail(FDTD_ERR_INVALID_ARGUMENT, "PML phase shape V=%d TY=%d kchunk=%d invalid for this grid",
+                    t.V, t.TY, t.kchunk);
+    g->ptune = t;
+    g->pml_path = 1;
+    return FDTD_OK;
+}
+
+int fdtd_get_pml_path(const fdtd_grid* g, int out[4]) {
+    if (!g || !out) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    out[0] = g->pml ? g->pml_path : 0;
+    out[1] = g->ptune.V;
+    out[2] = g->ptune.TY;
+    out[3] = g->ptune.kchunk;
+    return FDTD_OK;
+}
+
+int fdtd_set_tune_cache(const char* path) {
+    std::lock_guard<std::mutex> lock(fdtd::rt::g_tune_mu);
+    fdtd::rt::g_tune_path = path ? path : "";
+    fdtd::rt::g_tune_path_set = true;
+    return FDTD_OK;
+}
+
 int fdtd_autotune_blocking(fdtd_grid* g, int reps, int out[5], float* best_ms) {
     if (!g || reps <= 0) return fail(FDTD_ERR_INVALID_ARGUMENT, "invalid argument");
     if (g->pml) return fail(FDTD_ERR_UNSUPPORTED, "temporal blocking takes periodic grids (no PML layer)");
+    const std::string cache = tune_cache_path();
+    const std::string key = cache.empty() ? std::string() : tune_key(g);
+    {
+        BlockTuning t;
+        float ms = 0;
+        if (!cache.empty() && tune_cache_lookup(cache, key, &t, &ms) && apply_blocking(g, t)) {
+            if (out) fdtd_get_time_blocking(g, out);
+            if (best_ms) *best_ms = ms;
+            return FDTD_OK;
+        }
+    }
     if (int rc = flush_sources(g)) return rc;
     // Candidate passes are timed on the grid's buffers and the state is
     // restored afterwards (on every exit path); the full-row kernel's time
@@ -2397,6 +2674,7 @@ int fdtd_autotune_blocking(fdtd_grid* g, int reps, int out[5], float* best_ms) {
     g->btune = best_t < 1e30f ? best : orig;
     if (int rc = snap.restore()) return rc;
     if (best_t >= 1e30f) return fail(FDTD_ERR_UNSUPPORTED, "no launch shape fits this grid");
+    if (!cache.empty()) tune_cache_store(cache, key, g->btune, best_t);
     if (out) fdtd_get_time_blocking(g, out);
     if (best_ms) *best_ms = best_t;
     return FDTD_OK;

@@ -288,6 +288,13 @@ class FDTD:
         """Probe rows and energy records restart at row 0."""
         _lib.check(self._L.fdtd_rewind_records(self._h))
 
+    def count_nonfinite(self):
+        """Number of NaN/Inf values of Ex..Bz as stored (list of 6)."""
+        self._flush()
+        out = (ctypes.c_longlong * 6)()
+        _lib.check(self._L.fdtd_count_nonfinite(self._h, out))
+        return list(out)
+
     def set_point_sources(self, sources) -> None:
         """sources: iterable of dicts {i,j,k,comp(0..2),kind,amp,period,t0,sigma,phase}."""
         arr = (_lib.PointSource * max(1, len(sources)))()
@@ -468,6 +475,15 @@ class FDTD_PML(FDTD):
         _lib.check(self._L.fdtd_get_pml(self._h, w, ctypes.byref(pct)))
         return tuple(w)
 
+    def set_pml_path(self, path: int, V: int = 0, TY: int = 0, kchunk: int = 0) -> None:
+        """1: two in-place phase sweeps per step (default); 0: the fused PML sweep."""
+        _lib.check(self._L.fdtd_set_pml_path(self._h, int(path), int(V), int(TY), int(kchunk)))
+
+    def get_pml_path(self):
+        out = (ctypes.c_int * 4)()
+        _lib.check(self._L.fdtd_get_pml_path(self._h, out))
+        return dict(path=out[0], V=out[1], TY=out[2], kchunk=out[3])
+
     def split(self, name) -> np.ndarray:
         """FP64 copy of one split field, by name ("Exy", ...) or index 0..11."""
         q = self.SPLITS.index(name) if isinstance(name, str) else int(name)
@@ -484,6 +500,12 @@ def pml_profile(n: int, pml_percent: float, d: float, dt: float):
     _lib.check(L.fdtd_pml_profile(int(n), float(pml_percent), float(d), float(dt),
                                   *[_dptr(a) for a in out]))
     return tuple(out)
+
+
+def set_tune_cache(path) -> None:
+    """Keep fdtd_autotune_blocking's picks in the text file `path` (None: off)."""
+    L = _lib.load()
+    _lib.check(L.fdtd_set_tune_cache(None if path is None else str(path).encode()))
 
 
 def calibration_copy(nbytes: int, iters: int = 10) -> float:

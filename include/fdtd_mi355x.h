@@ -102,6 +102,20 @@ int fdtd_destroy(fdtd_grid* grid);
  * only (FDTD_ERR_UNSUPPORTED). */
 int fdtd_create_pml(const fdtd_parameters* params, double dt, double pml_percent, int precision,
                     fdtd_grid** out);
+/* How a PML grid's fused steps run (no counterpart in the reference, whose
+ * step is FDTD_PML.cpp:343-365): path 0 (the default) as the single fused
+ * PML sweep (shape: fdtd_set_tuning); path 1 as two in-place sweeps per
+ * step, an E phase and a B phase, with V cells per lane, TY full rows per
+ * workgroup and kchunk planes per z chunk (0: defaults; V*storage bytes at
+ * most 16).  At 512^3 with a 0.2 layer the phases take 3.94 ms per FP32
+ * step against the fused sweep's 3.56 (FP64: 7.51 both).  FP64
+ * and FP32 results are bit-identical either way (and FP64 to the
+ * reference); with FP16 storage the phase path's B update reads E as
+ * stored where the fused sweep uses it before rounding, so the two agree
+ * to FP16 rounding.  FP16 sweeps that rescale or track the max run the
+ * fused sweep.  get: {path, V, TY, kchunk}. */
+int fdtd_set_pml_path(fdtd_grid* grid, int path, int V, int TY, int kchunk);
+int fdtd_get_pml_path(const fdtd_grid* grid, int out[4]);
 /* Layer widths {p_i, p_j, p_k} (FDTD_PML.cpp:254-256) and pml_percent
  * (widths 0 and pml_percent 0 for fdtd_create's grids). */
 int fdtd_get_pml(const fdtd_grid* grid, int widths[3], double* pml_percent);
@@ -228,6 +242,10 @@ int fdtd_max_abs(fdtd_grid* grid, double out[6]);
  * on one GPU share that array: calls that overlap on one GPU (from several
  * host threads) race.  Synchronises the grid's stream. */
 int fdtd_sum_squares_noalloc(fdtd_grid* grid, double out[6]);
+/* Non-finite check (no counterpart in the reference, which has no failure
+ * detection): the number of NaN and +-Inf values of Ex..Bz as stored (an FP16
+ * overflow shows here as Inf).  Synchronises the grid's stream. */
+int fdtd_count_nonfinite(fdtd_grid* grid, long long out[6]);
 /* Error of grid `test` (any precision) against FP64 grid `ref` of the same
  * shape, per component c: out[3c] = sum (t-r)^2, out[3c+1] = sum r^2,
  * out[3c+2] = max |t-r|. */
@@ -290,6 +308,13 @@ int fdtd_autotune(fdtd_grid* grid, int reps, int out[3], float* best_ms);
 int fdtd_set_time_blocking(fdtd_grid* grid, int L, int V, int lanes, int TYg, int kchunk);
 int fdtd_get_time_blocking(const fdtd_grid* grid, int out[5]);
 int fdtd_autotune_blocking(fdtd_grid* grid, int reps, int out[5], float* best_ms);
+/* Autotuner cache (no counterpart in the reference): fdtd_autotune_blocking
+ * keeps its picks in the text file `path`, one line per (shape, precision,
+ * device architecture, CU count), and a later call on such a grid, in any
+ * process, takes the recorded pick and its ms per step without timing.
+ * NULL or "" turns the cache off; the default is $FDTD_MI355X_TUNE_CACHE
+ * (unset: off).  Process-wide. */
+int fdtd_set_tune_cache(const char* path);
 
 /* Temporal blocking with the wavefront-strip kernel: tiles of 64*V columns
  * by W*R rows (one wavefront per R rows, R rows per lane in registers;

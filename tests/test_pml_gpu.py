@@ -68,12 +68,14 @@ def spread_current(n, seed):
     return J
 
 
-@pytest.mark.parametrize("path", ["fused", "3sweep"])
+@pytest.mark.parametrize("path", ["phases", "fused", "3sweep"])
 @pytest.mark.parametrize("shape,pml", CASES, ids=lambda v: "x".join(map(str, v))
                          if isinstance(v, tuple) else str(v))
 def test_pml_fp64_bit_exact(shape, pml, path):
     """Random fields and a dense J = Jx = Jy = Jz on cells of the main box and
-    of the layer; 5 steps; totals and splits bit-exact."""
+    of the layer; 5 steps; totals and splits bit-exact.  fused: the fused
+    PML sweep (the default); phases: the two in-place phase sweeps per step;
+    3sweep: the reference's structure."""
     n = int(np.prod(shape))
     spacing = (1.0, 1.3, 0.7)
     dt = courant_dt(*spacing)
@@ -82,10 +84,11 @@ def test_pml_fp64_bit_exact(shape, pml, path):
     print(f"case shape={shape} pml={pml} path={path}", flush=True)
     g = pml_gpu(shape, spacing, dt, pml, st + [J, J, J])
     r = pml_oracle(shape, spacing, dt, pml, st, [J, J, J])
-    if path == "fused":
-        g.update_fields(5)
-    else:
+    g.set_pml_path(1 if path == "phases" else 0)
+    if path == "3sweep":
         g.update_fields_3sweep(5)
+    else:
+        g.update_fields(5)
     r.update_fields(5)
     assert g.pml_widths == O.pml_sizes(*shape, pml)
     assert_same(g, r)
@@ -104,11 +107,79 @@ def test_pml_fp64_launch_shapes(tune):
     st = random_state(n, seed=77)
     print(f"case tune={tune}", flush=True)
     g = pml_gpu(shape, spacing, dt, pml, st, tune=tune)
+    g.set_pml_path(0)  # the tuning shapes the fused sweep
     r = pml_oracle(shape, spacing, dt, pml, st)
     for steps in (1, 4):
         g.update_fields(steps)
         r.update_fields(steps)
         assert_same(g, r)
+
+
+PHASE_SHAPES = [(1, 1, 1), (1, 3, 4), (2, 2, 5), (2, 7, 18), (2, 1, 3), (1, 5, 2), (1, 23, 7),
+                (2, 24, 18)]
+
+
+@pytest.mark.parametrize("shape_v", PHASE_SHAPES, ids=lambda t: "V%d_TY%d_k%d" % t)
+def test_pml_fp64_phase_shapes(shape_v):
+    """The phase sweeps' tile heights (tiles of one row, of a ragged last
+    tile, of the whole grid), z chunks (one plane to whole) and lane widths
+    (FP64 packs of 1 and 2 cells; 4-cell packs run in the FP32 and FP16
+    tests against the fused sweep) put tile boundaries and chunk boundaries
+    inside and across the layer."""
+    shape, pml, spacing = (40, 24, 18), 0.2, (1.0, 1.0, 1.0)
+    n = int(np.prod(shape))
+    dt = courant_dt(*spacing)
+    st = random_state(n, seed=78)
+    print(f"case phases V,TY,kchunk={shape_v}", flush=True)
+    g = pml_gpu(shape, spacing, dt, pml, st)
+    g.set_pml_path(1, *shape_v)
+    assert g.get_pml_path() == dict(path=1, V=shape_v[0], TY=shape_v[1], kchunk=shape_v[2])
+    r = pml_oracle(shape, spacing, dt, pml, st)
+    for steps in (1, 4):
+        g.update_fields(steps)
+        r.update_fields(steps)
+        assert_same(g, r)
+
+
+@pytest.mark.parametrize("prec", [Precision.FP32, Precision.FP16])
+def test_pml_phases_equal_fused_sweep(prec):
+    """The two phase sweeps per step and the fused PML sweep give the same
+    bits in FP32, with point sources in the main box and the layer.  With
+    FP16 storage (power-of-two scales, then in-run rescaling, whose
+    rescaling sweeps are fused ones in both runs) the phase path's B update
+    reads E_{n+1} as stored, rounded to FP16, where the fused sweep uses it
+    before rounding: the two agree to FP16 rounding (relative L2 <= 5e-3,
+    3x the measured 1.7e-3), with the same scale decisions."""
+    shape, pml, spacing = (48, 40, 36), 0.2, (1.0, 1.0, 1.0)
+    n = int(np.prod(shape))
+    dt = 0.5 / (O.C * np.sqrt(3.0))
+    st = [0.01 * x for x in random_state(n, seed=31)]
+    src = [dict(i=24, j=20, k=18, comp=2, kind=1, amp=1.0, period=20 * dt, t0=10 * dt,
+                sigma=15 * dt),
+           dict(i=2, j=3, k=4, comp=0, kind=0, amp=0.5)]
+    runs = []
+    for path in (1, 0):
+        g = pml_gpu(shape, spacing, dt, pml, st, precision=prec)
+        g.set_point_sources(src)
+        if prec == Precision.FP16:
+            g.autoscale(target_log2=10)
+        g.set_pml_path(path)
+        g.update_fields(7)
+        g.update_fields(1)
+        if prec == Precision.FP16:
+            g.set_autoscale_interval(4, 10)
+        g.update_fields(13)
+        runs.append(g)
+    a, b = runs
+    assert a.get_scales() == b.get_scales()
+    if prec == Precision.FP16:
+        err = rel_l2([a.download(c) for c in range(6)], [b.download(c) for c in range(6)])
+        print("fp16 phases vs fused rel L2", err, flush=True)
+        assert 0 < err <= 5e-3
+        return
+    assert a.checksum() == b.checksum()
+    for q in range(12):
+        assert np.array_equal(a.split(q), b.split(q)), FDTD_PML.SPLITS[q]
 
 
 def test_pml_fp64_matches_golden(golden_dir):
