@@ -19,6 +19,11 @@
 
 using namespace fdtd::rt;
 
+namespace fdtd {
+// The batch's device-side step counter (graph replays advance it).
+static __global__ void advance_step(long long* step, long long by) { *step += by; }
+}  // namespace fdtd
+
 namespace {
 
 constexpr int kBatchLdsCap = 80 * 1024;  // per workgroup: two per CU

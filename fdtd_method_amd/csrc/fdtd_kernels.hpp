@@ -646,8 +646,7 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
     const int r = tid / lanes;
     const int ib = (PML ? pml_pack(tid - r * lanes, lanes, a.pml.pi, V) : tid - r * lanes) * V;
     const bool active = r <= ty;
-    const bool own = r < ty;
-    const bool halo = r == ty;
+    const bool own = r < ty;  // row r == ty is the halo row
     const int j = (j0 + r >= Nj) ? j0 + r - Nj : j0 + r;  // halo row wraps
     const int jm = (j0 == 0) ? Nj - 1 : j0 - 1;
     const int plane = Ni * Nj;
@@ -2859,8 +2858,6 @@ zero_bytes(uint4* __restrict__ p, size_t n16) {
          i += static_cast<size_t>(gridDim.x) * 256)
         p[i] = make_uint4(0u, 0u, 0u, 0u);
 }
-
-static __global__ void advance_step(long long* step, long long by) { *step += by; }
 
 // Order-independent checksum of the bit patterns of an array: the sum mod
 // 2^64 of a 64-bit mix of (bits, index).  Integer addition makes the result

@@ -943,7 +943,7 @@ void tune_cache_store(const std::string& path, const std::string& key, const Blo
     std::lock_guard<std::mutex> lock(g_tune_mu);
     FILE* f = std::fopen(path.c_str(), "a");
     if (!f) return;  // a cache that cannot be written only costs the next run a retune
-    std::fprintf(f, "%s | %d %d %d %d %d %d %.6f\n", key.c_str(), t.L, t.V, t.lanes, t.TYg,
+    std::fprintf(f, "%s | %d %d %d %d %d %d %.9g\n", key.c_str(), t.L, t.V, t.lanes, t.TYg,
                  t.kchunk, t.R, ms);
     std::fclose(f);
 }
@@ -2764,7 +2764,6 @@ int fdtd_autotune(fdtd_grid* g, int reps, int out[3], float* best_ms) {
     const int nv = v_candidates(g->precision, vc);
     Tuning best = g->tune;
     float best_t = 1e30f;
-    const int cs = (int)compute_size(g->precision);
     for (int q = 0; q < nv; ++q) {
         const int V = vc[q];
         if (g->Ni % V) continue;
