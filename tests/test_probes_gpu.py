@@ -298,3 +298,49 @@ def test_batches_refuse_recording_grids():
         batch.update_fields(2)
     for g in grids:
         g.close()
+
+
+def test_probes_and_energy_together_on_pml_and_fp16_strips():
+    """Both records at once: on a PML grid (probes gathered after single-step
+    runs, the run split at the energy records) FP64 rows equal the oracle's
+    fields bitwise and the energies its sums to 1e-12; on FP16 strip passes
+    with in-run rescaling, split at the energy records, the last row is the
+    final state and the records fall on the requested steps."""
+    shape, pml = (24, 20, 16), 0.25
+    dt = courant_dt(1, 1, 1, factor=0.5)
+    n = int(np.prod(shape))
+    st = random_state(n, 68)
+    src = constant_sources(shape)
+    cells = probe_cells(shape, 69, n=24)
+    expect, _ = oracle_rows(shape, dt, st, src, cells, 11, pml=pml)
+    r = O.OraclePmlGrid(*shape, 1.0, 1.0, 1.0, dt, pml)
+    for c in range(6):
+        r.fields()[c][:] = st[c]
+    sums = []
+    for s in range(1, 12):
+        r.run_point_sources(src, 1)
+        if s % 3 == 0:
+            sums.append(r.sum_squares())
+    g = make(shape, dt, st, Precision.FP64, src, cls=FDTD_PML, pml=pml)
+    g.set_probes(cells, capacity=11)
+    g.set_energy_interval(3, capacity=3)
+    g.update_fields(4)
+    g.update_fields(7)
+    assert np.array_equal(g.read_probes(), expect)
+    steps, got = g.energy_records()
+    assert steps == [3, 6, 9]
+    ref = np.array(sums)
+    assert np.all(np.abs(got - ref) <= 1e-12 * np.abs(ref))
+    g.close()
+
+    (a, b), cells, idx, comp = twin_grids(Precision.FP16, (4, 1, 2, 8, 0), rescale=True)
+    b.close()
+    a.set_probes(cells, capacity=18)
+    a.set_energy_interval(6, capacity=3)
+    a.update_fields(18)
+    rows = a.read_probes()
+    assert not np.isnan(rows).any()
+    assert np.array_equal(rows[-1], at_cells(a, idx, comp))
+    steps, _ = a.energy_records()
+    assert steps == [6, 12, 18]
+    a.close()
