@@ -1061,9 +1061,11 @@ fused_sweep(SweepArgs<S, C, JT> a) {
 // layer cells also read and write their split fields).  Halo rows and
 // chunk-boundary planes recompute layer E from the unmodified es_in, so the
 // E splits are ping-ponged like E and B; B splits are only touched by
-// their owning tile and stay in place.
+// their owning tile and stay in place.  Lanes of 8 bytes or less are
+// bounded at 768 threads (three wavefronts per SIMD, up to 168 VGPRs): at
+// 1024 the FP32 V=2 sweep spilled.
 __host__ __device__ constexpr int fused_pml_max_threads(int V, int csize) {
-    return V * csize >= 16 ? 512 : 1024;
+    return V * csize >= 16 ? 512 : 768;
 }
 
 template <typename S, typename C, typename JT, int V, bool STEADY, bool AUTO = false>

@@ -94,7 +94,7 @@ def test_pml_fp64_bit_exact(shape, pml, path):
     assert_same(g, r)
 
 
-LAUNCH = [(1, 1, 1), (1, 3, 4), (2, 2, 5), (2, 7, 18), (1, 23, 3), (1, 11, 7), (2, 23, 2)]
+LAUNCH = [(1, 1, 1), (1, 3, 4), (2, 2, 5), (2, 7, 18), (1, 17, 3), (1, 11, 7), (2, 23, 2)]
 
 
 @pytest.mark.parametrize("tune", LAUNCH, ids=lambda t: "V%d_TY%d_k%d" % t)
