@@ -368,7 +368,7 @@ int apply_strips(fdtd_batch* b, int L, int V, int R, int W) {
         return fail(FDTD_ERR_INVALID_ARGUMENT, "strip blocking L=%d V=%d R=%d W=%d not available",
                     L, V, R, W);
     int Gx = 0, TX = 0;
-    strip_x_geometry(L, V, storage_size(b->precision), &Gx, &TX);
+    strip_x_geometry(L, V, storage_size(b->precision), &Gx, &TX, false);
     const int TY = W * R - 2 * L;
     if (TX < 1 || TY < 1)
         return fail(FDTD_ERR_INVALID_ARGUMENT, "strip tiles of W=%d R=%d leave no stored rows", W, R);
