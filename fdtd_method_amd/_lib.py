@@ -27,6 +27,7 @@ EXPORTED = [
     "fdtd_pml_download_split", "fdtd_pml_profile", "fdtd_upload", "fdtd_download",
     "fdtd_download_plane",
     "fdtd_zero_currents", "fdtd_set_current_values", "fdtd_set_point_sources",
+    "fdtd_device_field", "fdtd_get_stream", "fdtd_update_b_box", "fdtd_update_e_box",
     "fdtd_update_fields_with_currents", "fdtd_get_stats",
     "fdtd_set_probes", "fdtd_probe_rows", "fdtd_read_probes", "fdtd_set_energy_interval",
     "fdtd_energy_rows", "fdtd_read_energy", "fdtd_rewind_records",
@@ -101,6 +102,11 @@ def load():
         "fdtd_download": ([_vp, _i, _dp, ctypes.c_size_t], _i),
         "fdtd_download_plane": ([_vp, _i, _i, _dp, ctypes.c_size_t], _i),
         "fdtd_zero_currents": ([_vp], _i),
+        "fdtd_device_field": ([_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_i),
+                               ctypes.POINTER(ctypes.c_double)], _i),
+        "fdtd_get_stream": ([_vp, ctypes.POINTER(_vp)], _i),
+        "fdtd_update_b_box": ([_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)], _i),
+        "fdtd_update_e_box": ([_vp, ctypes.POINTER(_i), ctypes.POINTER(_i)], _i),
         "fdtd_set_current_values": ([_vp, _i, ctypes.POINTER(ctypes.c_longlong),
                                      ctypes.POINTER(_i), _dp], _i),
         "fdtd_set_point_sources": ([_vp, _i, ctypes.POINTER(PointSource)], _i),

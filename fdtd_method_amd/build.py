@@ -90,12 +90,28 @@ def _gxx(src: str, out: str, rpath: str, verbose: bool) -> None:
         raise RuntimeError(f"g++ failed for {src}:\n{res.stderr[-6000:]}")
 
 
+def _hipcc_program(src: str, out: str, rpath: str, verbose: bool) -> None:
+    cmd = [hipcc(), "--offload-arch=gfx950", "-O2", "-std=c++17", "-ffp-contract=off",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-Wno-unused-command-line-argument", f"-I{INCLUDE}", src, "-o", out, f"-L{PKG}",
+           "-lfdtd_mi355x", f"-Wl,-rpath,{rpath}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr[-6000:]}")
+
+
 def build_cpp_tests(verbose: bool = False) -> None:
-    """Compile the host-side C++ programs that use the drop-in class:
-    the sample driver (fdtd_method_amd/bin/sample) and the C++ unit test
-    (tests/cpp/test_fdtd_mi355x).  Both link libfdtd_mi355x.so."""
+    """Compile the programs that use the drop-in class: the sample driver
+    (fdtd_method_amd/bin/sample), the device-current driver
+    (fdtd_method_amd/bin/device_sample, HIP: it writes J in a kernel of its
+    own) and the C++ unit test (tests/cpp/test_fdtd_mi355x).  All link
+    libfdtd_mi355x.so."""
     os.makedirs(BIN_DIR, exist_ok=True)
     _gxx(os.path.join(CSRC, "sample.cpp"), os.path.join(BIN_DIR, "sample"), "$ORIGIN/..", verbose)
+    _hipcc_program(os.path.join(CSRC, "device_sample.hip"), os.path.join(BIN_DIR, "device_sample"),
+                   "$ORIGIN/..", verbose)
     _gxx(os.path.join(CPP_TEST_DIR, "test_fdtd_mi355x.cpp"),
          os.path.join(CPP_TEST_DIR, "test_fdtd_mi355x"), "$ORIGIN/../../fdtd_method_amd", verbose)
 

@@ -108,6 +108,9 @@ struct fdtd_grid {
     int cur = 0;
     void* J[3] = {};
     bool j_dense_active = false;
+    // J's device addresses were handed out (fdtd_device_field): J stays
+    // dense and is cleared in place, since the caller may write it later.
+    bool j_device = false;
     double* staging = nullptr;
     double scales[6] = {1, 1, 1, 1, 1, 1};
     hipStream_t stream = nullptr;

@@ -406,6 +406,37 @@ plain_update_E(Fields<S> f, const JT* __restrict__ jx, const JT* __restrict__ jy
     plain_E_cell<S, C, JT>(f, jx, jy, jz, src, vals, nsrc, c, Ni, Nj, Nk, i, j, k);
 }
 
+// The plain updates over the box [lo, hi) of cells (the reference's
+// begin_main_* .. end_main_* loop bounds, src/FDTD/FDTD.cpp:63-130,
+// src/FDTD_kokkos/FDTD_PML_kokkos.cpp:160-198); neighbours still wrap over
+// the whole grid.  Block (64, 4), grid over the box's extent, z per block.
+struct Box {
+    int lo[3], hi[3];
+};
+
+template <typename S, typename C>
+__global__ void __launch_bounds__(256)
+box_update_B(Fields<S> f, Coefs<C> c, int Ni, int Nj, int Nk, Box box) {
+    const int i = box.lo[0] + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = box.lo[1] + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = box.lo[2] + blockIdx.z;
+    if (i >= box.hi[0] || j >= box.hi[1]) return;
+    plain_B_cell<S, C>(f, c, Ni, Nj, Nk, i, j, k);
+}
+
+template <typename S, typename C, typename JT>
+__global__ void __launch_bounds__(256)
+box_update_E(Fields<S> f, const JT* __restrict__ jx, const JT* __restrict__ jy,
+             const JT* __restrict__ jz, const PointSource* __restrict__ src,
+             const double* __restrict__ vals, int nsrc, Coefs<C> c, int Ni, int Nj, int Nk,
+             Box box) {
+    const int i = box.lo[0] + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = box.lo[1] + blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = box.lo[2] + blockIdx.z;
+    if (i >= box.hi[0] || j >= box.hi[1]) return;
+    plain_E_cell<S, C, JT>(f, jx, jy, jz, src, vals, nsrc, c, Ni, Nj, Nk, i, j, k);
+}
+
 // ---------------------------------------------------------------------------
 // PML absorbing layer (src/FDTD/FDTD_PML.cpp; Kokkos:
 // include/FDTD_kokkos/kokkos_functors.h:189-415).

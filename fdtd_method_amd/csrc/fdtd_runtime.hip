@@ -408,6 +408,40 @@ int plain_E(fdtd_grid* g, long long step) {
     }
 }
 
+// One B half step (e_step false) or E step (true) of the plain update over
+// the box; the E step adds dense J and the point sources of step g->step.
+// The caller has checked 0 <= lo < hi <= N on every axis.
+template <typename S>
+int launch_box(fdtd_grid* g, const fdtd::Box& box, bool e_step) {
+    using C = typename fdtd::ComputeOf<S>::type;
+    using JT = C;
+    const int nx = box.hi[0] - box.lo[0], ny = box.hi[1] - box.lo[1], nz = box.hi[2] - box.lo[2];
+    const dim3 block(64, 4, 1);
+    const dim3 grid((nx + 63) / 64, (ny + 3) / 4, nz);
+    if (!e_step) {
+        hipLaunchKernelGGL((fdtd::box_update_B<S, C>), grid, block, 0, g->stream,
+                           fields_of<S>(g, g->cur), make_coefs<C>(g), g->Ni, g->Nj, g->Nk, box);
+    } else {
+        const JT* j[3] = {nullptr, nullptr, nullptr};
+        if (g->j_dense_active)
+            for (int d = 0; d < 3; ++d) j[d] = static_cast<const JT*>(g->J[d]);
+        if (int rc = eval_sources(g, g->step, 1)) return rc;
+        hipLaunchKernelGGL((fdtd::box_update_E<S, C, JT>), grid, block, 0, g->stream,
+                           fields_of<S>(g, g->cur), j[0], j[1], j[2], g->d_src, g->d_vals,
+                           g->nsrc, make_coefs<C>(g), g->Ni, g->Nj, g->Nk, box);
+    }
+    HIP_TRY(hipGetLastError());
+    return FDTD_OK;
+}
+
+int box_update(fdtd_grid* g, const fdtd::Box& box, bool e_step) {
+    switch (g->precision) {
+        case FDTD_FP64: return launch_box<double>(g, box, e_step);
+        case FDTD_FP32: return launch_box<float>(g, box, e_step);
+        default: return launch_box<_Float16>(g, box, e_step);
+    }
+}
+
 // ---- PML layer ----------------------------------------------------------------
 
 // Layer width of an axis of n cells, src/FDTD/FDTD_PML.cpp:254-256.
@@ -1835,10 +1869,92 @@ int fdtd_download_plane(fdtd_grid* g, int c, int k, double* host, size_t count) 
 
 int fdtd_zero_currents(fdtd_grid* g) {
     if (!g) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
-    g->j_dense_active = false;
     if (!g->sparse_j.empty()) g->src_dirty = true;
     g->sparse_j.clear();
+    if (g->j_device) {
+        // the caller holds J's addresses: clear the arrays in place, queued
+        // after the steps already launched (Kokkos::deep_copy(J, 0),
+        // src/FDTD_kokkos/FDTD_kokkos.cpp:69-73)
+        for (int d = 0; d < 3; ++d)
+            if (int rc = zero_fill(g->J[d], g->N * current_size(g->precision), g->stream)) return rc;
+        return FDTD_OK;
+    }
+    g->j_dense_active = false;
     return FDTD_OK;
+}
+
+int fdtd_device_field(fdtd_grid* g, int c, void** ptr, int* elem_bytes, double* scale) {
+    if (!g || !ptr) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    if (int rc = check_component(c)) return rc;
+    if (c >= FDTD_JX) {
+        if (!g->j_device) {
+            if (int rc = ensure_dense_j(g)) return rc;
+            if (!g->j_dense_active) {
+                for (int d = 0; d < 3; ++d)
+                    if (int rc = zero_fill(g->J[d], g->N * current_size(g->precision), g->stream))
+                        return rc;
+                g->j_dense_active = true;
+                if (!g->sparse_j.empty()) {
+                    // single-cell currents set so far move into the arrays
+                    std::vector<long long> idx;
+                    std::vector<int> cc;
+                    std::vector<double> vv;
+                    for (const auto& kv : g->sparse_j) {
+                        idx.push_back(kv.first / 3);
+                        cc.push_back((int)(kv.first % 3) + FDTD_JX);
+                        vv.push_back(kv.second);
+                    }
+                    g->sparse_j.clear();
+                    g->src_dirty = true;
+                    if (int rc = fdtd_set_current_values(g, (int)idx.size(), idx.data(), cc.data(),
+                                                         vv.data()))
+                        return rc;
+                }
+            }
+            g->j_device = true;
+        }
+        *ptr = g->J[c - FDTD_JX];
+        if (elem_bytes) *elem_bytes = (int)current_size(g->precision);
+        if (scale) *scale = 1.0;
+        return FDTD_OK;
+    }
+    g->ba_cur = -1;  // the caller may change E or B before the next step
+    *ptr = field_ptr(g, c);
+    if (elem_bytes) *elem_bytes = (int)g->esize;
+    if (scale) *scale = g->scales[c];
+    return FDTD_OK;
+}
+
+int fdtd_get_stream(fdtd_grid* g, void** stream) {
+    if (!g || !stream) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    *stream = static_cast<void*>(g->stream);
+    return FDTD_OK;
+}
+
+static int box_entry(fdtd_grid* g, const int lo[3], const int hi[3], bool e_step) {
+    if (!g || !lo || !hi) return fail(FDTD_ERR_INVALID_ARGUMENT, "null argument");
+    const int n[3] = {g->Ni, g->Nj, g->Nk};
+    fdtd::Box box;
+    for (int a = 0; a < 3; ++a) {
+        if (lo[a] < 0 || hi[a] > n[a] || lo[a] > hi[a])
+            return fail(FDTD_ERR_INVALID_ARGUMENT, "box [%d, %d) outside axis %d of %d cells",
+                        lo[a], hi[a], a, n[a]);
+        box.lo[a] = lo[a];
+        box.hi[a] = hi[a];
+    }
+    g->ba_cur = -1;  // E, B change outside a run: the carried half step is stale
+    if (int rc = flush_sources(g)) return rc;
+    if (lo[0] == hi[0] || lo[1] == hi[1] || lo[2] == hi[2]) return FDTD_OK;
+    TraceRange tr(e_step ? "update_E_box" : "update_B_box");
+    return box_update(g, box, e_step);
+}
+
+int fdtd_update_b_box(fdtd_grid* g, const int lo[3], const int hi[3]) {
+    return box_entry(g, lo, hi, false);
+}
+
+int fdtd_update_e_box(fdtd_grid* g, const int lo[3], const int hi[3]) {
+    return box_entry(g, lo, hi, true);
 }
 
 int fdtd_set_current_values(fdtd_grid* g, int n, const long long* index, const int* comp,
@@ -2460,7 +2576,12 @@ int fdtd_zero_fields(fdtd_grid* g) {
                 if (int rc = zero_fill(g->ES[b][q], bytes, g->stream)) return rc;
             if (int rc = zero_fill(g->BS[q], bytes, g->stream)) return rc;
         }
-    g->j_dense_active = false;
+    if (g->j_device) {
+        for (int d = 0; d < 3; ++d)
+            if (int rc = zero_fill(g->J[d], g->N * current_size(g->precision), g->stream)) return rc;
+    } else {
+        g->j_dense_active = false;
+    }
     if (!g->sparse_j.empty()) g->src_dirty = true;
     g->sparse_j.clear();
     g->step = 0;

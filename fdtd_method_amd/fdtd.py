@@ -101,6 +101,22 @@ class Field:
         self._dirty = True
 
 
+class DeviceArray:
+    """One device array of a grid: address ``ptr``, element type ``dtype``,
+    ``size`` elements (index i + j*Ni + k*Ni*Nj) and FP16 storage ``scale``.
+    Exposes ``__cuda_array_interface__``, so ``torch.as_tensor(a,
+    device="cuda")`` aliases it; it keeps its grid alive."""
+
+    def __init__(self, ptr: int, dtype: np.dtype, size: int, scale: float, owner: "FDTD"):
+        self.ptr, self.dtype, self.size, self.scale = ptr, dtype, size, scale
+        self._owner = owner
+
+    @property
+    def __cuda_array_interface__(self):
+        return {"shape": (self.size,), "typestr": self.dtype.str, "data": (self.ptr, False),
+                "version": 2}
+
+
 class FDTD:
     """Yee solver on one MI355X grid (reference API, see module docstring)."""
 
@@ -126,6 +142,11 @@ class FDTD:
         self.shape = (parameters.Ni, parameters.Nj, parameters.Nk)
         self.cells = parameters.Ni * parameters.Nj * parameters.Nk
         self._fields = {}
+        self._device = False  # device addresses handed out: mirrors refresh after every call
+        # the update_E/update_B box, include/FDTD/FDTD.h:17-18 (whole grid here;
+        # FDTD_PML narrows it to the interior as src/FDTD/FDTD_PML.cpp:263-268)
+        self.begin_main = (0, 0, 0)
+        self.end_main = self.shape
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -180,7 +201,7 @@ class FDTD:
 
     def _invalidate(self) -> None:
         for c, f in self._fields.items():
-            if c < Component.JX:
+            if c < Component.JX or self._device:
                 f._invalidate()
 
     def upload(self, comp, values) -> None:
@@ -317,6 +338,72 @@ class FDTD:
 
     def synchronize(self) -> None:
         _lib.check(self._L.fdtd_synchronize(self._h))
+        if self._device:
+            self._invalidate()
+
+    # -- device access and the protected curl updates --------------------------
+    def device_field(self, component) -> "DeviceArray":
+        """The device array of a component (0..8), for the caller's own GPU
+        work: the Kokkos class's ``get_field`` returns device Views
+        (include/FDTD_kokkos/FDTD_kokkos.h:31) that its driver fills on the
+        device (perf-tests/kokkos_sample/kokkos_sample.cpp:82-108).
+        ``torch.as_tensor(a, device="cuda")`` aliases the returned array.
+        J arrays are FP64 on FP64 grids, FP32 otherwise, and stay at the same
+        address (J becomes dense, which turns temporal blocking off); E and B
+        hold the storage type (FP16: scale * value) and are valid until the
+        next step.  The grid's work runs on ``torch_stream()``: do the
+        writes there, or synchronize both ways.  From this call on, host
+        mirrors are refreshed after every stepping or synchronizing call."""
+        try:
+            comp = Component(component)
+        except ValueError:
+            raise LookupError("ERROR: Invalid field component") from None
+        self._flush()
+        ptr, eb, sc = ctypes.c_void_p(), ctypes.c_int(), ctypes.c_double()
+        _lib.check(self._L.fdtd_device_field(self._h, int(comp), ctypes.byref(ptr),
+                                             ctypes.byref(eb), ctypes.byref(sc)))
+        self._device = True
+        self._invalidate()
+        dtype = {8: np.float64, 4: np.float32, 2: np.float16}[eb.value]
+        return DeviceArray(ptr.value, np.dtype(dtype), self.cells, sc.value, self)
+
+    def stream_handle(self) -> int:
+        """The grid's HIP stream (hipStream_t) as an integer."""
+        s = ctypes.c_void_p()
+        _lib.check(self._L.fdtd_get_stream(self._h, ctypes.byref(s)))
+        return s.value or 0
+
+    def torch_stream(self):
+        """The grid's stream as a ``torch.cuda.ExternalStream``."""
+        import torch
+
+        return torch.cuda.ExternalStream(self.stream_handle())
+
+    def _box(self, lo, hi):
+        lo = self.begin_main if lo is None else lo
+        hi = self.end_main if hi is None else hi
+        return (ctypes.c_int * 3)(*[int(v) for v in lo]), (ctypes.c_int * 3)(*[int(v) for v in hi])
+
+    def update_B(self, lo=None, hi=None) -> None:
+        """The reference's protected ``update_B()`` (src/FDTD/FDTD.cpp:99-130):
+        one B half step over the cells [lo, hi), by default the main box
+        (``begin_main`` .. ``end_main``); neighbours wrap over the whole grid.
+        Does not advance ``step``."""
+        self._flush()
+        a, b = self._box(lo, hi)
+        _lib.check(self._L.fdtd_update_b_box(self._h, a, b))
+        self._invalidate()
+
+    def update_E(self, lo=None, hi=None) -> None:
+        """The reference's protected ``update_E()`` (src/FDTD/FDTD.cpp:63-97,
+        J paired as Jx->Ex, Jy->Ey, Jz->Ez like
+        include/FDTD_kokkos/kokkos_functors.h:64-90): one E step with J and
+        the point sources of the current step over [lo, hi), by default the
+        main box.  Does not advance ``step``."""
+        self._flush()
+        a, b = self._box(lo, hi)
+        _lib.check(self._L.fdtd_update_e_box(self._h, a, b))
+        self._invalidate()
 
     def fields(self):
         """[Ex, Ey, Ez, Bx, By, Bz] as FP64 numpy arrays."""
@@ -467,6 +554,10 @@ class FDTD_PML(FDTD):
     def __init__(self, parameters: Parameters, dt: float, pml_percent: float,
                  precision: Precision = Precision.FP64):
         self._open(parameters, dt, precision, pml_percent)
+        # main box = interior, src/FDTD/FDTD_PML.cpp:259-268
+        p = self.pml_widths
+        self.begin_main = tuple(p)
+        self.end_main = tuple(n - w for n, w in zip(self.shape, p))
 
     @property
     def pml_widths(self):
@@ -476,7 +567,7 @@ class FDTD_PML(FDTD):
         return tuple(w)
 
     def set_pml_path(self, path: int, V: int = 0, TY: int = 0, kchunk: int = 0) -> None:
-        """1: two in-place phase sweeps per step (default); 0: the fused PML sweep."""
+        """0: the fused PML sweep (default); 1: two in-place phase sweeps per step."""
         _lib.check(self._L.fdtd_set_pml_path(self._h, int(path), int(V), int(TY), int(kchunk)))
 
     def get_pml_path(self):

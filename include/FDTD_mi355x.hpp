@@ -21,6 +21,18 @@
 //    results of stepping one call at a time, bit for bit.
 //  * The constructor takes an optional storage precision (FP64 default).
 //  * update_fields(int n) advances n steps in one call.
+//  * device_field(Component) gives the device array (the Kokkos class's
+//    get_field() returns device Views, filled by its driver on the device,
+//    perf-tests/kokkos_sample/kokkos_sample.cpp:82-108).  From the first
+//    such call on, update_fields() runs at once (device writes between
+//    calls are invisible to the deferral) and host mirrors refresh after
+//    every step, zeroed_currents() and synchronize().
+//  * The reference's protected members are here for subclasses that write
+//    their own update_fields() (as FDTD_PML does, src/FDTD/FDTD_PML.cpp:343-365):
+//    update_E() / update_B() over begin_main_* .. end_main_*,
+//    applyPeriodicBoundary, the coefficients, Ni..dt, parameters, and Field
+//    references Jx..Bz to the host mirrors.  The accessors of the earlier
+//    public surface are get_parameters() and get_dt().
 //  * Only single-threaded use per object, like the reference.
 //
 // Header-only: it needs the C-ABI (fdtd_mi355x.h) and links against
@@ -183,13 +195,19 @@ public:
 
     // virtual void update_fields() -- src/FDTD/FDTD.cpp:153-157: one step,
     // deferred (see the header comment).
-    virtual void update_fields() { defer_step(); }
+    virtual void update_fields() {
+        if (device_mode_) {
+            update_fields(1);
+            return;
+        }
+        defer_step();
+    }
 
     // n steps in one call (after any deferred ones).
     void update_fields(int nsteps) {
         flush();
         check_status(fdtd_update_fields(grid_, nsteps));
-        for (int c = 0; c < 6; ++c) fields_[c].invalidate();
+        invalidate_mirrors();
     }
 
     // void zeroed_currents() -- src/FDTD/FDTD.cpp:132-136.
@@ -197,6 +215,37 @@ public:
         run_pending();
         for (int c = 6; c < 9; ++c) fields_[c].drop();
         check_status(fdtd_zero_currents(grid_));
+    }
+
+    // Device array of a component (see the header comment and
+    // fdtd_device_field): J in double (FP64 grids) or float, at a fixed
+    // address; E/B in the storage type, valid until the next step.
+    void* device_field(Component c, int* elem_bytes = nullptr, double* scale = nullptr) {
+        const int id = static_cast<int>(c);
+        if (id < 0 || id > 8) throw std::logic_error("ERROR: Invalid field component");
+        flush();
+        void* p = nullptr;
+        check_status(fdtd_device_field(grid_, id, &p, elem_bytes, scale));
+        device_mode_ = true;
+        for (auto& f : fields_) f.invalidate();
+        return p;
+    }
+    // Typed form: throws std::runtime_error unless T has the element size.
+    template <typename T>
+    T* device_field_as(Component c) {
+        int eb = 0;
+        void* p = device_field(c, &eb);
+        if (static_cast<std::size_t>(eb) != sizeof(T))
+            throw std::runtime_error("device_field: elements are " + std::to_string(eb) +
+                                     " bytes, not " + std::to_string(sizeof(T)));
+        return static_cast<T*>(p);
+    }
+    // The grid's HIP stream (hipStream_t): the library queues its device
+    // work there in call order.
+    void* stream() {
+        void* s = nullptr;
+        check_status(fdtd_get_stream(grid_, &s));
+        return s;
     }
 
     // Runs the deferred steps now.
@@ -218,7 +267,7 @@ public:
         w_index_.clear();
         w_comp_.clear();
         w_value_.clear();
-        for (int c = 0; c < 6; ++c) fields_[c].invalidate();
+        invalidate_mirrors();
         check_status(rc);
         check_status(rc2);
     }
@@ -249,8 +298,8 @@ public:
         std::vector<long long> index;
         std::vector<int> comp;
         for (const auto& p : cells) {
-            index.push_back(p.i + static_cast<long long>(parameters_.Ni) *
-                                      (p.j + static_cast<long long>(parameters_.Nj) * p.k));
+            index.push_back(p.i + static_cast<long long>(parameters.Ni) *
+                                      (p.j + static_cast<long long>(parameters.Nj) * p.k));
             comp.push_back(static_cast<int>(p.c));
         }
         check_status(fdtd_set_probes(grid_, static_cast<int>(cells.size()), index.data(),
@@ -308,19 +357,21 @@ public:
 
     std::size_t cells() const { return cells_; }
     fdtd_grid* handle() { return grid_; }
-    const Parameters& parameters() const { return parameters_; }
-    FP dt() const { return dt_; }
+    const Parameters& get_parameters() const { return parameters; }
+    FP get_dt() const { return dt; }
 
     void synchronize() {
         run_pending();
         check_status(fdtd_synchronize(grid_));
+        if (device_mode_)
+            for (auto& f : fields_) f.invalidate();
     }
 
     // FP64 copy of plane k of a component (Ni*Nj values, index i + j*Ni),
     // after pending host writes are uploaded.
     std::vector<double> get_plane(Component c, int k) {
         flush();
-        std::vector<double> out(static_cast<std::size_t>(parameters_.Ni) * parameters_.Nj);
+        std::vector<double> out(static_cast<std::size_t>(parameters.Ni) * parameters.Nj);
         check_status(fdtd_download_plane(grid_, static_cast<int>(c), k, out.data(), out.size()));
         return out;
     }
@@ -328,22 +379,82 @@ public:
 protected:
     // pml_percent < 0: periodic grid (fdtd_create); otherwise the PML
     // subclass's grid (fdtd_create_pml).
-    FDTD(Parameters parameters, FP dt, Precision precision, FP pml_percent)
-        : parameters_(parameters), dt_(dt) {
-        fdtd_parameters p{parameters.Ni, parameters.Nj, parameters.Nk, parameters.ax,
-                          parameters.bx, parameters.ay, parameters.by, parameters.az,
-                          parameters.bz, parameters.dx, parameters.dy, parameters.dz};
+    FDTD(Parameters params, FP time_step, Precision precision, FP pml_percent)
+        : fields_(make_fields(this)),
+          Jx(fields_[6]), Jy(fields_[7]), Jz(fields_[8]),
+          Ex(fields_[0]), Ey(fields_[1]), Ez(fields_[2]),
+          Bx(fields_[3]), By(fields_[4]), Bz(fields_[5]),
+          parameters(params) {
+        fdtd_parameters p{params.Ni, params.Nj, params.Nk, params.ax, params.bx, params.ay,
+                          params.by, params.az, params.bz, params.dx, params.dy, params.dz};
         if (pml_percent < 0)
-            check_status(fdtd_create(&p, dt, static_cast<int>(precision), &grid_));
+            check_status(fdtd_create(&p, time_step, static_cast<int>(precision), &grid_));
         else
-            check_status(fdtd_create_pml(&p, dt, pml_percent, static_cast<int>(precision), &grid_));
-        cells_ = static_cast<std::size_t>(parameters.Ni) * parameters.Nj * parameters.Nk;
-        fields_.reserve(9);
-        for (int c = 0; c < 9; ++c) fields_.emplace_back(this, c);
+            check_status(
+                fdtd_create_pml(&p, time_step, pml_percent, static_cast<int>(precision), &grid_));
+        cells_ = static_cast<std::size_t>(params.Ni) * params.Nj * params.Nk;
+        // the reference's protected members, src/FDTD/FDTD.cpp:35-60
+        Ni = params.Ni;
+        Nj = params.Nj;
+        Nk = params.Nk;
+        dx = params.dx;
+        dy = params.dy;
+        dz = params.dz;
+        dt = time_step;
+        const FP cdt = FDTD_const::C * dt;
+        coef_E_dx = cdt / dx;
+        coef_E_dy = cdt / dy;
+        coef_E_dz = cdt / dz;
+        coef_B_dx = cdt / (2.0 * dx);
+        coef_B_dy = cdt / (2.0 * dy);
+        coef_B_dz = cdt / (2.0 * dz);
+        cur_coef = -4.0 * FDTD_const::PI * dt;
+        begin_main_i = 0;
+        begin_main_j = 0;
+        begin_main_k = 0;
+        end_main_i = Ni;
+        end_main_j = Nj;
+        end_main_k = Nk;
     }
+
+    // include/FDTD/FDTD.h:24-31: -1 -> N-1, N -> 0, otherwise unchanged.
+    static void applyPeriodicBoundary(int& i, const int& N) {
+        if (i < 0) i = N - 1;
+        else if (i == N) i = 0;
+    }
+
+    // void update_E() / void update_B() -- include/FDTD/FDTD.h:33-34,
+    // src/FDTD/FDTD.cpp:63-130: one E step (J paired Jx->Ex, Jy->Ey, Jz->Ez
+    // as include/FDTD_kokkos/kokkos_functors.h:64-90) / one B half step
+    // over the main box, on the device and in place; deferred steps and
+    // host-mirror writes go first.
+    void update_E() { box_update(true); }
+    void update_B() { box_update(false); }
 
 private:
     friend class Field;
+
+    static std::vector<Field> make_fields(FDTD* owner) {
+        std::vector<Field> f;
+        f.reserve(9);
+        for (int c = 0; c < 9; ++c) f.emplace_back(owner, c);
+        return f;
+    }
+
+    void box_update(bool e_step) {
+        flush();
+        const int lo[3] = {begin_main_i, begin_main_j, begin_main_k};
+        const int hi[3] = {end_main_i, end_main_j, end_main_k};
+        check_status(e_step ? fdtd_update_e_box(grid_, lo, hi) : fdtd_update_b_box(grid_, lo, hi));
+        invalidate_mirrors();
+    }
+
+    // E/B mirrors after device work; J too once the caller holds device
+    // addresses.
+    void invalidate_mirrors() {
+        for (int c = 0; c < 9; ++c)
+            if (c < 6 || device_mode_) fields_[c].invalidate();
+    }
 
     static constexpr int kMaxPending = 256;
 
@@ -374,17 +485,31 @@ private:
     }
 
     int pending_ = 0;
+    bool device_mode_ = false;
     std::size_t nprobes_ = 0;
     std::vector<int> w_counts_;  // writes before deferred step s; the last: after them
     std::vector<long long> w_index_;
     std::vector<int> w_comp_;
     std::vector<double> w_value_;
 
-    Parameters parameters_;
-    FP dt_;
     fdtd_grid* grid_ = nullptr;
     std::size_t cells_ = 0;
-    std::vector<Field> fields_;
+    std::vector<Field> fields_;  // before the references below, which bind to it
+
+protected:
+    // The reference's protected data, include/FDTD/FDTD.h:10-22.  Field
+    // references are the host mirrors get_field() returns.
+    Field &Jx, &Jy, &Jz;
+    Field &Ex, &Ey, &Ez;
+    Field &Bx, &By, &Bz;
+    Parameters parameters;
+    int Ni = 0, Nj = 0, Nk = 0;
+    double dx = 0, dy = 0, dz = 0, dt = 0;
+    double cur_coef = 0;
+    double coef_E_dx = 0, coef_E_dy = 0, coef_E_dz = 0;
+    double coef_B_dx = 0, coef_B_dy = 0, coef_B_dz = 0;
+    int begin_main_i = 0, begin_main_j = 0, begin_main_k = 0;
+    int end_main_i = 0, end_main_j = 0, end_main_k = 0;
 };
 
 // FDTD_PML(Parameters, FP dt, FP pml_percent) -- include/FDTD/FDTD_PML.h:10-44,
@@ -393,9 +518,18 @@ private:
 // pml_percent must lie in [0, 0.5] (std::runtime_error otherwise).
 class FDTD_PML : public FDTD {
 public:
-    FDTD_PML(Parameters parameters, FP dt, FP pml_percent,
+    FDTD_PML(Parameters params, FP time_step, FP pml_percent,
              Precision precision = Precision::FP64)
-        : FDTD(parameters, dt, precision, pml_percent) {}
+        : FDTD(params, time_step, precision, pml_percent) {
+        // main box = interior, src/FDTD/FDTD_PML.cpp:259-268
+        const std::array<int, 3> p = pml_widths();
+        begin_main_i = p[0];
+        begin_main_j = p[1];
+        begin_main_k = p[2];
+        end_main_i = Ni - p[0];
+        end_main_j = Nj - p[1];
+        end_main_k = Nk - p[2];
+    }
 
     using FDTD::update_fields;
     // void update_fields() override -- src/FDTD/FDTD_PML.cpp:343-365
@@ -420,7 +554,7 @@ private:
 // Ni values of that row, ';'-separated, in %.9g with a decimal comma.
 inline void write_csv_slice(FDTD& m, Component c, int k, const std::string& path) {
     const std::vector<double> v = m.get_plane(c, k);
-    const int Ni = m.parameters().Ni, Nj = m.parameters().Nj;
+    const int Ni = m.get_parameters().Ni, Nj = m.get_parameters().Nj;
     std::FILE* f = std::fopen(path.c_str(), "w");
     if (!f) throw std::runtime_error("cannot write " + path);
     for (int i = 0; i < Ni; ++i) std::fprintf(f, i ? ";%d" : "%d", i);

@@ -142,8 +142,54 @@ int fdtd_download(fdtd_grid* grid, int component, double* host, size_t count);
 int fdtd_download_plane(fdtd_grid* grid, int component, int k, double* host, size_t count);
 
 /* zeroed_currents() -- src/FDTD/FDTD.cpp:132-136: J := 0 (dense J and the
- * values set by fdtd_set_current_values; point sources are kept). */
+ * values set by fdtd_set_current_values; point sources are kept).  Once
+ * J's device addresses are handed out (fdtd_device_field), the arrays are
+ * cleared in place, queued on the grid's stream, as the Kokkos class's
+ * deep_copy(J, 0) (src/FDTD_kokkos/FDTD_kokkos.cpp:69-73). */
 int fdtd_zero_currents(fdtd_grid* grid);
+
+/* Device side of get_field(Component) -- the Kokkos class returns device
+ * Views (include/FDTD_kokkos/FDTD_kokkos.h:31, src/FDTD_kokkos/FDTD_kokkos.cpp:75-88)
+ * that its driver fills in its own device loop
+ * (perf-tests/kokkos_sample/kokkos_sample.cpp:82-108): the device address
+ * of component 0..8's array of Ni*Nj*Nk elements, index i + j*Ni + k*Ni*Nj,
+ * for the caller's own GPU kernels.
+ *   J (6..8): elements of *elem_bytes = 8 (double) on FP64 grids, 4 (float)
+ *     otherwise, physical units (*scale = 1).  J becomes dense (zeroed, the
+ *     single-cell values set so far copied in) and stays dense at the same
+ *     addresses for the grid's life: the steps read it, fdtd_zero_currents
+ *     clears it in place.  Dense J turns temporal blocking off.
+ *   E, B (0..5): elements of the storage type (8 double, 4 float, 2 _Float16
+ *     holding scale * physical value).  The address holds the state between
+ *     steps and is valid until the next stepping call (sweeps alternate
+ *     between two buffers): fetch it again after stepping.  Each fetch drops
+ *     the carried half step, since the caller may change E or B.
+ * Ordering: the grid's work is queued on its own stream (fdtd_get_stream).
+ * Launch on that stream, or synchronize: the grid's stream
+ * (fdtd_synchronize) before touching the arrays, and your own work before
+ * the next call on the grid. */
+int fdtd_device_field(fdtd_grid* grid, int component, void** ptr, int* elem_bytes,
+                      double* scale);
+/* The grid's HIP stream (a hipStream_t) as void*: every call on the grid
+ * queues its device work there, in call order. */
+int fdtd_get_stream(fdtd_grid* grid, void** stream);
+
+/* The reference's protected curl updates, for subclasses that compose their
+ * own update_fields() (FDTD_PML::update_fields, src/FDTD/FDTD_PML.cpp:343-365
+ * and src/FDTD_kokkos/FDTD_PML_kokkos.cpp:160-198):
+ *   fdtd_update_b_box: update_B() -- src/FDTD/FDTD.cpp:99-130 (one B half
+ *     step, B += c*dt/(2d) * (-curl E));
+ *   fdtd_update_e_box: update_E() -- src/FDTD/FDTD.cpp:63-97 with the
+ *     Kokkos J pairing Jx->Ex, Jy->Ey, Jz->Ez
+ *     (include/FDTD_kokkos/kokkos_functors.h:64-90): one E step, with dense
+ *     or single-cell J and the point sources of the current step;
+ * over the cells [lo, hi) (the begin_main_* .. end_main_* bounds,
+ * include/FDTD/FDTD.h:17-18), neighbours wrapped over the whole grid, in
+ * place on E and B only (a PML grid's split fields are not touched).
+ * Neither advances the step counter.  0 <= lo <= hi <= N per axis; an
+ * empty box does nothing.  Queued on the grid's stream without a wait. */
+int fdtd_update_b_box(fdtd_grid* grid, const int lo[3], const int hi[3]);
+int fdtd_update_e_box(fdtd_grid* grid, const int lo[3], const int hi[3]);
 
 /* Sparse form of get_field(JX|JY|JZ)[index] = value (the drivers'
  * per-step box source, perf-tests/sample/sample.cpp:67-81): sets J at n

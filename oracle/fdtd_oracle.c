@@ -114,6 +114,22 @@ static void update_E_box(int Ni, int Nj, int Nk, const double c[7], const int bo
     }
 }
 
+/* The box forms themselves (the reference's protected update_B/update_E
+ * over begin_main_* .. end_main_*, src/FDTD/FDTD.cpp:63-130);
+ * box = {i0, i1, j0, j1, k0, k1}. */
+void oracle_update_B_box(int Ni, int Nj, int Nk, const double c[7], const int box[6],
+                         const double* Ex, const double* Ey, const double* Ez,
+                         double* Bx, double* By, double* Bz) {
+    update_B_box(Ni, Nj, Nk, c, box, Ex, Ey, Ez, Bx, By, Bz);
+}
+
+void oracle_update_E_box(int Ni, int Nj, int Nk, const double c[7], const int box[6],
+                         double* Ex, double* Ey, double* Ez,
+                         const double* Bx, const double* By, const double* Bz,
+                         const double* Jx, const double* Jy, const double* Jz) {
+    update_E_box(Ni, Nj, Nk, c, box, Ex, Ey, Ez, Bx, By, Bz, Jx, Jy, Jz);
+}
+
 void oracle_update_B(int Ni, int Nj, int Nk, const double c[7],
                      const double* Ex, const double* Ey, const double* Ez,
                      double* Bx, double* By, double* Bz) {

@@ -50,6 +50,9 @@ def lib():
         L.oracle_coefficients.argtypes = [ctypes.c_double] * 4 + [_dp]
         L.oracle_update_B.argtypes = [ctypes.c_int] * 3 + [_dp] * 7
         L.oracle_update_E.argtypes = [ctypes.c_int] * 3 + [_dp] * 10
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.oracle_update_B_box.argtypes = [ctypes.c_int] * 3 + [_dp, _ip] + [_dp] * 6
+        L.oracle_update_E_box.argtypes = [ctypes.c_int] * 3 + [_dp, _ip] + [_dp] * 9
         L.oracle_update_fields.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double] * 4
                                            + [ctypes.c_int] + [ctypes.POINTER(_dp)] * 3)
         L.oracle_waveform.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6
@@ -83,6 +86,8 @@ def ref_lib():
         L.ref_create.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 10
         L.ref_create.restype = ctypes.c_void_p
         L.ref_destroy.argtypes = [ctypes.c_void_p]
+        L.ref_update_B_box.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
+        L.ref_update_E_box.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]
         L.ref_set.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_longlong]
         L.ref_get.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_longlong]
         L.ref_set_value.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong,
@@ -139,6 +144,21 @@ class OracleGrid:
         c = coefficients(self.dx, self.dy, self.dz, self.dt)
         lib().oracle_update_E(self.Ni, self.Nj, self.Nk, _ptr(c), *[_ptr(a) for a in self.E],
                               *[_ptr(a) for a in self.B], *[_ptr(a) for a in self.J])
+
+    def update_B_box(self, lo, hi):
+        """update_B over the cells [lo, hi) (lo, hi: (i, j, k))."""
+        c = coefficients(self.dx, self.dy, self.dz, self.dt)
+        box = (ctypes.c_int * 6)(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+        lib().oracle_update_B_box(self.Ni, self.Nj, self.Nk, _ptr(c), box,
+                                  *[_ptr(a) for a in self.E], *[_ptr(a) for a in self.B])
+
+    def update_E_box(self, lo, hi):
+        """update_E over the cells [lo, hi)."""
+        c = coefficients(self.dx, self.dy, self.dz, self.dt)
+        box = (ctypes.c_int * 6)(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2])
+        lib().oracle_update_E_box(self.Ni, self.Nj, self.Nk, _ptr(c), box,
+                                  *[_ptr(a) for a in self.E], *[_ptr(a) for a in self.B],
+                                  *[_ptr(a) for a in self.J])
 
     def run_point_sources(self, sources, nsteps):
         """Advance nsteps with point sources (list of dicts, see source_array)."""
@@ -258,6 +278,17 @@ class RefGrid:
 
     def zeroed_currents(self):
         ref_lib().ref_zeroed_currents(self.h)
+
+    def update_B_box(self, lo, hi):
+        """The class's protected update_B() over the cells [lo, hi)."""
+        ref_lib().ref_update_B_box(self.h, (ctypes.c_int * 6)(lo[0], hi[0], lo[1], hi[1],
+                                                               lo[2], hi[2]))
+
+    def update_E_box(self, lo, hi):
+        """The class's protected update_E() over [lo, hi) (Jx feeds all three
+        E components there, src/FDTD/FDTD.cpp:85-93)."""
+        ref_lib().ref_update_E_box(self.h, (ctypes.c_int * 6)(lo[0], hi[0], lo[1], hi[1],
+                                                               lo[2], hi[2]))
 
 
 class RefPmlGrid(RefGrid):

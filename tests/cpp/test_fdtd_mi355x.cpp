@@ -5,6 +5,7 @@
 // class's error behaviour (src/FDTD/FDTD.cpp:5-7, :149).
 //
 // Usage: test_fdtd_mi355x [fp64|fp32]   (exit status 0 = all passed)
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -378,6 +379,95 @@ void test_pml_class() {
     EXPECT_TRUE(thrown);
 }
 
+// A subclass written against the reference's protected interface
+// (include/FDTD/FDTD.h:10-34), composing update_fields() from update_B() and
+// update_E() as FDTD_PML does (src/FDTD/FDTD_PML.cpp:343-365).
+class ThreeSweep : public FDTD {
+public:
+    ThreeSweep(Parameters p, FP dt, Precision prec) : FDTD(p, dt, prec) {}
+    void update_fields() override {
+        update_B();
+        update_E();
+        update_B();
+    }
+    bool members_ok(const Parameters& p, FP time_step) {
+        const FP cdt = FDTD_const::C * time_step;  // src/FDTD/FDTD.cpp:43-53
+        int a = -1, b = Ni, c = 3;
+        applyPeriodicBoundary(a, Ni);
+        applyPeriodicBoundary(b, Ni);
+        applyPeriodicBoundary(c, Ni);
+        return Ni == p.Ni && Nj == p.Nj && Nk == p.Nk && dx == p.dx && dt == time_step &&
+               coef_E_dy == cdt / p.dy && coef_B_dz == cdt / (2.0 * p.dz) &&
+               cur_coef == -4.0 * FDTD_const::PI * time_step && begin_main_i == 0 &&
+               end_main_k == p.Nk && a == Ni - 1 && b == 0 && c == 3 &&
+               &Ex == &get_field(Component::EX) && &Jz == &get_field(Component::JZ) &&
+               parameters.Nj == p.Nj;
+    }
+    // update_B() over a caller-chosen main box
+    void half_b_in(int i0, int i1) {
+        begin_main_i = i0;
+        end_main_i = i1;
+        update_B();
+        begin_main_i = 0;
+        end_main_i = Ni;
+    }
+};
+
+class InteriorPml : public FDTD_PML {
+public:
+    InteriorPml(Parameters p, FP dt, Precision prec) : FDTD_PML(p, dt, 0.25, prec) {}
+    std::array<int, 6> main_box() const {
+        return {begin_main_i, end_main_i, begin_main_j, end_main_j, begin_main_k, end_main_k};
+    }
+};
+
+void test_subclass_hooks() {
+    // update_B(); update_E(); update_B() in a subclass equals the class's
+    // own steps bit for bit, with single-cell J written between steps
+    // through the protected Field references; the protected members hold
+    // the reference's values; FDTD_PML's main box is its interior
+    // (src/FDTD/FDTD_PML.cpp:259-268).
+    Parameters p{40, 24, 16, 0, 1, 0, 1, 0, 1, 1.0, 1.3, 0.7};
+    const FP dt = 0.4 / (FDTD_const::C * std::sqrt(3.0));
+    ThreeSweep s(p, dt, g_precision);
+    FDTD f(p, dt, g_precision);
+    EXPECT_TRUE(s.members_ok(p, dt));
+    for (FDTD* g : {static_cast<FDTD*>(&s), &f}) {
+        Field& ex = g->get_field(Component::EX);
+        Field& bz = g->get_field(Component::BZ);
+        for (int i = 0; i < (int)g->cells(); i += 7) {
+            ex[i] = 1e-3 * std::cos(0.1 * i);
+            bz[i] = 2e-3 * std::sin(0.07 * i);
+        }
+    }
+    const int centre = 20 + 12 * 40 + 8 * 40 * 24;
+    for (int t = 0; t < 9; ++t) {
+        for (FDTD* g : {static_cast<FDTD*>(&s), &f}) {
+            g->get_field(Component::JX)[centre] = std::sin(0.3 * (t + 1));
+            g->get_field(Component::JZ)[centre + 1] = std::cos(0.2 * (t + 1));
+        }
+        s.update_fields();
+        f.update_fields();
+    }
+    bool same = true;
+    for (int c = 0; c < 6; ++c)
+        for (int i = 0; i < (int)f.cells(); ++i)
+            same = same && (double)s.get_field(static_cast<Component>(c))[i] ==
+                               (double)f.get_field(static_cast<Component>(c))[i];
+    EXPECT_TRUE(same);
+    // a box update leaves the cells outside the box alone
+    const double outside = s.get_field(Component::BX)[0 + 3 * 40];
+    const double inside = s.get_field(Component::BX)[20 + 3 * 40 + 8 * 40 * 24];
+    s.half_b_in(10, 30);
+    EXPECT_TRUE((double)s.get_field(Component::BX)[0 + 3 * 40] == outside);
+    EXPECT_TRUE((double)s.get_field(Component::BX)[20 + 3 * 40 + 8 * 40 * 24] != inside);
+    InteriorPml m(p, dt, g_precision);
+    const auto b = m.main_box();
+    const auto w = m.pml_widths();
+    EXPECT_TRUE(b[0] == w[0] && b[1] == p.Ni - w[0] && b[2] == w[1] && b[3] == p.Nj - w[1] &&
+                b[4] == w[2] && b[5] == p.Nk - w[2]);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -399,7 +489,8 @@ int main(int argc, char** argv) {
     test_time_blocking();
     test_deferred_steps();
     test_probes_and_energy();
-    ran += 7;
+    test_subclass_hooks();
+    ran += 8;
     std::printf("%d tests, %d failures\n", ran, g_failures);
     return g_failures == 0 ? 0 : 1;
 }

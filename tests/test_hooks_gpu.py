@@ -1,0 +1,240 @@
+"""Subclass hooks and device-side field access, against the oracle.
+
+* The reference's protected update_B() / update_E() over begin_main_* ..
+  end_main_* (include/FDTD/FDTD.h:17-18,33-34, src/FDTD/FDTD.cpp:63-130),
+  which FDTD_PML::update_fields composes (src/FDTD/FDTD_PML.cpp:343-365):
+  ``update_B_box`` / ``update_E_box`` bit-exact against the oracle's box
+  updates (pinned to the compiled reference class on the CPU).
+* Device arrays (the Kokkos class's get_field returns device Views,
+  include/FDTD_kokkos/FDTD_kokkos.h:31, filled on the device by its driver,
+  perf-tests/kokkos_sample/kokkos_sample.cpp:88-128): J written by torch on
+  the grid's stream equals J uploaded from the host, bit for bit; E/B
+  written on the device drop the carried half step.
+* The driver counterpart ``device_sample`` (its own HIP kernel writes J
+  before every step) against the host-J driver, and its transposed slice.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, courant_dt, random_state
+from fdtd_method_amd import FDTD, FDTD_PML, Component, Parameters, Precision
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SPACING = (1.0, 1.3, 0.7)
+
+
+def grid_pair(shape, seed, prec=Precision.FP64, pml=None):
+    n = int(np.prod(shape))
+    dt = courant_dt(*SPACING)
+    st = random_state(n, seed)
+    J = random_state(n, seed + 100, 3)
+    p = Parameters(*shape, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0, *SPACING)
+    g = FDTD(p, dt, prec) if pml is None else FDTD_PML(p, dt, pml, prec)
+    for c in range(6):
+        g.upload(c, st[c])
+    for c in range(3):
+        g.upload(6 + c, J[c])
+    o = O.OracleGrid(*shape, *SPACING, dt)
+    for c in range(6):
+        o.fields()[c][:] = st[c]
+    for c in range(3):
+        o.J[c][:] = J[c]
+    return g, o
+
+
+BOXES = [((40, 24, 16), (0, 0, 0), (40, 24, 16)), ((40, 24, 16), (5, 3, 2), (33, 20, 15)),
+         ((70, 9, 5), (64, 0, 0), (70, 9, 5)), ((33, 17, 12), (0, 16, 0), (33, 17, 12)),
+         ((16, 12, 10), (4, 4, 4), (4, 9, 9))]
+
+
+@pytest.mark.parametrize("shape,lo,hi", BOXES)
+def test_box_updates_match_oracle(shape, lo, hi):
+    """B, E, B, E, B over a box, with dense J on distinct components:
+    bitwise against the oracle after every call; cells outside untouched."""
+    g, o = grid_pair(shape, seed=sum(shape) + lo[0])
+    for step in ("B", "E", "B", "E", "B"):
+        getattr(g, f"update_{step}")(lo, hi)
+        getattr(o, f"update_{step}_box")(lo, hi)
+        for c in range(6):
+            assert np.array_equal(g.download(c), o.fields()[c]), (step, c)
+    assert g.step == 0
+    g.close()
+
+
+def test_box_hooks_compose_to_steps():
+    """update_B(); update_E(); update_B() on the whole grid == update_fields
+    (FP64 and FP32 bitwise; FP16 == its own three-sweep path), including
+    after fused steps that leave a carried half step."""
+    shape = (48, 20, 12)
+    for prec in (Precision.FP64, Precision.FP32, Precision.FP16):
+        a, _ = grid_pair(shape, seed=5, prec=prec)
+        b, _ = grid_pair(shape, seed=5, prec=prec)
+        if prec == Precision.FP16:
+            for g in (a, b):
+                g.set_scales([2.0 ** -1] * 6)
+        a.update_fields(3)
+        b.update_fields(3)
+        for _ in range(2):
+            a.update_B()
+            a.update_E()
+            a.update_B()
+            a.step = a.step + 1
+        if prec == Precision.FP16:
+            b.update_fields_3sweep(2)
+        else:
+            b.update_fields(2)
+        a.update_fields(4)
+        b.update_fields(4)
+        for c in range(6):
+            assert np.array_equal(a.download(c), b.download(c)), (prec, c)
+        a.close()
+        b.close()
+
+
+def test_pml_main_box_is_interior_and_matches_oracle():
+    """FDTD_PML's default update box is its interior
+    (src/FDTD/FDTD_PML.cpp:259-268); the interior curl update matches the
+    oracle's box update there, with the layer cells untouched."""
+    shape = (30, 20, 15)
+    g, o = grid_pair(shape, seed=11, pml=0.2)
+    p = g.pml_widths
+    assert g.begin_main == p and g.end_main == tuple(n - w for n, w in zip(shape, p))
+    g.update_B()
+    o.update_B_box(g.begin_main, g.end_main)
+    g.update_E()
+    o.update_E_box(g.begin_main, g.end_main)
+    for c in range(6):
+        assert np.array_equal(g.download(c), o.fields()[c]), c
+    g.close()
+
+
+def test_box_rejects_bad_boxes():
+    g, _ = grid_pair((8, 6, 4), seed=1)
+    for lo, hi in (((0, 0, 0), (9, 6, 4)), ((-1, 0, 0), (8, 6, 4)), ((3, 0, 0), (2, 6, 4))):
+        with pytest.raises(Exception):
+            g.update_B(lo, hi)
+    g.update_E((2, 0, 0), (2, 6, 4))  # empty box: nothing happens
+    g.close()
+
+
+def _torch():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible to torch")
+    return torch
+
+
+@pytest.mark.parametrize("prec", [Precision.FP64, Precision.FP32])
+def test_device_currents_equal_uploaded(prec):
+    """J written through torch on the grid's stream before every step equals
+    the same J uploaded from the host, bit for bit; zeroed_currents() clears
+    the device arrays in place and later device writes still act."""
+    torch = _torch()
+    shape = (40, 24, 16)
+    n = int(np.prod(shape))
+    a, _ = grid_pair(shape, seed=21, prec=prec)
+    b, _ = grid_pair(shape, seed=21, prec=prec)
+    jd = [torch.as_tensor(a.device_field(6 + c), device="cuda") for c in range(3)]
+    jt = torch.float64 if prec == Precision.FP64 else torch.float32
+    assert all(t.dtype == jt and t.numel() == n for t in jd)
+    rng = np.random.default_rng(4)
+    s = a.torch_stream()
+    for t in range(5):
+        vals = [rng.standard_normal(n) for _ in range(3)]
+        with torch.cuda.stream(s):
+            for c in range(3):
+                jd[c].copy_(torch.from_numpy(vals[c]).to(device="cuda", dtype=jt))
+        for c in range(3):
+            # FP32 grids hold J in float: the host upload rounds the same way
+            b.upload(6 + c, vals[c])
+        a.update_fields(1)
+        b.update_fields(1)
+    a.zeroed_currents()
+    b.zeroed_currents()
+    a.synchronize()
+    assert all(int(torch.count_nonzero(t)) == 0 for t in jd)
+    with torch.cuda.stream(s):
+        jd[2][123] = 0.5
+    b.set_current_values([123], [Component.JZ], [0.5])
+    a.update_fields(3)
+    b.update_fields(3)
+    for c in range(6):
+        assert np.array_equal(a.download(c), b.download(c)), c
+    assert np.array_equal(a.download(8), b.download(8))
+    a.close()
+    b.close()
+
+
+def test_device_fields_read_and_write():
+    """E/B device arrays hold the stored values (FP16: scale * value); a
+    write through them after fused steps (carried half step) equals the same
+    change uploaded from the host."""
+    torch = _torch()
+    shape = (48, 20, 12)
+    for prec in (Precision.FP32, Precision.FP16, Precision.FP64):
+        a, _ = grid_pair(shape, seed=9, prec=prec)
+        b, _ = grid_pair(shape, seed=9, prec=prec)
+        if prec == Precision.FP16:
+            for g in (a, b):
+                g.set_scales([2.0 ** -2] * 6)
+        a.update_fields(5)
+        b.update_fields(5)
+        arr = a.device_field(Component.BY)
+        t = torch.as_tensor(arr, device="cuda")
+        a.synchronize()
+        host = b.download(Component.BY)
+        got = t.double().cpu().numpy() / arr.scale
+        assert np.array_equal(got, host), prec
+        with torch.cuda.stream(a.torch_stream()):
+            t[7::11] = t[7::11] * 2
+        host[7::11] *= 2
+        b.upload(Component.BY, host)
+        a.update_fields(4)
+        b.update_fields(4)
+        for c in range(6):
+            assert np.array_equal(a.download(c), b.download(c)), (prec, c)
+        a.close()
+        b.close()
+
+
+def _slice(stdout):
+    lines = stdout.strip().splitlines()
+    assert lines[0].startswith("Execution time: ")
+    return np.array([[float(x) for x in ln.split()] for ln in lines[1:11]])
+
+
+def test_device_sample_driver_matches_host_driver(tmp_path):
+    """`device_sample 32 100` (J from the driver's own HIP kernel, the Kokkos
+    driver's loop) against `sample 32 100` (J written cell by cell from the
+    host, bit-identical to the reference class): Ex within 1e-12 relative
+    (device and host libm sin/cos/pow may differ in the last bit); the
+    printed slice is the Kokkos driver's transposed one (x index Nk/2, rows
+    j, columns k, kokkos_sample.cpp:141-149)."""
+    n, it = 32, 100
+    dev = os.path.join(ROOT, "fdtd_method_amd", "bin", "device_sample")
+    host = os.path.join(ROOT, "fdtd_method_amd", "bin", "sample")
+    d1, d2 = tmp_path / "dev.bin", tmp_path / "host.bin"
+    r1 = subprocess.run([dev, str(n), str(it), "--dump", str(d1)], capture_output=True,
+                        text=True, timeout=300)
+    assert r1.returncode == 0, r1.stdout + r1.stderr
+    r2 = subprocess.run([host, str(n), str(it), "--dump", str(d2)], capture_output=True,
+                        text=True, timeout=300)
+    assert r2.returncode == 0, r2.stderr
+    ex_dev, ex_host = np.fromfile(d1, dtype="<f8"), np.fromfile(d2, dtype="<f8")
+    err = np.abs(ex_dev - ex_host).max() / np.abs(ex_host).max()
+    print("device vs host current: max rel", err, "bitwise", np.array_equal(ex_dev, ex_host))
+    assert err <= 1e-12
+    cube = ex_dev.reshape(n, n, n)  # [k, j, i]
+    want = cube[n // 2 - 5:n // 2 + 5, n // 2 - 5:n // 2 + 5, n // 2].T  # rows j, cols k
+    np.testing.assert_allclose(_slice(r1.stdout), want, rtol=0, atol=5e-6)
+    r3 = subprocess.run([dev, str(n), str(it), "fp32"], capture_output=True, text=True,
+                        timeout=300)
+    assert r3.returncode == 0, r3.stderr
+    np.testing.assert_allclose(_slice(r3.stdout), want, rtol=0, atol=5e-5)
+    bad = subprocess.run([dev, "32"], capture_output=True, text=True, timeout=60)
+    assert bad.returncode == 1 and "ERROR: Incorrect number of parameters" in bad.stdout

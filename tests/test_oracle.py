@@ -56,6 +56,41 @@ def test_oracle_matches_compiled_reference(shape):
         assert np.array_equal(g.fields()[c], r.get(c)), f"component {c}"
 
 
+BOXES = [((16, 12, 10), (3, 2, 1), (13, 10, 9)), ((16, 12, 10), (0, 5, 0), (16, 12, 4)),
+         ((7, 5, 3), (6, 0, 2), (7, 5, 3)), ((5, 1, 2), (1, 0, 0), (4, 1, 2))]
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
+@pytest.mark.parametrize("shape,lo,hi", BOXES)
+def test_oracle_box_updates_match_compiled_reference(shape, lo, hi):
+    """The box updates (the reference's protected update_B / update_E over
+    begin_main_* .. end_main_*, reached through a subclass of the compiled
+    class) bit-exact, cells outside the box untouched, J = Jx = Jy = Jz."""
+    n = int(np.prod(shape))
+    spacing = (1.0, 1.3, 0.7)
+    dt = courant_dt(*spacing)
+    st = random_state(n, seed=n + 7)
+    Jc = np.random.default_rng(3).standard_normal(n)
+    g = make(shape, spacing, dt, st, [Jc, Jc, Jc])
+    r = O.RefGrid(*shape, *spacing, dt)
+    for c in range(6):
+        r.set(c, st[c])
+    for c in range(6, 9):
+        r.set(c, Jc)
+    inside = np.zeros(shape[::-1], dtype=bool)
+    inside[lo[2]:hi[2], lo[1]:hi[1], lo[0]:hi[0]] = True
+    inside = inside.reshape(-1)
+    for step in ("B", "E", "B", "E"):
+        before = [a.copy() for a in g.fields()[:6]]
+        getattr(g, f"update_{step}_box")(lo, hi)
+        getattr(r, f"update_{step}_box")(lo, hi)
+        for c in range(6):
+            assert np.array_equal(g.fields()[c], r.get(c)), f"{step} component {c}"
+            assert np.array_equal(g.fields()[c][~inside], before[c][~inside])
+        moved = [c for c in range(6) if not np.array_equal(g.fields()[c], before[c])]
+        assert moved == ([3, 4, 5] if step == "B" else [0, 1, 2])
+
+
 def test_oracle_matches_golden_random(golden_dir):
     f = np.load(os.path.join(golden_dir, "ref_16x12x10_random_50steps.npz"))
     shape = tuple(int(x) for x in f["dims"])
