@@ -353,7 +353,10 @@ class FDTD:
         hold the storage type (FP16: scale * value) and are valid until the
         next step.  The grid's work runs on ``torch_stream()``: do the
         writes there, or synchronize both ways.  From this call on, host
-        mirrors are refreshed after every stepping or synchronizing call."""
+        mirrors are refreshed after every stepping or synchronizing call.
+        torch brings its own HIP runtime: import torch and touch
+        ``torch.cuda`` before the first grid is created (started after this
+        library has launched work, it reports no device)."""
         try:
             comp = Component(component)
         except ValueError:

@@ -18,6 +18,21 @@ GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device 0)")
+    config.addinivalue_line("markers", "torch: uses torch's GPU tensors")
+
+
+def pytest_collection_modifyitems(config, items):
+    # torch ships its own HIP runtime; started after this library has run
+    # kernels in the process it reports no device.  GPU tests that hand the
+    # library's arrays to torch therefore start torch before any grid exists.
+    if "not gpu" in (config.option.markexpr or ""):
+        return
+    if any(item.get_closest_marker("torch") for item in items):
+        try:
+            import torch
+            torch.cuda.is_available()
+        except Exception:
+            pass
 
 
 def random_state(n, seed, count=6):

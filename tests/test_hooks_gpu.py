@@ -129,6 +129,7 @@ def _torch():
     return torch
 
 
+@pytest.mark.torch
 @pytest.mark.parametrize("prec", [Precision.FP64, Precision.FP32])
 def test_device_currents_equal_uploaded(prec):
     """J written through torch on the grid's stream before every step equals
@@ -170,6 +171,7 @@ def test_device_currents_equal_uploaded(prec):
     b.close()
 
 
+@pytest.mark.torch
 def test_device_fields_read_and_write():
     """E/B device arrays hold the stored values (FP16: scale * value); a
     write through them after fused steps (carried half step) equals the same
