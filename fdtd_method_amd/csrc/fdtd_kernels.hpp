@@ -49,6 +49,22 @@ template <typename C> struct Coefs {
     C bx_ey, bx_ez, by_ez, by_ex, bz_ex, bz_ey;
 };
 
+// a - k*x, the second curl term of every E and B update.  Float compute
+// (FP32 and FP16 storage) forms it with one fused multiply-add (one
+// rounding); double keeps the separate multiply and subtract of the
+// reference's CPU code (this file is compiled with -ffp-contract=off), so
+// FP64 stays bit-identical to it.  Every kernel forms the terms through
+// this, so all float paths (three sweeps, fused and blocked sweeps,
+// strips, batches, PML) stay bit-identical to each other.
+template <typename C>
+__device__ __forceinline__ C sub_mul(C a, C k, C x) {
+    return a - k * x;
+}
+template <>
+__device__ __forceinline__ float sub_mul<float>(float a, float k, float x) {
+    return __builtin_fmaf(-k, x, a);
+}
+
 // A point current source.  Its J component is evaluated in FP64 at
 // t = (n+1)*dt before step n (perf-tests/sample/sample.cpp:74):
 //   kind 0: amp
@@ -233,9 +249,9 @@ __device__ __forceinline__ void plain_B_cell(const Fields<S>& f, const Coefs<C>&
     const int j_next = i + wrap(j + 1, Nj) * Ni + k * plane;
     const int k_next = i + j * Ni + wrap(k + 1, Nk) * plane;
     const C ex = f.e[0][idx], ey = f.e[1][idx], ez = f.e[2][idx];
-    const C dbx = c.bx_ey * (C(f.e[1][k_next]) - ey) - c.bx_ez * (C(f.e[2][j_next]) - ez);
-    const C dby = c.by_ez * (C(f.e[2][i_next]) - ez) - c.by_ex * (C(f.e[0][k_next]) - ex);
-    const C dbz = c.bz_ex * (C(f.e[0][j_next]) - ex) - c.bz_ey * (C(f.e[1][i_next]) - ey);
+    const C dbx = sub_mul(c.bx_ey * (C(f.e[1][k_next]) - ey), c.bx_ez, C(f.e[2][j_next]) - ez);
+    const C dby = sub_mul(c.by_ez * (C(f.e[2][i_next]) - ez), c.by_ex, C(f.e[0][k_next]) - ex);
+    const C dbz = sub_mul(c.bz_ex * (C(f.e[0][j_next]) - ex), c.bz_ey, C(f.e[1][i_next]) - ey);
     f.b[0][idx] = static_cast<S>(C(f.b[0][idx]) + dbx);
     f.b[1][idx] = static_cast<S>(C(f.b[1][idx]) + dby);
     f.b[2][idx] = static_cast<S>(C(f.b[2][idx]) + dbz);
@@ -386,9 +402,9 @@ __device__ __forceinline__ void plain_E_cell(const Fields<S>& f, const JT* __res
         t[2][0] = c.jz * C(jz[idx]) + t[2][0];
     }
     if (nsrc > 0) add_point_sources<C, 1>(t, c, src, vals, 0, nsrc, i, j, k);
-    const C ex = C(f.e[0][idx]) + (t[0][0] - c.ex_by * (by - C(f.b[1][k_pred])));
-    const C ey = C(f.e[1][idx]) + (t[1][0] - c.ey_bz * (bz - C(f.b[2][i_pred])));
-    const C ez = C(f.e[2][idx]) + (t[2][0] - c.ez_bx * (bx - C(f.b[0][j_pred])));
+    const C ex = C(f.e[0][idx]) + sub_mul(t[0][0], c.ex_by, by - C(f.b[1][k_pred]));
+    const C ey = C(f.e[1][idx]) + sub_mul(t[1][0], c.ey_bz, bz - C(f.b[2][i_pred]));
+    const C ez = C(f.e[2][idx]) + sub_mul(t[2][0], c.ez_bx, bx - C(f.b[0][j_pred]));
     f.e[0][idx] = static_cast<S>(ex);
     f.e[1][idx] = static_cast<S>(ey);
     f.e[2][idx] = static_cast<S>(ez);
@@ -867,11 +883,11 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
             }
 #pragma unroll
             for (int v = 0; v < V; ++v) {
-                e1[0][v] = en[0][v] + (t[0][v] - c.ex_by * (bn[1][v] - bP[1][v]));
-                e1[2][v] = en[2][v] + (t[2][v] - c.ez_bx * (bn[0][v] - sBx[r * Ni + ib + v]));
+                e1[0][v] = en[0][v] + sub_mul(t[0][v], c.ex_by, bn[1][v] - bP[1][v]);
+                e1[2][v] = en[2][v] + sub_mul(t[2][v], c.ez_bx, bn[0][v] - sBx[r * Ni + ib + v]);
                 if (own) {
                     const C bz_im = (v > 0) ? bn[2][v - 1] : sBz[(r + 1) * Ni + im0];
-                    e1[1][v] = en[1][v] + (t[1][v] - c.ey_bz * (bn[2][v] - bz_im));
+                    e1[1][v] = en[1][v] + sub_mul(t[1][v], c.ey_bz, bn[2][v] - bz_im);
                 } else {
                     e1[1][v] = C(0);
                 }
@@ -934,9 +950,9 @@ __device__ __forceinline__ void fused_tile(const SweepArgs<S, C, JT>& a, int til
                     const C ez_ip = (v < V - 1) ? eP[2][v + 1] : sEz[r * Ni + ipl];
                     const C ex_jp = sEx[r * Ni + ib + v];
                     const C ez_jp = sEz[(r + 1) * Ni + ib + v];
-                    const C dbx = c.bx_ey * (e1[1][v] - eP[1][v]) - c.bx_ez * (ez_jp - eP[2][v]);
-                    const C dby = c.by_ez * (ez_ip - eP[2][v]) - c.by_ex * (e1[0][v] - eP[0][v]);
-                    const C dbz = c.bz_ex * (ex_jp - eP[0][v]) - c.bz_ey * (ey_ip - eP[1][v]);
+                    const C dbx = sub_mul(c.bx_ey * (e1[1][v] - eP[1][v]), c.bx_ez, ez_jp - eP[2][v]);
+                    const C dby = sub_mul(c.by_ez * (ez_ip - eP[2][v]), c.by_ex, e1[0][v] - eP[0][v]);
+                    const C dbz = sub_mul(c.bz_ex * (ex_jp - eP[0][v]), c.bz_ey, ey_ip - eP[1][v]);
                     nb[0][v] = bP[0][v] + dbx;
                     nb[1][v] = bP[1][v] + dby;
                     nb[2][v] = bP[2][v] + dbz;
@@ -1301,9 +1317,9 @@ pml_e_phase(PmlPhaseArgs<S, C, JT> a) {
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     const C bz_im = (v > 0) ? bn[2][v - 1] : sBz[(r + 1) * Ni + im0];
-                    e1[0][v] = en[0][v] + (t[0][v] - c.ex_by * (bn[1][v] - bP[1][v]));
-                    e1[1][v] = en[1][v] + (t[1][v] - c.ey_bz * (bn[2][v] - bz_im));
-                    e1[2][v] = en[2][v] + (t[2][v] - c.ez_bx * (bn[0][v] - sBx[r * Ni + ib + v]));
+                    e1[0][v] = en[0][v] + sub_mul(t[0][v], c.ex_by, bn[1][v] - bP[1][v]);
+                    e1[1][v] = en[1][v] + sub_mul(t[1][v], c.ey_bz, bn[2][v] - bz_im);
+                    e1[2][v] = en[2][v] + sub_mul(t[2][v], c.ez_bx, bn[0][v] - sBx[r * Ni + ib + v]);
                 }
             }
             if (!mall) {
@@ -1482,9 +1498,9 @@ pml_b_phase(PmlPhaseArgs<S, C, JT> a) {
                     const C ez_ip = (v < V - 1) ? en[2][v + 1] : sEz[r * Ni + ipl];
                     const C ex_jp = sEx[(r + 1) * Ni + ib + v];
                     const C ez_jp = sEz[(r + 1) * Ni + ib + v];
-                    const C dbx = c.bx_ey * (eN[1][v] - en[1][v]) - c.bx_ez * (ez_jp - en[2][v]);
-                    const C dby = c.by_ez * (ez_ip - en[2][v]) - c.by_ex * (eN[0][v] - en[0][v]);
-                    const C dbz = c.bz_ex * (ex_jp - en[0][v]) - c.bz_ey * (ey_ip - en[1][v]);
+                    const C dbx = sub_mul(c.bx_ey * (eN[1][v] - en[1][v]), c.bx_ez, ez_jp - en[2][v]);
+                    const C dby = sub_mul(c.by_ez * (ez_ip - en[2][v]), c.by_ex, eN[0][v] - en[0][v]);
+                    const C dbz = sub_mul(c.bz_ex * (ex_jp - en[0][v]), c.bz_ey, ey_ip - en[1][v]);
                     nb[0][v] = bn[0][v] + dbx;
                     nb[1][v] = bn[1][v] + dby;
                     nb[2][v] = bn[2][v] + dbz;
@@ -1719,9 +1735,9 @@ __device__ __forceinline__ void blocked_tile(const BlockedArgs<S, C>& a, int tx,
                 for (int v = 0; v < V; ++v) {
                     const C bx_jm = Bx_[rm * TXg + cl + v];
                     const C bz_im = (v > 0) ? bin[2][v - 1] : Bz_[r * TXg + cm];
-                    enew[l][0][v] = ein[0][v] + (t[0][v] - c.ex_by * (bin[1][v] - bP[l][1][v]));
-                    enew[l][1][v] = ein[1][v] + (t[1][v] - c.ey_bz * (bin[2][v] - bz_im));
-                    enew[l][2][v] = ein[2][v] + (t[2][v] - c.ez_bx * (bin[0][v] - bx_jm));
+                    enew[l][0][v] = ein[0][v] + sub_mul(t[0][v], c.ex_by, bin[1][v] - bP[l][1][v]);
+                    enew[l][1][v] = ein[1][v] + sub_mul(t[1][v], c.ey_bz, bin[2][v] - bz_im);
+                    enew[l][2][v] = ein[2][v] + sub_mul(t[2][v], c.ez_bx, bin[0][v] - bx_jm);
                 }
                 // B_l at plane pe-1 from E_l(pe-1) (eP, LDS) and E_l(pe) (enew)
                 C* Ex_ = sE(l, 0);
@@ -1733,9 +1749,9 @@ __device__ __forceinline__ void blocked_tile(const BlockedArgs<S, C>& a, int tx,
                     const C ez_ip = (v < V - 1) ? eP[l][2][v + 1] : Ez_[r * TXg + cp];
                     const C ex_jp = Ex_[rp * TXg + cl + v];
                     const C ez_jp = Ez_[rp * TXg + cl + v];
-                    const C dbx = c.bx_ey * (enew[l][1][v] - eP[l][1][v]) - c.bx_ez * (ez_jp - eP[l][2][v]);
-                    const C dby = c.by_ez * (ez_ip - eP[l][2][v]) - c.by_ex * (enew[l][0][v] - eP[l][0][v]);
-                    const C dbz = c.bz_ex * (ex_jp - eP[l][0][v]) - c.bz_ey * (ey_ip - eP[l][1][v]);
+                    const C dbx = sub_mul(c.bx_ey * (enew[l][1][v] - eP[l][1][v]), c.bx_ez, ez_jp - eP[l][2][v]);
+                    const C dby = sub_mul(c.by_ez * (ez_ip - eP[l][2][v]), c.by_ex, enew[l][0][v] - eP[l][0][v]);
+                    const C dbz = sub_mul(c.bz_ex * (ex_jp - eP[l][0][v]), c.bz_ey, ey_ip - eP[l][1][v]);
                     bnew[0][v] = bP[l][0][v] + dbx;
                     bnew[1][v] = bP[l][1][v] + dby;
                     bnew[2][v] = bP[l][2][v] + dbz;
@@ -2227,9 +2243,9 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                 for (int v = 0; v < V; ++v) {
                     const C bx_jm = r > 0 ? bin[0][r - 1][v] : sB(l, 0, wm)[v];
                     const C bz_im = v > 0 ? bin[2][r][v - 1] : lane_prev(bin[2][r][V - 1]);
-                    enew[0][r][v] = ein[0][r][v] + (t[0][v] - c.ex_by * (bin[1][r][v] - bPi[l][1][r][v]));
-                    enew[1][r][v] = ein[1][r][v] + (t[1][v] - c.ey_bz * (bin[2][r][v] - bz_im));
-                    enew[2][r][v] = ein[2][r][v] + (t[2][v] - c.ez_bx * (bin[0][r][v] - bx_jm));
+                    enew[0][r][v] = ein[0][r][v] + sub_mul(t[0][v], c.ex_by, bin[1][r][v] - bPi[l][1][r][v]);
+                    enew[1][r][v] = ein[1][r][v] + sub_mul(t[1][v], c.ey_bz, bin[2][r][v] - bz_im);
+                    enew[2][r][v] = ein[2][r][v] + sub_mul(t[2][v], c.ez_bx, bin[0][r][v] - bx_jm);
                 }
             }
             // B_l at plane pe-1 from E_l(pe-1) (eP) and E_l(pe) (enew)
@@ -2243,9 +2259,9 @@ __device__ __forceinline__ void strip_tile(const StripArgs<S, C>& a, const int t
                     const C ez_ip = v < V - 1 ? ePi[l][2][r][v + 1] : lane_next(ePi[l][2][r][0]);
                     const C ex_jp = r < R - 1 ? ePi[l][0][r + 1][v] : sE(l, 0, wp)[v];
                     const C ez_jp = r < R - 1 ? ePi[l][2][r + 1][v] : sE(l, 1, wp)[v];
-                    const C dbx = c.bx_ey * (enew[1][r][v] - ePi[l][1][r][v]) - c.bx_ez * (ez_jp - ePi[l][2][r][v]);
-                    const C dby = c.by_ez * (ez_ip - ePi[l][2][r][v]) - c.by_ex * (enew[0][r][v] - ePi[l][0][r][v]);
-                    const C dbz = c.bz_ex * (ex_jp - ePi[l][0][r][v]) - c.bz_ey * (ey_ip - ePi[l][1][r][v]);
+                    const C dbx = sub_mul(c.bx_ey * (enew[1][r][v] - ePi[l][1][r][v]), c.bx_ez, ez_jp - ePi[l][2][r][v]);
+                    const C dby = sub_mul(c.by_ez * (ez_ip - ePi[l][2][r][v]), c.by_ex, enew[0][r][v] - ePi[l][0][r][v]);
+                    const C dbz = sub_mul(c.bz_ex * (ex_jp - ePi[l][0][r][v]), c.bz_ey, ey_ip - ePi[l][1][r][v]);
                     bnew[0][r][v] = bPi[l][0][r][v] + dbx;
                     bnew[1][r][v] = bPi[l][1][r][v] + dby;
                     bnew[2][r][v] = bPi[l][2][r][v] + dbz;
