@@ -471,6 +471,7 @@ void test_subclass_hooks() {
 }  // namespace
 
 int main(int argc, char** argv) {
+    std::setvbuf(stdout, nullptr, _IOLBF, 0);  // each result line reaches a log at once
     if (argc > 1 && std::strcmp(argv[1], "fp32") == 0) g_precision = Precision::FP32;
     const char* prec_name = g_precision == Precision::FP64 ? "fp64" : "fp32";
     int ran = 0;
