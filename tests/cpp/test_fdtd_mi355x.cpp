@@ -468,6 +468,45 @@ void test_subclass_hooks() {
                 b[4] == w[2] && b[5] == p.Nk - w[2]);
 }
 
+void test_device_access() {
+    // device_field(): typed access checks the element size; once device
+    // addresses are out, update_fields() runs at once (no deferral) and
+    // host-mirror J writes still reach the steps, so the results equal a
+    // class that never handed out an address.
+    Parameters p{40, 24, 16, 0, 1, 0, 1, 0, 1, 1.0, 1.0, 1.0};
+    const FP dt = 0.4 / (FDTD_const::C * std::sqrt(3.0));
+    FDTD a(p, dt, g_precision), b(p, dt, g_precision);
+    bool thrown = false;
+    try {
+        if (g_precision == Precision::FP64) a.device_field_as<float>(Component::JX);
+        else a.device_field_as<double>(Component::JX);
+    } catch (const std::runtime_error&) {
+        thrown = true;
+    }
+    EXPECT_TRUE(thrown);
+    int eb = 0;
+    EXPECT_TRUE(a.device_field(Component::JZ, &eb) != nullptr);
+    EXPECT_TRUE(eb == (g_precision == Precision::FP64 ? 8 : 4));
+    EXPECT_TRUE(a.stream() != nullptr);
+    const int centre = 20 + 12 * 40 + 8 * 40 * 24;
+    for (int t = 0; t < 6; ++t) {
+        for (FDTD* g : {&a, &b}) g->get_field(Component::JZ)[centre] = std::sin(0.4 * (t + 1));
+        a.update_fields();
+        b.update_fields();
+        EXPECT_TRUE(a.pending_steps() == 0);
+    }
+    a.zeroed_currents();
+    b.zeroed_currents();
+    a.update_fields(3);
+    b.update_fields(3);
+    bool same = true;
+    for (int c = 0; c < 9; ++c)
+        for (int i = 0; i < (int)a.cells(); ++i)
+            same = same && (double)a.get_field(static_cast<Component>(c))[i] ==
+                               (double)b.get_field(static_cast<Component>(c))[i];
+    EXPECT_TRUE(same);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -491,7 +530,8 @@ int main(int argc, char** argv) {
     test_deferred_steps();
     test_probes_and_energy();
     test_subclass_hooks();
-    ran += 8;
+    test_device_access();
+    ran += 9;
     std::printf("%d tests, %d failures\n", ran, g_failures);
     return g_failures == 0 ? 0 : 1;
 }

@@ -48,7 +48,7 @@ def test_cpp_unit_tests(prec):
     res = subprocess.run([exe, prec], capture_output=True, text=True, timeout=300)
     print(res.stdout[-3000:])
     assert res.returncode == 0, res.stdout[-3000:] + res.stderr[-2000:]
-    assert "20 tests, 0 failures" in res.stdout
+    assert "21 tests, 0 failures" in res.stdout
 
 
 def test_cpp_sample_driver_matches_reference(tmp_path, golden_dir):

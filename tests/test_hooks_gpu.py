@@ -130,7 +130,7 @@ def _torch():
 
 
 @pytest.mark.torch
-@pytest.mark.parametrize("prec", [Precision.FP64, Precision.FP32])
+@pytest.mark.parametrize("prec", [Precision.FP64, Precision.FP32, Precision.FP16])
 def test_device_currents_equal_uploaded(prec):
     """J written through torch on the grid's stream before every step equals
     the same J uploaded from the host, bit for bit; zeroed_currents() clears
@@ -140,6 +140,9 @@ def test_device_currents_equal_uploaded(prec):
     n = int(np.prod(shape))
     a, _ = grid_pair(shape, seed=21, prec=prec)
     b, _ = grid_pair(shape, seed=21, prec=prec)
+    if prec == Precision.FP16:
+        for g in (a, b):
+            g.set_scales([2.0 ** -2] * 6)
     jd = [torch.as_tensor(a.device_field(6 + c), device="cuda") for c in range(3)]
     jt = torch.float64 if prec == Precision.FP64 else torch.float32
     assert all(t.dtype == jt and t.numel() == n for t in jd)
@@ -151,7 +154,7 @@ def test_device_currents_equal_uploaded(prec):
             for c in range(3):
                 jd[c].copy_(torch.from_numpy(vals[c]).to(device="cuda", dtype=jt))
         for c in range(3):
-            # FP32 grids hold J in float: the host upload rounds the same way
+            # FP32/FP16 grids hold J in float: the host upload rounds the same way
             b.upload(6 + c, vals[c])
         a.update_fields(1)
         b.update_fields(1)
