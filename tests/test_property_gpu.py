@@ -5,6 +5,8 @@ blockings (strip passes) bit-identical to unblocked steps in every
 precision.  Complements the hand-picked cases of test_parity_gpu.py /
 test_pml_gpu.py / test_strip_gpu.py with shapes nobody chose (ragged tiles,
 chunk boundaries on the last plane, one-cell axes)."""
+import os
+
 import numpy as np
 import pytest
 from hypothesis import HealthCheck, given, settings, strategies as st
@@ -15,8 +17,13 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-SETTINGS = settings(max_examples=40, deadline=None, derandomize=True,
-                    suppress_health_check=[HealthCheck.too_slow])
+# FDTD_PROPERTY_EXAMPLES=N runs N cases per property; FDTD_PROPERTY_RANDOM=1
+# draws new random cases on every run instead of the fixed derandomized set.
+SETTINGS = settings(max_examples=int(os.environ.get("FDTD_PROPERTY_EXAMPLES", "40")),
+                    deadline=None,
+                    derandomize=os.environ.get("FDTD_PROPERTY_RANDOM", "0") != "1",
+                    database=None,
+                    suppress_health_check=[HealthCheck.too_slow, HealthCheck.filter_too_much])
 
 
 @st.composite
