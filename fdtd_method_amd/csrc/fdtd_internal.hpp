@@ -123,6 +123,9 @@ struct fdtd_grid {
     std::vector<long long> jtab_keys;
     double* d_jtab = nullptr;
     size_t d_jtab_cap = 0;       // doubles
+    // single-cell writes into dense J (fdtd_set_current_values)
+    fdtd::CurrentWrite* d_jw = nullptr;
+    size_t d_jw_cap = 0;         // writes
     long long jtab_step0 = 0;
     int jtab_rows = 0;
     // launches so far: {steps, blocked passes, single-step sweeps, B/2 prologues}

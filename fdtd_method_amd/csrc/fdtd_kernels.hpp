@@ -422,6 +422,25 @@ plain_update_E(Fields<S> f, const JT* __restrict__ jx, const JT* __restrict__ jy
     plain_E_cell<S, C, JT>(f, jx, jy, jz, src, vals, nsrc, c, Ni, Nj, Nk, i, j, k);
 }
 
+// Single-cell current writes into dense J arrays: write q stores value q at
+// cell index q of component comp (0..2).  The host passes each cell once.
+struct CurrentWrite {
+    long long index;
+    int comp, pad;
+    double value;
+};
+
+template <typename JT>
+__global__ void __launch_bounds__(256)
+scatter_currents(JT* __restrict__ jx, JT* __restrict__ jy, JT* __restrict__ jz,
+                 const CurrentWrite* __restrict__ w, int n) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const CurrentWrite x = w[q];
+    JT* j = x.comp == 0 ? jx : (x.comp == 1 ? jy : jz);
+    j[x.index] = static_cast<JT>(x.value);
+}
+
 // The plain updates over the box [lo, hi) of cells (the reference's
 // begin_main_* .. end_main_* loop bounds, src/FDTD/FDTD.cpp:63-130,
 // src/FDTD_kokkos/FDTD_PML_kokkos.cpp:160-198); neighbours still wrap over

@@ -1676,6 +1676,7 @@ int fdtd_destroy(fdtd_grid* g) {
         if (g->J[d]) (void)hipFree(g->J[d]);
     if (g->staging) (void)hipFree(g->staging);
     if (g->d_jtab) (void)hipFree(g->d_jtab);
+    if (g->d_jw) (void)hipFree(g->d_jw);
     for (int b = 0; b < 2; ++b) {
         if (g->d_src_buf[b]) (void)hipFree(g->d_src_buf[b]);
         if (g->h_src[b]) (void)hipHostFree(g->h_src[b]);
@@ -1968,20 +1969,40 @@ int fdtd_set_current_values(fdtd_grid* g, int n, const long long* index, const i
             return fail(FDTD_ERR_INVALID_ARGUMENT, "index %lld out of range", index[q]);
     }
     if (g->j_dense_active) {
-        // scatter into the dense arrays (host round trip of single values)
-        for (int q = 0; q < n; ++q) {
-            const int d = comp[q] - FDTD_JX;
-            if (g->precision == FDTD_FP64) {
-                HIP_TRY(hipMemcpyAsync(static_cast<double*>(g->J[d]) + index[q], &value[q], 8,
-                                       hipMemcpyHostToDevice, g->stream));
-            } else {
-                const float v = (float)value[q];
-                HIP_TRY(hipMemcpyAsync(static_cast<float*>(g->J[d]) + index[q], &v, 4,
-                                       hipMemcpyHostToDevice, g->stream));
-                HIP_TRY(hipStreamSynchronize(g->stream));
-            }
+        // scatter into the dense arrays: one upload and one kernel, each
+        // cell once with the last value written to it (the order of the
+        // writes, as the reference's element assignments)
+        if (n == 0) return FDTD_OK;
+        std::map<long long, double> last;
+        for (int q = 0; q < n; ++q) last[index[q] * 3 + (comp[q] - FDTD_JX)] = value[q];
+        std::vector<fdtd::CurrentWrite> w;
+        w.reserve(last.size());
+        for (const auto& kv : last)
+            w.push_back({kv.first / 3, static_cast<int>(kv.first % 3), 0, kv.second});
+        if (w.size() > g->d_jw_cap) {
+            HIP_TRY(hipStreamSynchronize(g->stream));
+            if (g->d_jw) HIP_TRY(hipFree(g->d_jw));
+            g->d_jw = nullptr;
+            g->d_jw_cap = 0;
+            const size_t cap = std::max<size_t>(64, 2 * w.size());
+            HIP_TRY(hipMalloc(&g->d_jw, cap * sizeof(fdtd::CurrentWrite)));
+            g->d_jw_cap = cap;
         }
-        HIP_TRY(hipStreamSynchronize(g->stream));
+        HIP_TRY(hipMemcpyAsync(g->d_jw, w.data(), w.size() * sizeof(fdtd::CurrentWrite),
+                               hipMemcpyHostToDevice, g->stream));
+        const int nw = static_cast<int>(w.size());
+        if (g->precision == FDTD_FP64)
+            hipLaunchKernelGGL(fdtd::scatter_currents<double>, dim3((nw + 255) / 256), dim3(256), 0,
+                               g->stream, static_cast<double*>(g->J[0]),
+                               static_cast<double*>(g->J[1]), static_cast<double*>(g->J[2]),
+                               g->d_jw, nw);
+        else
+            hipLaunchKernelGGL(fdtd::scatter_currents<float>, dim3((nw + 255) / 256), dim3(256), 0,
+                               g->stream, static_cast<float*>(g->J[0]),
+                               static_cast<float*>(g->J[1]), static_cast<float*>(g->J[2]),
+                               g->d_jw, nw);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(g->stream));  // w is released on return
         return FDTD_OK;
     }
     for (int q = 0; q < n; ++q)

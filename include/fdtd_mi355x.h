@@ -193,7 +193,9 @@ int fdtd_update_e_box(fdtd_grid* grid, const int lo[3], const int hi[3]);
 
 /* Sparse form of get_field(JX|JY|JZ)[index] = value (the drivers'
  * per-step box source, perf-tests/sample/sample.cpp:67-81): sets J at n
- * cells; values persist until changed or zeroed like the reference's J. */
+ * cells; values persist until changed or zeroed like the reference's J.
+ * Writes are applied in order (a cell written twice keeps the last value);
+ * into dense J they go as one upload and one scatter kernel per call. */
 int fdtd_set_current_values(fdtd_grid* grid, int n, const long long* index,
                             const int* component, const double* value);
 

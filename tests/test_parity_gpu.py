@@ -274,6 +274,28 @@ def test_dense_and_sparse_currents_bit_exact():
         assert np.array_equal(out[c], r2.fields()[c])
     for c in range(3):
         assert np.array_equal(g2.download(6 + c), J[c])
+    # writes into dense J: one call with repeated cells (the last value
+    # wins, as in element-by-element assignment), in FP64 and FP32
+    for prec in (Precision.FP64, Precision.FP32):
+        g3 = gpu_grid(shape, spacing, dt, st[:6], prec)
+        g3.set_current_values(cells, comps + 6, vals)
+        rep = np.concatenate([cells[:40], cells[:40], cells[100:120]])
+        rcomp = np.concatenate([comps[:40], comps[:40], comps[100:120]])
+        rval = rng.standard_normal(rep.size)
+        g3.set_current_values(rep, rcomp + 6, rval)
+        J3 = [a.copy() for a in J]
+        for q in range(rep.size):
+            J3[rcomp[q]][rep[q]] = rval[q]
+        for c in range(3):
+            want = J3[c] if prec == Precision.FP64 else J3[c].astype(np.float32).astype(np.float64)
+            assert np.array_equal(g3.download(6 + c), want), (prec, c)
+        if prec == Precision.FP64:
+            r3 = oracle_grid(shape, spacing, dt, st[:6], J3)
+            r3.update_fields(3)
+            g3.update_fields(3)
+            out = g3.fields()
+            for c in range(6):
+                assert np.array_equal(out[c], r3.fields()[c])
 
 
 def test_currents_persist_and_zero():
