@@ -356,7 +356,9 @@ class FDTD:
         mirrors are refreshed after every stepping or synchronizing call.
         torch brings its own HIP runtime: import torch and touch
         ``torch.cuda`` before the first grid is created (started after this
-        library has launched work, it reports no device)."""
+        library has launched work, it reports no device).  ``close()``
+        frees the arrays whatever aliases them: tensors made from a
+        DeviceArray must not be used after it."""
         try:
             comp = Component(component)
         except ValueError:

@@ -219,7 +219,9 @@ public:
 
     // Device array of a component (see the header comment and
     // fdtd_device_field): J in double (FP64 grids) or float, at a fixed
-    // address; E/B in the storage type, valid until the next step.
+    // address; E/B in the storage type, valid until the next step.  The
+    // arrays belong to the object (a Kokkos View's reference count does not
+    // exist here): no pointer outlives it.
     void* device_field(Component c, int* elem_bytes = nullptr, double* scale = nullptr) {
         const int id = static_cast<int>(c);
         if (id < 0 || id > 8) throw std::logic_error("ERROR: Invalid field component");
