@@ -53,6 +53,15 @@ def test_null_arguments_are_rejected():
     assert L.fdtd_create(None, 1.0, 0, None) == _lib.ERR_INVALID_ARGUMENT
     assert L.fdtd_update_fields(None, 1) == _lib.ERR_INVALID_ARGUMENT
     assert L.fdtd_destroy(None) == _lib.OK
+    # device arrays, the stream and the protected box updates
+    ptr, eb, sc = ctypes.c_void_p(), ctypes.c_int(), ctypes.c_double()
+    assert L.fdtd_device_field(None, 0, ctypes.byref(ptr), ctypes.byref(eb),
+                               ctypes.byref(sc)) == _lib.ERR_INVALID_ARGUMENT
+    assert L.fdtd_get_stream(None, ctypes.byref(ptr)) == _lib.ERR_INVALID_ARGUMENT
+    box = (ctypes.c_int * 3)(0, 0, 0)
+    assert L.fdtd_update_b_box(None, box, box) == _lib.ERR_INVALID_ARGUMENT
+    assert L.fdtd_update_e_box(None, box, box) == _lib.ERR_INVALID_ARGUMENT
+    assert b"null" in L.fdtd_last_error()
 
 
 def test_cpp_programs_are_built():
